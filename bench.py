@@ -1,0 +1,239 @@
+"""Benchmark: DOF-stage-updates/s of the fused tet DG Burgers stage on one MI355X.
+
+Workload (default): 96^3 hexes x 6 = 5,308,416 tets on the periodic unit
+cube, P=3 (Np=20), FP64 state, u0 = 0.5 + 0.4 sin(2 pi x) cos(2 pi y)
+sin(2 pi z), 64 contiguous x-slab partitions with seeded +-40% imbalance,
+strong-form Lax-Friedrichs flux + LIFT, SSP-RK3 with dt = 0.3 hmin/max|u|
+recomputed on the device every step. One persistent launch per stage covers
+all partitions.
+
+A step of the timed region is one SSP-RK3 step (three fused stage launches
+plus the one-thread dt kernel); inputs are resident in HBM before timing
+starts. Each stage launch is bracketed by HIP events on the library's
+stream, which gives the per-launch duration used for the roofline figure.
+
+Prints one JSON line. ``--gpus`` above 1 is rejected: this build runs on a
+single GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_bytes_per_elem(Np: int) -> float:
+    """Algorithmic HBM bytes per element per SSP-RK3 stage, FP64 state, affine tets:
+    (8/3)*8*Np state (2, 3, 3 arrays over the three stages) + 25 geometry doubles
+    + 4 neighbour ids + 4 orientation codes (int32)."""
+    return (8.0 / 3.0) * 8.0 * Np + 8.0 * 25 + 32.0
+
+
+def stage_flops_per_elem(Np: int, Nfp: int) -> float:
+    return 6.0 * Np * Np + 8.0 * Np * Nfp + 12.0 * Np + 72.0 * Nfp
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch recorded from rocprofv3 --pmc runs (profiles/<name>)."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip().split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def build_c3(n: int, N: int, nparts: int, seed: int):
+    from dgkernel_amd import dg
+    from dgkernel_amd import mesh as M
+    m = M.cube_mesh(n, n, n, periodic=True)
+    Kp = M.slab_partition(m.K, nparts, 0.4, seed=seed) if nparts > 1 else None
+    st = dg.make_setup(m, N, Kp=Kp)
+    x, y, z = st.node_coords()
+    u0 = 0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+    del x, y, z
+    return m, st, u0
+
+
+def cpu_baseline(N: int, n: int, steps: int, threads: int | None):
+    """The CPU oracle (C++ restatement, OpenMP element-parallel) on a bounded sample."""
+    from oracle import oracle as O
+    m, st, u0 = build_c3(n, N, 1, 0)
+    if threads:
+        O.set_threads(threads)
+    o = O.Setup.from_mesh(N, m)
+    hmin = o.min_node_distance()
+    t0 = time.perf_counter()
+    o.ssprk3(u0, steps, 0.3, hmin)
+    dt = time.perf_counter() - t0
+    dof_stage = m.K * st.ref.Np * 3 * steps
+    return {"value": dof_stage / dt, "unit": "DOF-stage-updates/s", "cores": O.num_threads(),
+            "kind": "port",
+            "sample": f"{n}^3 hexes x 6 = {m.K} tets, P={N}, {steps} SSP-RK3 steps "
+                      f"({3 * steps} stages) of the same periodic Burgers problem; "
+                      f"C++ oracle, OpenMP element-parallel, {dt:.2f} s",
+            "cpu": _cpu_model()}
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=96, help="hexes per side")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--parts", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--tile", type=int, default=0, help="elements per block (0 = tuned)")
+    ap.add_argument("--bpc", type=int, default=0, help="blocks per CU (0 = tuned)")
+    ap.add_argument("--cpu-n", type=int, default=48, help="hexes per side of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    a = ap.parse_args(argv)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}, WORLD_SIZE={world}); "
+              "multi-GPU sharding is described in DESIGN.md but not built.", file=sys.stderr)
+        return 2
+
+    from dgkernel_amd import _lib, dg
+
+    t_setup = time.perf_counter()
+    m, st, u0 = build_c3(a.n, a.order, a.parts, a.seed)
+    log(f"host setup {time.perf_counter() - t_setup:.1f} s, K={m.K}, parts={st.npart}")
+    dev = _lib.Device()
+    solver = dg.BurgersSolver(dev, st, cfl=0.3)
+    if a.tile or a.bpc:
+        solver.mesh.set_tile(a.tile or 64, a.bpc)
+    solver.set_state(u0)
+    del u0
+    dev.sync()
+    log(f"device setup done {time.perf_counter() - t_setup:.1f} s, hmin={solver.hmin:.6g}")
+    K, Np, Nfp = m.K, st.ref.Np, st.ref.Nfp
+
+    solver.step(a.warmup, graph=False)
+    dev.sync()
+
+    # Timed region: eager launches, one event pair per stage launch.
+    timers = [dev.timer() for _ in range(3 * a.steps)]
+    region = dev.timer()
+    import ctypes as C
+    L = _lib
+    msh = solver.mesh.h
+    dev.sync()
+    region.start()
+    for s in range(a.steps):
+        L.call("dgk_cfl_dt", dev.h, solver.maxabs_ptr, solver.cfl, solver.hmin, solver.dt_ptr)
+        tm = timers[3 * s:3 * s + 3]
+        tm[0].start()
+        L.call("dgk_burgers_stage", msh, solver.u.ptr, None, solver.u1.ptr, 0.0, 1.0, solver.dt_ptr,
+               None)
+        tm[0].stop()
+        tm[1].start()
+        L.call("dgk_burgers_stage", msh, solver.u1.ptr, solver.u.ptr, solver.u2.ptr, 0.75, 0.25,
+               solver.dt_ptr, None)
+        tm[1].stop()
+        tm[2].start()
+        L.call("dgk_burgers_stage", msh, solver.u2.ptr, solver.u.ptr, solver.u.ptr, 1.0 / 3.0,
+               2.0 / 3.0, solver.dt_ptr, solver.maxabs_ptr)
+        tm[2].stop()
+    region.stop()
+    dev.sync()
+    total_ms = region.ms()
+    stage_ms = np.array([t.ms() for t in timers])
+    log(f"timed {a.steps} steps: {total_ms:.3f} ms, stage mean {stage_ms.mean():.4f} ms "
+        f"(min {stage_ms.min():.4f}, max {stage_ms.max():.4f})")
+
+    graph_ms = None
+    if not a.no_graph:
+        solver.step(1, graph=True)
+        dev.sync()
+        g = dev.timer()
+        g.start()
+        solver.step(a.steps, graph=True)
+        g.stop()
+        graph_ms = g.ms()
+        log(f"graph replay {a.steps} steps: {graph_ms:.3f} ms")
+
+    u_end = solver.state()
+    finite = bool(np.isfinite(u_end).all())
+    dof_stage = K * Np * 3 * a.steps
+    value = dof_stage / (total_ms * 1e-3)
+    bpe = stage_bytes_per_elem(Np)
+    achieved = bpe * K / (stage_ms.mean() * 1e-3) / 1e9
+    flops = stage_flops_per_elem(Np, Nfp) * K / (stage_ms.mean() * 1e-3) / 1e12
+    traffic = read_traffic(f"burgers_stage_p{a.order}_{a.n}cube_pmc.md")
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(a.order, a.cpu_n, a.cpu_steps, a.cpu_threads or None)
+            log(f"cpu baseline {cpu['value']:.3e} DOF-stage/s on {cpu['cores']} threads")
+        except Exception as e:  # the oracle library may be absent on a bare install
+            log(f"cpu baseline skipped: {e}")
+
+    out = {
+        "metric": "DOF-stage-updates/s at P=3 and P=6, % HBM/MFMA roofline, L2 err vs FP64",
+        "value": value,
+        "unit": "DOF-stage-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": total_ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (smooth initial field on a generated periodic mesh)",
+        "config": {"workload": f"batched Burgers SSP-RK3 stages, {a.n}^3x6={K} tets, P={a.order}, "
+                               f"{st.npart} slab partitions (+-40% imbalance), one persistent launch",
+                   "tets": K, "order": a.order, "Np": Np, "partitions": st.npart,
+                   "kpart_max": int(st.Kp.max()), "stages_timed": 3 * a.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "burgers_stage_kernel", "bytes_per_elem": bpe,
+                     "stage_ms_mean": float(stage_ms.mean()),
+                     "fp64_tflops": flops, "fp64_frac": flops / FP64_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+        "graph_ms_per_step": None if graph_ms is None else graph_ms / a.steps,
+        "graph_value": None if graph_ms is None else dof_stage / (graph_ms * 1e-3),
+        "state_finite": finite,
+        "device": dev.name,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

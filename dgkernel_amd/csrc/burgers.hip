@@ -1,0 +1,644 @@
+// Tetrahedral DG Burgers stage on gfx950: device mesh, FP64 affine geometry,
+// the fused strong-form RHS + SSP-RK3 stage, max|u| and the CFL step.
+//
+// The reference has only a volume-only, non-conservative Burgers RHS
+// (examples/3d_scalar_burgers_equation.go:295-340) and sketches LF + LIFT in
+// documentation/DGKernel/example_application.md:296-353. This file
+// implements the strong form of Hesthaven & Warburton (2008) §2.4/§6:
+//   L(u) = -div f(u) + LIFT (Fscale (n.f(u-) - n.f*)),  f(u) = u^2/2 (1,1,1),
+//   n.f* = (n.f(u-) + n.f(u+))/2 - alpha |n.(1,1,1)| (u+ - u-)/2,
+//   alpha = max(|u-|, |u+|),
+// and one SSP-RK3 (Shu & Osher 1988) stage out = a0 u0 + a1 (u + dt L(u)).
+//
+// Design (one element per thread, one tile of TB elements per block):
+//  * the tile's nodal values are contiguous in the runner layout
+//    (X_offsets[p] + l*Np), so they are staged through LDS with unit-stride
+//    loads and each thread then holds its element's Np values in VGPRs;
+//  * Dr, Ds, Dt and LIFT are read with wave-uniform addresses (scalar loads,
+//    SGPR operands of v_fma_f64), so the O(Np^2) contractions use no LDS;
+//  * affine geometry is folded per element at setup into 11 FP64 numbers
+//    (rx+ry+rz, sx+sy+sz, tx+ty+tz, and per face n.(1,1,1) and Fscale);
+//  * the neighbour trace u+ is gathered straight from the neighbour's nodes
+//    (value offset + orientation code -> node table in LDS); no trace arrays
+//    are written between stages;
+//  * the RK combination is written back through LDS with unit stride;
+//  * one persistent launch walks a flat list of tiles covering every
+//    partition, so uneven partition sizes cost no padding.
+#include <cmath>
+
+#include "dgk_internal.h"
+
+namespace {
+
+template <int N>
+struct Tet {
+  static constexpr int Np = (N + 1) * (N + 2) * (N + 3) / 6;
+  static constexpr int Nfp = (N + 1) * (N + 2) / 2;
+};
+
+// Face masks of the NUDG node ordering (t=-1, s=-1, r+s+t=-1, r=-1), derived
+// from the lattice indices of EquiNodes3D (element/library/gonudg/nodes3d.go:
+// 191-201); the warp & blend map keeps face nodes on their faces. Checked
+// against the caller's Fmask in dgk_dgmesh_create.
+template <int N>
+struct FmaskTable {
+  int v[4][Tet<N>::Nfp];
+  constexpr FmaskTable() : v{} {
+    int c[4] = {0, 0, 0, 0};
+    int sk = 0;
+    for (int n = 0; n <= N; ++n)
+      for (int m = 0; m <= N - n; ++m)
+        for (int l = 0; l <= N - n - m; ++l) {
+          if (n == 0) v[0][c[0]++] = sk;
+          if (m == 0) v[1][c[1]++] = sk;
+          if (l + m + n == N) v[2][c[2]++] = sk;
+          if (l == 0) v[3][c[3]++] = sk;
+          ++sk;
+        }
+  }
+};
+
+std::vector<int> host_fmask(int N) {
+  std::vector<int> v;
+  int Nfp = dgk::nfp_of(N);
+  v.assign(4 * Nfp, 0);
+  int c[4] = {0, 0, 0, 0}, sk = 0;
+  for (int n = 0; n <= N; ++n)
+    for (int m = 0; m <= N - n; ++m)
+      for (int l = 0; l <= N - n - m; ++l) {
+        if (n == 0) v[0 * Nfp + c[0]++] = sk;
+        if (m == 0) v[1 * Nfp + c[1]++] = sk;
+        if (l + m + n == N) v[2 * Nfp + c[2]++] = sk;
+        if (l == 0) v[3 * Nfp + c[3]++] = sk;
+        ++sk;
+      }
+  return v;
+}
+
+struct Tile {
+  int64_t vbase;  // value offset of the tile's first element in state arrays
+  int32_t gbase;  // dense global index of the tile's first element
+  int32_t count;  // elements in the tile
+};
+
+constexpr int kGeoRows = 26;   // rx..tz, J, 4 x (nx, ny, nz, Fscale)
+constexpr int kBGeoRows = 11;  // rx+ry+rz, sx+sy+sz, tx+ty+tz, 4 x n.(1,1,1), 4 x Fscale
+
+// FP64 affine geometry per element from its four vertices: the
+// GeometricFactors3D / Normals3D / ComputeFscale formulas
+// (element/library/gonudg/normals3d.go:11-207) for a map whose Jacobian is
+// constant: xr = (vb - va)/2, xs = (vc - va)/2, xt = (vd - va)/2.
+__global__ void geometry_kernel(int64_t K, const double* __restrict__ vtx,
+                                double* __restrict__ geo, double* __restrict__ bgeo,
+                                int* __restrict__ bad) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const double* v = vtx + k * 12;
+  double xr = 0.5 * (v[3] - v[0]), yr = 0.5 * (v[4] - v[1]), zr = 0.5 * (v[5] - v[2]);
+  double xs = 0.5 * (v[6] - v[0]), ys = 0.5 * (v[7] - v[1]), zs = 0.5 * (v[8] - v[2]);
+  double xt = 0.5 * (v[9] - v[0]), yt = 0.5 * (v[10] - v[1]), zt = 0.5 * (v[11] - v[2]);
+  double J = xr * (ys * zt - zs * yt) - yr * (xs * zt - zs * xt) + zr * (xs * yt - ys * xt);
+  if (!(J > 0.0)) atomicAdd(bad, 1);
+  double rx = (ys * zt - zs * yt) / J, ry = -(xs * zt - zs * xt) / J, rz = (xs * yt - ys * xt) / J;
+  double sx = -(yr * zt - zr * yt) / J, sy = (xr * zt - zr * xt) / J, sz = -(xr * yt - yr * xt) / J;
+  double tx = (yr * zs - zr * ys) / J, ty = -(xr * zs - zr * xs) / J, tz = (xr * ys - yr * xs) / J;
+  double m[9] = {rx, ry, rz, sx, sy, sz, tx, ty, tz};
+  for (int q = 0; q < 9; ++q) geo[q * K + k] = m[q];
+  geo[9 * K + k] = J;
+  double nv[4][3] = {{-tx, -ty, -tz}, {-sx, -sy, -sz}, {rx + sx + tx, ry + sy + ty, rz + sz + tz},
+                     {-rx, -ry, -rz}};
+  bgeo[0 * K + k] = rx + ry + rz;
+  bgeo[1 * K + k] = sx + sy + sz;
+  bgeo[2 * K + k] = tx + ty + tz;
+  for (int f = 0; f < 4; ++f) {
+    double mag = sqrt(nv[f][0] * nv[f][0] + nv[f][1] * nv[f][1] + nv[f][2] * nv[f][2]);
+    double nx = nv[f][0] / mag, ny = nv[f][1] / mag, nz = nv[f][2] / mag;
+    double sJ = mag * J;
+    double fs = sJ / J;
+    geo[(10 + 4 * f) * K + k] = nx;
+    geo[(11 + 4 * f) * K + k] = ny;
+    geo[(12 + 4 * f) * K + k] = nz;
+    geo[(13 + 4 * f) * K + k] = fs;
+    bgeo[(3 + f) * K + k] = nx + ny + nz;
+    bgeo[(7 + f) * K + k] = fs;
+  }
+}
+
+struct StageParams {
+  const double* Dr;
+  const double* Ds;
+  const double* Dt;
+  const double* LIFT;
+  const uint8_t* face_nodes;  // [4][24][Nfp]
+  const double* bgeo;         // [11][K]
+  const int32_t* nbr;         // [4][K] value offset of each neighbour's node 0
+  const int32_t* codes;       // [K] four 5-bit orientation codes
+  const Tile* tiles;
+  int ntiles;
+  int chunked;                // 1: each block takes a contiguous run of tiles
+  int64_t K;
+  const double* u;
+  const double* u0;
+  double* out;
+  double a0, a1;
+  const double* dt;
+  unsigned long long* maxabs;
+  int mode;                   // 0 stage, 1 rhs only (out = L(u)), 2 trace u+ (out = uP)
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Keeps a pointer opaque to the optimizer so that loads through it are not
+// hoisted out of the tile loop (which would pin the whole operator in
+// registers and spill); the value stays in SGPRs.
+template <typename T>
+__device__ __forceinline__ const T* launder_sgpr(const T* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+template <int N, int TB>
+__global__ void __launch_bounds__(TB) burgers_stage_kernel(StageParams p) {
+  constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP;
+  constexpr int S = NP + 1;  // padded LDS row: odd stride in doubles spreads banks
+  constexpr FmaskTable<N> FM{};
+  __shared__ double sU[TB * S];
+  __shared__ double sV[TB * S];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += TB) sFN[i] = p.face_nodes[i];
+  const double dt = p.mode == 0 ? *p.dt : 0.0;
+  const int64_t K = p.K;
+  double lmax = 0.0;
+  int t0, t1, ts;
+  if (p.chunked) {
+    // XCD-aware: consecutive hardware blocks land on different XCDs
+    // (round robin over 8), so renumber blocks so that each XCD owns a
+    // contiguous range of tiles and its L2 sees the neighbour data of
+    // nearby tiles.
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int vb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    t0 = (int)((int64_t)p.ntiles * vb / G);
+    t1 = (int)((int64_t)p.ntiles * (vb + 1) / G);
+    ts = 1;
+  } else {
+    t0 = blockIdx.x;
+    t1 = p.ntiles;
+    ts = gridDim.x;
+  }
+  for (int t = t0; t < t1; t += ts) {
+    const Tile tile = p.tiles[t];
+    const int cnt = tile.count;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt * NP; i += TB) {
+      int e = i / NP, n = i - e * NP;
+      sU[e * S + n] = p.u[tile.vbase + i];
+    }
+    __syncthreads();
+    const int e = threadIdx.x;
+    if (e < cnt) {
+      const int64_t g = tile.gbase + e;
+      const double* su = sU + e * S;
+      const int packed = p.codes[g];
+      int32_t nb[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) nb[f] = p.nbr[f * K + g];
+      if (p.mode == 2) {
+        // Trace of the neighbour, as the stage sees it.
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const int code = (packed >> (5 * f)) & 31;
+#pragma unroll
+          for (int fp = 0; fp < NFP; ++fp)
+            p.out[g * NF + f * NFP + fp] = p.u[nb[f] + sFN[(f * 24 + code) * NFP + fp]];
+        }
+      } else {
+        const double* Dr = launder_sgpr(p.Dr);
+        const double* Ds = launder_sgpr(p.Ds);
+        const double* Dt = launder_sgpr(p.Dt);
+        const double* LIFT = launder_sgpr(p.LIFT);
+        double v[NP];
+        {
+          const double ga = p.bgeo[0 * K + g], gb = p.bgeo[1 * K + g], gc = p.bgeo[2 * K + g];
+          double F[NP];
+#pragma unroll
+          for (int n = 0; n < NP; ++n) F[n] = 0.5 * su[n] * su[n];
+#pragma unroll
+          for (int i = 0; i < NP; ++i) {
+            double r = 0.0, s = 0.0, q = 0.0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+              r = fma(Dr[i * NP + j], F[j], r);
+              s = fma(Ds[i * NP + j], F[j], s);
+              q = fma(Dt[i * NP + j], F[j], q);
+            }
+            v[i] = -(ga * r + gb * s + gc * q);
+          }
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const int code = (packed >> (5 * f)) & 31;
+          const double sn = p.bgeo[(3 + f) * K + g];
+          const double fs = p.bgeo[(7 + f) * K + g];
+          const double asn = fabs(sn);
+          double gf[NFP];
+#pragma unroll
+          for (int fp = 0; fp < NFP; ++fp) {
+            const double uM = su[FM.v[f][fp]];
+            const double uP = p.u[nb[f] + sFN[(f * 24 + code) * NFP + fp]];
+            const double fM = 0.5 * uM * uM, fP = 0.5 * uP * uP;
+            const double al = fmax(fabs(uM), fabs(uP));
+            gf[fp] = fs * (0.5 * (fM - fP) * sn + 0.5 * al * asn * (uP - uM));
+          }
+#pragma unroll
+          for (int i = 0; i < NP; ++i) {
+            double acc = v[i];
+#pragma unroll
+            for (int fp = 0; fp < NFP; ++fp) acc = fma(LIFT[i * NF + f * NFP + fp], gf[fp], acc);
+            v[i] = acc;
+          }
+        }
+        double* sv = sV + e * S;
+#pragma unroll
+        for (int n = 0; n < NP; ++n) sv[n] = v[n];
+      }
+    }
+    if (p.mode == 2) continue;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt * NP; i += TB) {
+      int ee = i / NP, n = i - ee * NP;
+      double o = sV[ee * S + n];
+      if (p.mode == 0) {
+        o = p.a1 * fma(dt, o, sU[ee * S + n]);
+        if (p.a0 != 0.0) o = fma(p.a0, p.u0[tile.vbase + i], o);
+      }
+      p.out[tile.vbase + i] = o;
+      lmax = fmax(lmax, fabs(o));
+    }
+  }
+  if (p.maxabs) {
+    lmax = wave_max(lmax);
+    if ((threadIdx.x & 63) == 0 && lmax > 0.0)
+      atomicMax(p.maxabs, (unsigned long long)__double_as_longlong(lmax));
+  }
+}
+
+__global__ void max_abs_kernel(const Tile* __restrict__ tiles, int ntiles, int np,
+                               const double* __restrict__ u, unsigned long long* maxabs) {
+  double m = 0.0;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Tile tl = tiles[t];
+    for (int i = threadIdx.x; i < tl.count * np; i += blockDim.x) m = fmax(m, fabs(u[tl.vbase + i]));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxabs, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void cfl_dt_kernel(double* maxabs, double cfl, double hmin, double* dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double m = *maxabs;
+    *dt = cfl * hmin / m;
+    *maxabs = 0.0;
+  }
+}
+
+// Minimum distance between distinct nodes of an element (node coordinates
+// by the affine map of startup3d.go:126-148), reduced over all elements.
+template <int NP>
+__global__ void min_node_distance_kernel(int64_t K, const double* __restrict__ vtx,
+                                         const double* __restrict__ rst,
+                                         unsigned long long* out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double best = INFINITY;
+  if (k < K) {
+    const double* v = vtx + k * 12;
+    double X[NP], Y[NP], Z[NP];
+    for (int i = 0; i < NP; ++i) {
+      double r = rst[i], s = rst[NP + i], t = rst[2 * NP + i];
+      double la = -(1.0 + r + s + t), lb = 1.0 + r, lc = 1.0 + s, ld = 1.0 + t;
+      X[i] = 0.5 * (la * v[0] + lb * v[3] + lc * v[6] + ld * v[9]);
+      Y[i] = 0.5 * (la * v[1] + lb * v[4] + lc * v[7] + ld * v[10]);
+      Z[i] = 0.5 * (la * v[2] + lb * v[5] + lc * v[8] + ld * v[11]);
+    }
+    for (int i = 0; i < NP; ++i)
+      for (int j = i + 1; j < NP; ++j) {
+        double dx = X[j] - X[i], dy = Y[j] - Y[i], dz = Z[j] - Z[i];
+        double len = sqrt(dx * dx + dy * dy + dz * dz);
+        if (len > 1e-12 && len < best) best = len;
+      }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) best = fmin(best, __shfl_xor(best, o, 64));
+  if ((threadIdx.x & 63) == 0 && best < INFINITY)
+    atomicMin(out, (unsigned long long)__double_as_longlong(best));
+}
+
+}  // namespace
+
+struct dgk_dgmesh_s {
+  dgk_device dev = nullptr;
+  int N = 0, Np = 0, Nfp = 0, npart = 0;
+  int64_t K = 0;
+  std::vector<int64_t> Kp, state_off;
+  double *Dr = nullptr, *Ds = nullptr, *Dt = nullptr, *LIFT = nullptr, *rst = nullptr;
+  uint8_t* face_nodes = nullptr;
+  double *vtx = nullptr, *geo = nullptr, *bgeo = nullptr;
+  int32_t *nbr = nullptr, *codes = nullptr;
+  Tile* tiles = nullptr;
+  int ntiles = 0;
+  int tb = 64, blocks_per_cu = 0, chunked = 1;
+  unsigned long long* scratch = nullptr;  // reduction slot for synchronous helpers
+};
+
+namespace {
+
+int build_tiles(dgk_dgmesh m) {
+  std::vector<Tile> tl;
+  int64_t g = 0;
+  for (int p = 0; p < m->npart; ++p) {
+    for (int64_t f = 0; f < m->Kp[p]; f += m->tb) {
+      Tile t;
+      t.vbase = m->state_off[p] + f * m->Np;
+      t.gbase = (int32_t)(g + f);
+      t.count = (int32_t)std::min<int64_t>(m->tb, m->Kp[p] - f);
+      tl.push_back(t);
+    }
+    g += m->Kp[p];
+  }
+  if (m->tiles) dgk_free(m->dev, m->tiles);
+  m->tiles = nullptr;
+  m->ntiles = (int)tl.size();
+  if (tl.empty()) return DGK_OK;
+  return dgk_malloc(m->dev, tl.size() * sizeof(Tile), tl.data(), (void**)&m->tiles);
+}
+
+template <int N, int TB>
+int launch_stage_t(dgk_dgmesh m, const StageParams& p) {
+  int grid = m->ntiles;
+  int bpc = m->blocks_per_cu > 0 ? m->blocks_per_cu : (TB == 64 ? 8 : (TB == 128 ? 4 : 2));
+  int cap = m->dev->cu_count * bpc;
+  if (grid > cap) grid = cap;
+  if (grid < 1) return DGK_OK;
+  if (p.chunked && grid >= 8) grid -= grid % 8;
+  hipLaunchKernelGGL((burgers_stage_kernel<N, TB>), dim3(grid), dim3(TB), 0, m->dev->stream, p);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+template <int N>
+int launch_stage_n(dgk_dgmesh m, const StageParams& p) {
+  switch (m->tb) {
+    case 64: return launch_stage_t<N, 64>(m, p);
+    case 128: return launch_stage_t<N, 128>(m, p);
+    case 256: return launch_stage_t<N, 256>(m, p);
+    default:
+      dgk::set_error("burgers stage: unsupported tile size %d", m->tb);
+      return DGK_ERR_INVALID;
+  }
+}
+
+int launch_stage(dgk_dgmesh m, StageParams p) {
+  p.Dr = m->Dr; p.Ds = m->Ds; p.Dt = m->Dt; p.LIFT = m->LIFT;
+  p.face_nodes = m->face_nodes;
+  p.bgeo = m->bgeo;
+  p.nbr = m->nbr;
+  p.codes = m->codes;
+  p.tiles = m->tiles;
+  p.ntiles = m->ntiles;
+  p.chunked = m->chunked;
+  p.K = m->K;
+  switch (m->N) {
+    case 1: return launch_stage_n<1>(m, p);
+    case 2: return launch_stage_n<2>(m, p);
+    case 3: return launch_stage_n<3>(m, p);
+    case 4: return launch_stage_n<4>(m, p);
+    default:
+      dgk::set_error("burgers stage: order N=%d not built (1..4)", m->N);
+      return DGK_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace
+
+extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_dgmesh* out) {
+  dgk::clear_error();
+  DGK_REQUIRE(dev && d && out, "dgk_dgmesh_create: NULL argument");
+  *out = nullptr;
+  DGK_REQUIRE(d->N >= 1 && d->N <= 6, "dgk_dgmesh_create: N=%d outside 1..6", d->N);
+  DGK_REQUIRE(d->npart >= 1, "dgk_dgmesh_create: npart must be >= 1");
+  DGK_REQUIRE(d->K && d->state_off && d->Dr && d->Ds && d->Dt && d->LIFT && d->Fmask &&
+                  d->face_nodes && d->EToE && d->codes && d->vtx && d->r && d->s && d->t,
+              "dgk_dgmesh_create: NULL field in descriptor");
+  const int N = d->N, Np = dgk::np_of(N), Nfp = dgk::nfp_of(N);
+  std::vector<int> fm = host_fmask(N);
+  for (int i = 0; i < 4 * Nfp; ++i)
+    DGK_REQUIRE(fm[i] == d->Fmask[i],
+                "dgk_dgmesh_create: Fmask differs from the NUDG lattice ordering at %d", i);
+  dgk_dgmesh m = new dgk_dgmesh_s();
+  m->dev = dev;
+  m->N = N; m->Np = Np; m->Nfp = Nfp; m->npart = d->npart;
+  m->Kp.assign(d->K, d->K + d->npart);
+  m->state_off.assign(d->state_off, d->state_off + d->npart + 1);
+  std::vector<int64_t> gstart(d->npart + 1, 0);
+  for (int p = 0; p < d->npart; ++p) {
+    if (m->Kp[p] < 0) { delete m; dgk::set_error("negative K"); return DGK_ERR_INVALID; }
+    gstart[p + 1] = gstart[p] + m->Kp[p];
+  }
+  const int64_t K = gstart[d->npart];
+  m->K = K;
+  if (K >= (1LL << 31) || m->state_off[d->npart] >= (1LL << 31)) {
+    delete m;
+    dgk::set_error("dgk_dgmesh_create: mesh too large for 32-bit neighbour offsets");
+    return DGK_ERR_UNSUPPORTED;
+  }
+  // Neighbour value offsets and packed codes, SoA.
+  std::vector<int32_t> nbr((size_t)4 * K), codes((size_t)K);
+  int p2 = 0;
+  for (int64_t g = 0; g < K; ++g) {
+    int packed = 0;
+    for (int f = 0; f < 4; ++f) {
+      int64_t g2 = d->EToE[g * 4 + f];
+      int c = d->codes[g * 4 + f];
+      if (g2 < 0 || g2 >= K || c < 0 || c >= 24) {
+        delete m;
+        dgk::set_error("dgk_dgmesh_create: bad neighbour/code at element %lld face %d", (long long)g, f);
+        return DGK_ERR_INVALID;
+      }
+      // partition of g2 (partition-major numbering)
+      if (!(g2 >= gstart[p2] && g2 < gstart[p2 + 1])) {
+        p2 = (int)(std::upper_bound(gstart.begin(), gstart.end(), g2) - gstart.begin()) - 1;
+      }
+      nbr[(size_t)f * K + g] = (int32_t)(m->state_off[p2] + (g2 - gstart[p2]) * Np);
+      packed |= c << (5 * f);
+    }
+    codes[g] = packed;
+  }
+  std::vector<uint8_t> fn((size_t)4 * 24 * Nfp);
+  for (size_t i = 0; i < fn.size(); ++i) {
+    int v = d->face_nodes[i];
+    if (v < 0 || v >= Np) {
+      delete m;
+      dgk::set_error("dgk_dgmesh_create: face node table entry %d out of range", v);
+      return DGK_ERR_INVALID;
+    }
+    fn[i] = (uint8_t)v;
+  }
+  std::vector<double> rst(3 * Np);
+  for (int i = 0; i < Np; ++i) { rst[i] = d->r[i]; rst[Np + i] = d->s[i]; rst[2 * Np + i] = d->t[i]; }
+  int rc = DGK_OK;
+  rc |= dgk_malloc(dev, (size_t)Np * Np * 8, d->Dr, (void**)&m->Dr);
+  rc |= dgk_malloc(dev, (size_t)Np * Np * 8, d->Ds, (void**)&m->Ds);
+  rc |= dgk_malloc(dev, (size_t)Np * Np * 8, d->Dt, (void**)&m->Dt);
+  rc |= dgk_malloc(dev, (size_t)Np * 4 * Nfp * 8, d->LIFT, (void**)&m->LIFT);
+  rc |= dgk_malloc(dev, rst.size() * 8, rst.data(), (void**)&m->rst);
+  rc |= dgk_malloc(dev, fn.size(), fn.data(), (void**)&m->face_nodes);
+  rc |= dgk_malloc(dev, (size_t)K * 12 * 8 + 8, d->vtx, (void**)&m->vtx);
+  rc |= dgk_malloc(dev, (size_t)K * kGeoRows * 8 + 8, nullptr, (void**)&m->geo);
+  rc |= dgk_malloc(dev, (size_t)K * kBGeoRows * 8 + 8, nullptr, (void**)&m->bgeo);
+  rc |= dgk_malloc(dev, nbr.size() * 4 + 4, nbr.data(), (void**)&m->nbr);
+  rc |= dgk_malloc(dev, codes.size() * 4 + 4, codes.data(), (void**)&m->codes);
+  rc |= dgk_malloc(dev, 64, nullptr, (void**)&m->scratch);
+  if (rc == DGK_OK) rc = build_tiles(m);
+  if (rc != DGK_OK) { dgk_dgmesh_destroy(m); return DGK_ERR_HIP; }
+  // Geometry on the device (FP64), with a count of non-positive Jacobians.
+  int* bad = (int*)m->scratch;
+  rc = dgk_fill_bytes(dev, m->scratch, 0, 64);
+  if (rc == DGK_OK && K > 0) {
+    hipLaunchKernelGGL(geometry_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, dev->stream,
+                       K, m->vtx, m->geo, m->bgeo, bad);
+    if (hipGetLastError() != hipSuccess) rc = DGK_ERR_HIP;
+  }
+  int nbad = 0;
+  if (rc == DGK_OK) rc = dgk_copy_d2h(dev, &nbad, bad, sizeof(int), 0);
+  if (rc != DGK_OK) { dgk_dgmesh_destroy(m); return rc; }
+  if (nbad) {
+    dgk_dgmesh_destroy(m);
+    dgk::set_error("dgk_dgmesh_create: %d elements with non-positive Jacobian", nbad);
+    return DGK_ERR_NUMERIC;
+  }
+  *out = m;
+  return DGK_OK;
+}
+
+extern "C" int dgk_dgmesh_destroy(dgk_dgmesh m) {
+  if (!m) return DGK_OK;
+  dgk_device dev = m->dev;
+  void* ptrs[] = {m->Dr, m->Ds, m->Dt, m->LIFT, m->rst, m->face_nodes, m->vtx, m->geo,
+                  m->bgeo, m->nbr, m->codes, m->tiles, m->scratch};
+  for (void* q : ptrs) dgk_free(dev, q);
+  delete m;
+  return DGK_OK;
+}
+
+extern "C" int dgk_dgmesh_geometry(dgk_dgmesh m, double* geo_host) {
+  DGK_REQUIRE(m && geo_host, "dgk_dgmesh_geometry: NULL argument");
+  return dgk_copy_d2h(m->dev, geo_host, m->geo, (size_t)m->K * kGeoRows * 8, 0);
+}
+
+extern "C" int dgk_dgmesh_geometry_dev(dgk_dgmesh m, const double** geo_dev) {
+  DGK_REQUIRE(m && geo_dev, "dgk_dgmesh_geometry_dev: NULL argument");
+  *geo_dev = m->geo;
+  return DGK_OK;
+}
+
+extern "C" int dgk_dgmesh_min_node_distance(dgk_dgmesh m, double* hmin) {
+  dgk::clear_error();
+  DGK_REQUIRE(m && hmin, "dgk_dgmesh_min_node_distance: NULL argument");
+  unsigned long long init = (unsigned long long)0x7FF0000000000000ULL;  // +inf
+  int rc = dgk_copy_h2d(m->dev, m->scratch, &init, 8, 0);
+  if (rc) return rc;
+  dim3 grid((unsigned)((m->K + 255) / 256)), block(256);
+  switch (m->N) {
+#define DGK_CASE(n)                                                                              \
+  case n:                                                                                        \
+    hipLaunchKernelGGL(min_node_distance_kernel<(n + 1) * (n + 2) * (n + 3) / 6>, grid, block, 0, \
+                       m->dev->stream, m->K, m->vtx, m->rst, m->scratch);                        \
+    break;
+    DGK_CASE(1) DGK_CASE(2) DGK_CASE(3) DGK_CASE(4) DGK_CASE(5) DGK_CASE(6)
+#undef DGK_CASE
+    default: return DGK_ERR_UNSUPPORTED;
+  }
+  DGK_LAUNCH_CHECK();
+  return dgk_copy_d2h(m->dev, hmin, m->scratch, 8, 0);
+}
+
+extern "C" int dgk_dgmesh_trace_plus(dgk_dgmesh m, const double* u_dev, double* uP_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(m && u_dev && uP_dev, "dgk_dgmesh_trace_plus: NULL argument");
+  StageParams p{};
+  p.u = u_dev;
+  p.u0 = u_dev;
+  p.out = uP_dev;
+  p.mode = 2;
+  return launch_stage(m, p);
+}
+
+extern "C" int dgk_burgers_stage(dgk_dgmesh m, const double* u_dev, const double* u0_dev,
+                                 double* out_dev, double a0, double a1, const double* dt_dev,
+                                 double* maxabs_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(m && u_dev && out_dev && dt_dev, "dgk_burgers_stage: NULL argument");
+  DGK_REQUIRE(out_dev != u_dev, "dgk_burgers_stage: out must not alias u");
+  DGK_REQUIRE(a0 == 0.0 || u0_dev, "dgk_burgers_stage: u0 is NULL with a0 != 0");
+  StageParams p{};
+  p.u = u_dev;
+  p.u0 = u0_dev ? u0_dev : u_dev;
+  p.out = out_dev;
+  p.a0 = a0;
+  p.a1 = a1;
+  p.dt = dt_dev;
+  p.maxabs = (unsigned long long*)maxabs_dev;
+  p.mode = 0;
+  return launch_stage(m, p);
+}
+
+extern "C" int dgk_burgers_rhs(dgk_dgmesh m, const double* u_dev, double* rhs_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(m && u_dev && rhs_dev, "dgk_burgers_rhs: NULL argument");
+  DGK_REQUIRE(rhs_dev != u_dev, "dgk_burgers_rhs: rhs must not alias u");
+  StageParams p{};
+  p.u = u_dev;
+  p.u0 = u_dev;
+  p.out = rhs_dev;
+  p.mode = 1;
+  return launch_stage(m, p);
+}
+
+extern "C" int dgk_max_abs(dgk_dgmesh m, const double* u_dev, double* maxabs_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(m && u_dev && maxabs_dev, "dgk_max_abs: NULL argument");
+  int grid = std::min(m->ntiles, m->dev->cu_count * 8);
+  if (grid < 1) return DGK_OK;
+  hipLaunchKernelGGL(max_abs_kernel, dim3(grid), dim3(256), 0, m->dev->stream, m->tiles, m->ntiles,
+                     m->Np, u_dev, (unsigned long long*)maxabs_dev);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+extern "C" int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double hmin,
+                          double* dt_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(dev && maxabs_dev && dt_dev, "dgk_cfl_dt: NULL argument");
+  hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(64), 0, dev->stream, maxabs_dev, cfl, hmin, dt_dev);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+extern "C" int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu) {
+  dgk::clear_error();
+  DGK_REQUIRE(m, "dgk_burgers_set_tile: NULL mesh");
+  int tb = elems_per_block == 0 ? 64 : elems_per_block;
+  DGK_REQUIRE(tb == 64 || tb == 128 || tb == 256, "dgk_burgers_set_tile: elems_per_block %d", tb);
+  DGK_REQUIRE(blocks_per_cu >= 0 && blocks_per_cu <= 64, "dgk_burgers_set_tile: blocks_per_cu %d",
+              blocks_per_cu);
+  m->blocks_per_cu = blocks_per_cu;
+  if (tb != m->tb) {
+    m->tb = tb;
+    return build_tiles(m);
+  }
+  return DGK_OK;
+}

@@ -1,0 +1,273 @@
+/*
+ * dgk.h -- C ABI of libdgk, the MI355X (gfx950) native backend for the
+ * tetrahedral DG hot path of DGKernel.
+ *
+ * Two layers:
+ *
+ *  1. The device/memory/kernel/graph layer that replaces the gocca API the
+ *     reference's runner calls through cgo (citations are the reference call
+ *     sites each function replaces). A registry of hand-written kernels keyed
+ *     by kernel name + the runner-generated signature replaces OCCA's JIT.
+ *
+ *  2. Hot-path entry points, callable directly (and from inside a graph
+ *     capture): physical derivative, FaceConnector exchange, fused
+ *     Lax-Friedrichs/LIFT/SSP-RK3 stage, CFL step size, Persson-Peraire
+ *     sensor and the learned-viscosity MLP.
+ *
+ * Conventions: every function returns an int status (DGK_OK == 0). On
+ * failure dgk_last_error() returns a message for the calling thread. No
+ * function aborts the process. Pointers named *_dev are device pointers;
+ * all other pointers are host memory. Entry points enqueue work on the
+ * device's stream and return without synchronizing unless stated; host
+ * buffers passed to copies may be reused as soon as the call returns.
+ *
+ * Partitioned arrays follow the runner's pooled layout
+ * (runner/builder/builder.go:113-159): one allocation X_global per array,
+ * partition p's values start at value offset X_offsets[p], element l of
+ * partition p has its Np nodal values at X_global[X_offsets[p] + l*Np + n]
+ * (element-major, runner/kernel_copy.go:86-110).
+ */
+#ifndef DGK_H
+#define DGK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DGK_OK = 0,
+  DGK_ERR_INVALID = 1,     /* bad argument, shape or state */
+  DGK_ERR_HIP = 2,         /* a HIP runtime call failed */
+  DGK_ERR_NOT_FOUND = 3,   /* unknown kernel name / signature in the registry */
+  DGK_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+  DGK_ERR_NUMERIC = 5      /* setup found a non-positive Jacobian or similar */
+};
+
+typedef struct dgk_device_s* dgk_device;
+typedef struct dgk_kernel_s* dgk_kernel;
+typedef struct dgk_graph_s* dgk_graph;
+typedef struct dgk_dgmesh_s* dgk_dgmesh;
+typedef struct dgk_timer_s* dgk_timer;
+
+/* Last error message of the calling thread ("" if none). */
+const char* dgk_last_error(void);
+/* Library version string. */
+const char* dgk_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Device: replaces gocca.NewDevice (utils/device_helpers.go:9-39),        */
+/* OCCADevice.Finish (runner/kernel_execution.go:61), .Mode, .Free.        */
+/* props: JSON such as {"mode": "HIP", "device_id": 0}. Modes "HIP" and    */
+/* "CUDA" (accepted for existing callers) select the GPU; other modes      */
+/* return DGK_ERR_UNSUPPORTED.                                              */
+/* ---------------------------------------------------------------------- */
+int dgk_device_create(const char* props, dgk_device* out);
+int dgk_device_destroy(dgk_device dev);
+int dgk_device_sync(dgk_device dev);
+const char* dgk_device_mode(dgk_device dev);
+/* name: buffer of at least 256 bytes. cu_count/lds_bytes may be NULL. */
+int dgk_device_info(dgk_device dev, char* name, int* cu_count, int* lds_bytes, int* arch_major,
+                    int* arch_minor);
+
+/* ---------------------------------------------------------------------- */
+/* Memory: replaces OCCADevice.Malloc (runner/runner.go:83,196,210) and    */
+/* OCCAMemory.CopyFrom/CopyTo[WithOffset] (runner/kernel_copy.go:30,98,    */
+/* 331; runner/memory_operations.go:13-418). Copies are synchronous with   */
+/* respect to the host buffer (cgo may not retain Go pointers).            */
+/* ---------------------------------------------------------------------- */
+int dgk_malloc(dgk_device dev, size_t bytes, const void* init_host, void** out_dev);
+int dgk_free(dgk_device dev, void* ptr_dev);
+int dgk_copy_h2d(dgk_device dev, void* dst_dev, const void* src_host, size_t bytes,
+                 size_t dst_offset_bytes);
+int dgk_copy_d2h(dgk_device dev, void* dst_host, const void* src_dev, size_t bytes,
+                 size_t src_offset_bytes);
+int dgk_copy_d2d(dgk_device dev, void* dst_dev, const void* src_dev, size_t bytes);
+/* Fill with a byte value by a small kernel (safe under graph capture). */
+int dgk_fill_bytes(dgk_device dev, void* dst_dev, int value, size_t bytes);
+
+/* ---------------------------------------------------------------------- */
+/* Kernel registry: replaces OCCADevice.BuildKernelFromString              */
+/* (runner/runner.go:106-109) and OCCAKernel.RunWithArgs                   */
+/* (runner/kernel_execution.go:57). signature is the string the runner     */
+/* generates (runner/kernel_config.go:186-246): "const int_t* K, const     */
+/* double* Dr_NudgTet2, ..., const double* U_global, const int_t*          */
+/* U_offsets, ...". Arguments are matched to the native kernel by name.    */
+/* Known kernels: see dgk_kernel_list().                                    */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  int npart;      /* NPART */
+  int kpart_max;  /* KpartMax */
+  int int_size;   /* sizeof(int_t): 8 (INT64 default) or 4 (INT32) */
+  int np;         /* nodes per element of the element arrays (NP) */
+} dgk_kernel_consts;
+
+enum { DGK_ARG_MEM = 0, DGK_ARG_F64 = 1, DGK_ARG_F32 = 2, DGK_ARG_I64 = 3, DGK_ARG_I32 = 4 };
+
+typedef struct {
+  int kind;
+  int pad;
+  union {
+    const void* mem;
+    double f64;
+    float f32;
+    int64_t i64;
+    int32_t i32;
+  } v;
+} dgk_arg;
+
+/* Newline-separated registry names into buf (size bytes). */
+int dgk_kernel_list(char* buf, size_t size);
+int dgk_kernel_get(dgk_device dev, const char* name, const char* signature,
+                   const dgk_kernel_consts* consts, dgk_kernel* out);
+/* Enqueue one launch. host_K: the partition sizes (the runner's K array,
+ * npart values of int_size bytes) -- used to size the launch without a
+ * device read. args: in signature order. */
+int dgk_kernel_launch(dgk_kernel k, const void* host_K, int nargs, const dgk_arg* args);
+int dgk_kernel_free(dgk_kernel k);
+
+/* ---------------------------------------------------------------------- */
+/* Graphs and timing                                                       */
+/* ---------------------------------------------------------------------- */
+int dgk_graph_begin(dgk_device dev);
+int dgk_graph_end(dgk_device dev, dgk_graph* out);
+int dgk_graph_launch(dgk_graph g);
+int dgk_graph_destroy(dgk_graph g);
+int dgk_timer_create(dgk_device dev, dgk_timer* out);
+int dgk_timer_start(dgk_timer t);
+int dgk_timer_stop(dgk_timer t);
+/* Synchronizes on the stop event; milliseconds between start and stop. */
+int dgk_timer_elapsed_ms(dgk_timer t, double* ms);
+int dgk_timer_destroy(dgk_timer t);
+
+/* ---------------------------------------------------------------------- */
+/* Hot path: physical derivative                                            */
+/* Replaces the OKL kernel "differentiate" of                              */
+/* tetnudg/tet_nudg_mesh_physical_derivative_test.go:850-892 (FP64) and    */
+/* :240-278 (FP32 metrics/outputs) with the host result of                 */
+/* calcPhysicalDerivative (:344-367):                                       */
+/*   Ur = Dr u, Us = Ds u, Ut = Dt u;                                       */
+/*   du/dx = rx Ur + sx Us + tx Ut (likewise y, z).                         */
+/* ---------------------------------------------------------------------- */
+enum { DGK_F64 = 0, DGK_F32 = 1 };
+
+/* Per-node metrics (the reference's API layout: 13 Np*K arrays).
+ * Dr/Ds/Dt_dev: Np x Np row-major FP64 on the device.
+ * metrics_dev[9]: rx, ry, rz, sx, sy, sz, tx, ty, tz global pointers.
+ * Offsets arrays (value offsets, int64, device, npart+1 entries) for u,
+ * metrics and outputs. mtype/otype: DGK_F64 or DGK_F32 for the metric and
+ * output arrays (u is always FP64). */
+int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int64_t* K_host,
+                         const double* Dr_dev, const double* Ds_dev, const double* Dt_dev,
+                         const double* u_dev, const int64_t* u_off_dev,
+                         const void* const* metrics_dev, int mtype,
+                         const int64_t* m_off_dev, void* dudx_dev, void* dudy_dev, void* dudz_dev,
+                         int otype, const int64_t* o_off_dev);
+
+/* Affine elements (native layout): metrics are one 3x3 block per element,
+ * geo_dev[q*K + k] for q = 0..8 (rx ry rz sx sy sz tx ty tz), FP64.
+ * u/out: dense element-major [k*Np + n]. prec: DGK_F64 (FP64 FMA),
+ * DGK_F32 (FP32 math, FP32 outputs), or DGK_PREC_BF16X3 (split-bf16 MFMA,
+ * FP32 accumulate, FP32 outputs). */
+enum { DGK_PREC_F64 = 0, DGK_PREC_F32 = 1, DGK_PREC_BF16X3 = 2 };
+int dgk_phys_deriv_affine(dgk_device dev, int N, int64_t K, const double* Dr_dev,
+                          const double* Ds_dev, const double* Dt_dev, const double* u_dev,
+                          const double* geo_dev, void* dudx_dev, void* dudy_dev, void* dudz_dev,
+                          int prec);
+
+/* MATMUL_<X>(IN, OUT, K[part]) / MATMUL_ADD_<X> of
+ * runner/builder/builder.go:355-410 over all partitions in one launch:
+ * OUT[e*rows + i] (=|+=) sum_j A(i,j) IN[e*cols + j]. A_dev is the
+ * column-major device matrix of runner/runner.go:395-400 (A[j*rows + i]). */
+int dgk_matmul(dgk_device dev, int rows, int cols, const double* A_colmajor_dev, int npart,
+               const int64_t* K_host, const double* in_dev, const int64_t* in_off_dev,
+               double* out_dev, const int64_t* out_off_dev, int add);
+
+/* ---------------------------------------------------------------------- */
+/* Hot path: FaceConnector exchange (utils/face_connector.go:143-187 index */
+/* semantics; runtime exchange of utils/face_connector_test.go:104-156):   */
+/*   P_global[P_off[dst] + place[j]] = u_global[u_off[src] + pick[i]]       */
+/* for every (src, dst) pair, including src == dst, in one launch.         */
+/* A plan holds the concatenated lists on the device: pick lists src-major */
+/* (pick[src][dst] for src, then dst), place lists dst-major (place[dst]   */
+/* [src]); counts[src*npart + dst] gives each pair's length.               */
+/* ---------------------------------------------------------------------- */
+typedef struct dgk_face_plan_s* dgk_face_plan;
+int dgk_face_plan_create(dgk_device dev, int npart, const int64_t* counts_host,
+                         const int32_t* pick_host, const int32_t* place_host,
+                         dgk_face_plan* out);
+int dgk_face_plan_destroy(dgk_face_plan plan);
+int dgk_face_exchange(dgk_face_plan plan, const double* u_dev, const int64_t* u_off_dev,
+                      double* P_dev, const int64_t* P_off_dev);
+
+/* ---------------------------------------------------------------------- */
+/* DG mesh on the device: partitions, neighbours, orientation codes and    */
+/* FP64 affine geometry computed on the GPU from vertex coordinates.       */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  int N;                       /* polynomial order */
+  int npart;                   /* partitions */
+  const int64_t* K;            /* [npart] elements per partition */
+  const int64_t* state_off;    /* [npart+1] value offsets of state arrays (runner layout) */
+  const double* Dr;            /* Np x Np row-major */
+  const double* Ds;
+  const double* Dt;
+  const double* LIFT;          /* Np x 4Nfp row-major */
+  const int* Fmask;            /* [4][Nfp] */
+  const double* r;             /* [Np] reference node coordinates */
+  const double* s;
+  const double* t;
+  const int* face_nodes;       /* [4][24][Nfp]: neighbour volume node of own face node, per code */
+  const int64_t* EToE;         /* [Ktot][4] global element of each face's neighbour;  */
+                               /* global elements are numbered partition-major        */
+  const int* codes;            /* [Ktot][4] orientation code 6*f2 + sigma */
+  const double* vtx;           /* [Ktot][4][3] vertex coordinates (affine map corners) */
+} dgk_dgmesh_desc;
+
+int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* desc, dgk_dgmesh* out);
+int dgk_dgmesh_destroy(dgk_dgmesh m);
+/* Device geometry, SoA [q][Ktot]: q = 0..8 rx ry rz sx sy sz tx ty tz,
+ * 9 J, then for faces f = 0..3: 10+4f nx, 11+4f ny, 12+4f nz, 13+4f Fscale
+ * (26 rows). Copied to host. */
+int dgk_dgmesh_geometry(dgk_dgmesh m, double* geo_host);
+/* Device pointer to the SoA geometry (26 x Ktot doubles). */
+int dgk_dgmesh_geometry_dev(dgk_dgmesh m, const double** geo_dev);
+/* Minimum distance between distinct nodes of one element over the mesh,
+ * FP64 on the device (examples/3d_scalar_burgers_equation.go:697-717).
+ * Synchronous. */
+int dgk_dgmesh_min_node_distance(dgk_dgmesh m, double* hmin);
+/* Writes uP[k][f][fp] (dense global element order) as the fused stage
+ * kernel sees it: the neighbour trace gathered through EToE + codes. */
+int dgk_dgmesh_trace_plus(dgk_dgmesh m, const double* u_dev, double* uP_dev);
+
+/* ---------------------------------------------------------------------- */
+/* Hot path: fused Burgers stage (strong-form DG, Lax-Friedrichs flux,     */
+/* LIFT, SSP-RK3 combination) on all partitions in one persistent launch:  */
+/*   out = a0*u0 + a1*(u + dt*L(u)),                                        */
+/*   L(u) = -div(u^2/2 (1,1,1)) + LIFT (Fscale (n.f(u-) - n.f*)),           */
+/*   n.f* = (n.f(u-) + n.f(u+))/2 - alpha |n.(1,1,1)| (u+ - u-)/2,          */
+/*   alpha = max(|u-|, |u+|).                                               */
+/* dt is read from device memory (dt_dev). If maxabs_dev is not NULL the   */
+/* kernel also folds max|out| into *maxabs_dev (atomic, bitwise max of the */
+/* non-negative double). out may alias u0 but not u.                        */
+/* ---------------------------------------------------------------------- */
+int dgk_burgers_stage(dgk_dgmesh m, const double* u_dev, const double* u0_dev, double* out_dev,
+                      double a0, double a1, const double* dt_dev, double* maxabs_dev);
+/* L(u) alone into rhs_dev (runner layout). */
+int dgk_burgers_rhs(dgk_dgmesh m, const double* u_dev, double* rhs_dev);
+/* max|u| over all elements into *maxabs_dev (which must hold 0 or a
+ * previous max). */
+int dgk_max_abs(dgk_dgmesh m, const double* u_dev, double* maxabs_dev);
+/* dt = cfl * hmin / *maxabs_dev into *dt_dev (FP64), then *maxabs_dev = 0. */
+int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double hmin, double* dt_dev);
+/* Tile shape of the fused stage: elements per block (64, 128 or 256) and
+ * blocks per CU of the persistent grid. 0 restores the tuned default. */
+int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGK_H */

@@ -1,0 +1,297 @@
+"""GPU parity of the hot path against the CPU oracle (calls through the C ABI)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dgkernel_amd import _lib, dg
+from dgkernel_amd import mesh as M
+from dgkernel_amd.nudg import build_ref_tet
+from oracle import oracle as O
+from tests.helpers import neu_mesh, pooled, rel_l2, smooth_u0
+
+pytestmark = pytest.mark.gpu
+
+METRICS = ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz")
+
+
+def _pool(parts, Kp, per_elem, dtype):
+    """Runner pooled layout (WarpAlign) of per-partition arrays of one dtype."""
+    vs = np.dtype(dtype).itemsize
+    offs, nbytes = dg.aligned_offsets(np.asarray(Kp), per_elem, vs)
+    out = np.zeros(nbytes // vs, dtype=dtype)
+    for p, v in enumerate(parts):
+        out[offs[p]:offs[p] + v.size] = v.reshape(-1)
+    return out, offs
+
+
+def _deriv_gpu(device, N, Kp, U_parts, met_parts, mtype=_lib.DGK_F64, otype=_lib.DGK_F64, el=None):
+    """Run dgk_phys_deriv_nodal on per-partition [K_p, Np] arrays."""
+    el = el or O.element(N)
+    Np = el["Np"]
+    mdt = np.float64 if mtype == _lib.DGK_F64 else np.float32
+    odt = np.float64 if otype == _lib.DGK_F64 else np.float32
+    u_pool, uoff = _pool(U_parts, Kp, Np, np.float64)
+    du, duo = device.upload(u_pool), device.upload(uoff)
+    mbufs, moff = [], None
+    for q in range(9):
+        mp, moff = _pool([m[q] for m in met_parts], Kp, Np, mdt)
+        mbufs.append(device.upload(mp))
+    dmo = device.upload(moff)
+    o_pool, ooff = _pool([np.zeros((k, Np)) for k in Kp], Kp, Np, odt)
+    outs = [device.upload(o_pool) for _ in range(3)]
+    doo = device.upload(ooff)
+    Dm = [device.upload(np.ascontiguousarray(el[k])) for k in ("Dr", "Ds", "Dt")]
+    Karr = np.asarray(Kp, dtype=np.int64)
+    mptrs = (C.c_void_p * 9)(*[b.ptr.value for b in mbufs])
+    _lib.call("dgk_phys_deriv_nodal", device.h, N, len(Kp), Karr.ctypes.data_as(C.POINTER(C.c_int64)),
+              Dm[0].ptr, Dm[1].ptr, Dm[2].ptr, du.ptr, duo.ptr, mptrs, mtype, dmo.ptr,
+              outs[0].ptr, outs[1].ptr, outs[2].ptr, otype, doo.ptr)
+    device.sync()
+    res = []
+    for o in outs:
+        h = o.to_host()
+        res.append([h[ooff[p]:ooff[p] + Kp[p] * Np].reshape(Kp[p], Np) for p in range(len(Kp))])
+    return res
+
+
+def test_physical_derivative_neu_p2_closed_form(device):
+    """tetnudg/tet_nudg_mesh_physical_derivative_test.go:747-919 on the GPU."""
+    N = 2
+    m = neu_mesh()
+    o = O.Setup(N, m.VX, m.VY, m.VZ, m.EToV)
+    assert o.status == 0
+    el = O.element(N)
+    X, Y, Z = o.field("X"), o.field("Y"), o.field("Z")
+    U = X ** N + Y ** N + Z ** N
+    mets = [o.field(k) for k in METRICS]
+    host = O.phys_deriv(el["Dr"], el["Ds"], el["Dt"], U, mets)
+    out = _deriv_gpu(device, N, [m.K], [U], [mets], el=el)
+    exact = [N * X ** (N - 1), N * Y ** (N - 1), N * Z ** (N - 1)]
+    for d in range(3):
+        g = out[d][0]
+        assert np.max(np.abs(g - exact[d])) < 1e-8
+        assert np.max(np.abs(g - host[d])) <= 1e-13 * np.max(np.abs(host[d]))
+
+
+@pytest.mark.parametrize("N,parts", [(4, [700, 1100, 813]), (3, [1500, 1113]), (1, [2613]),
+                                     (5, [900, 1713]), (6, [2613])])
+def test_physical_derivative_partitioned_fp64(device, N, parts):
+    m = M.cube_mesh(8, 8, 7, jitter=0.1, seed=3)  # 2688 tets, jittered, non-periodic
+    K = sum(parts)
+    m.EToV, m.EToVc = m.EToV[:K], m.EToVc[:K]
+    o = O.Setup(N, m.VX, m.VY, m.VZ, m.EToV)
+    assert o.status == 0
+    el = O.element(N)
+    U = M.uniform(11, 3, K * el["Np"], -1.0, 1.0).reshape(K, el["Np"])
+    mets = [o.field(k) for k in METRICS]
+    host = O.phys_deriv(el["Dr"], el["Ds"], el["Dt"], U, mets)
+    starts = np.concatenate(([0], np.cumsum(parts)))
+    Up = [U[starts[p]:starts[p + 1]] for p in range(len(parts))]
+    Mp = [[mm[starts[p]:starts[p + 1]] for mm in mets] for p in range(len(parts))]
+    out = _deriv_gpu(device, N, parts, Up, Mp, el=el)
+    for d in range(3):
+        g = np.concatenate(out[d])
+        scale = np.max(np.abs(host[d]))
+        assert np.max(np.abs(g - host[d])) <= 1e-13 * scale
+
+
+def test_physical_derivative_fp32_metrics(device):
+    """FP32 metrics and outputs (the reference's CalculatePhysicalDerivative32)."""
+    N = 4
+    m = M.cube_mesh(6, 6, 6, jitter=0.1, seed=4)
+    o = O.Setup(N, m.VX, m.VY, m.VZ, m.EToV)
+    el = O.element(N)
+    U = M.uniform(12, 3, m.K * el["Np"], -1.0, 1.0).reshape(m.K, el["Np"])
+    mets = [o.field(k) for k in METRICS]
+    mets32 = [mm.astype(np.float32).astype(np.float64) for mm in mets]
+    host = O.phys_deriv(el["Dr"], el["Ds"], el["Dt"], U, mets32)
+    out = _deriv_gpu(device, N, [m.K], [U], [mets], mtype=_lib.DGK_F32, otype=_lib.DGK_F32, el=el)
+    for d in range(3):
+        g = out[d][0].astype(np.float64)
+        # FP64 arithmetic on FP32 metrics, rounded once to FP32 on store.
+        assert np.max(np.abs(g - host[d]) / (np.abs(host[d]) + 1e-3 * np.abs(host[d]).max())) < 2e-7
+
+
+def test_matmul_known_answer_and_random(device):
+    """runner/runner_matrix_test.go:13-125 known answer, plus MATMUL_ADD."""
+    A = np.array([[2.0, 1.0, 0.0], [0.0, 1.0, 1.0], [1.0, 0.0, 2.0]])
+    Kp = [3, 5]
+    IN = np.arange(1.0, 25.0).reshape(8, 3)
+    want = O.matmul(A, IN)
+    np.testing.assert_allclose(want[0], A @ IN[0])
+    inp, ioff = pooled([IN[:3], IN[3:]], Kp, 3)
+    outp, ooff = pooled([np.zeros((3, 3)), np.zeros((5, 3))], Kp, 3)
+    dA = device.upload(np.ascontiguousarray(A.T))  # column-major device matrix
+    din, dout = device.upload(inp), device.upload(outp)
+    dio, doo = device.upload(ioff), device.upload(ooff)
+    Karr = np.array(Kp, dtype=np.int64)
+    _lib.call("dgk_matmul", device.h, 3, 3, dA.ptr, 2, Karr.ctypes.data_as(C.POINTER(C.c_int64)),
+              din.ptr, dio.ptr, dout.ptr, doo.ptr, 0)
+    _lib.call("dgk_matmul", device.h, 3, 3, dA.ptr, 2, Karr.ctypes.data_as(C.POINTER(C.c_int64)),
+              din.ptr, dio.ptr, dout.ptr, doo.ptr, 1)
+    h = dout.to_host()
+    got = np.concatenate([h[ooff[0]:ooff[0] + 9].reshape(3, 3), h[ooff[1]:ooff[1] + 15].reshape(5, 3)])
+    np.testing.assert_array_equal(got, O.matmul(A, IN, add_to=want))
+
+
+def _exchange_gpu(device, fc, u_parts, Np, Nfp):
+    """dgk_face_exchange of per-partition u into per-partition P buffers."""
+    NP = fc.num_partitions
+    Kp = fc.elems_per_partition
+    u_pool, uoff = pooled(u_parts, Kp, Np)
+    P_pool, Poff = pooled([np.full(int(k) * 4 * Nfp, np.nan) for k in Kp], Kp, 4 * Nfp)
+    counts = np.array([[fc.pick[(s, d)].size for d in range(NP)] for s in range(NP)], dtype=np.int64)
+    pick = np.concatenate([fc.pick[(s, d)] for s in range(NP) for d in range(NP)]).astype(np.int32)
+    place = np.concatenate([fc.place[(d, s)] for d in range(NP) for s in range(NP)]).astype(np.int32)
+    plan = C.c_void_p()
+    _lib.call("dgk_face_plan_create", device.h, NP, counts.ctypes.data_as(C.POINTER(C.c_int64)),
+              pick.ctypes.data_as(C.POINTER(C.c_int32)), place.ctypes.data_as(C.POINTER(C.c_int32)),
+              C.byref(plan))
+    du, dP = device.upload(u_pool), device.upload(P_pool)
+    duo, dPo = device.upload(uoff), device.upload(Poff)
+    _lib.call("dgk_face_exchange", plan, du.ptr, duo.ptr, dP.ptr, dPo.ptr)
+    device.sync()
+    h = dP.to_host()
+    _lib.call("dgk_face_plan_destroy", plan)
+    return [h[Poff[p]:Poff[p] + int(Kp[p]) * 4 * Nfp] for p in range(NP)]
+
+
+def _simulate_exchange(fc, u_parts, Nfp):
+    """utils/face_connector_test.go:104-156 semantics on the host."""
+    out = [np.full(int(k) * 4 * Nfp, np.nan) for k in fc.elems_per_partition]
+    for (s, d), pk in fc.pick.items():
+        out[d][fc.place[(d, s)]] = u_parts[s].reshape(-1)[pk]
+    return out
+
+
+def test_face_exchange_neu_five_partitions_bit_exact(device):
+    """cube-partitioned.neu: groups p1..p4 give 5 partitions, p0 empty (face_connector.go:59-64)."""
+    N = 2
+    m = neu_mesh()
+    o = O.Setup(N, m.VX, m.VY, m.VZ, m.EToV)
+    el = O.element(N)
+    vP = o.imap("vmapP").reshape(-1)
+    fc = M.new_face_connector(m.K, 4, el["Nfp"], el["Np"], vP, m.EToP)
+    assert fc.num_partitions == 5 and fc.elems_per_partition[0] == 0
+    fc.verify()
+    NPo, counts, pk, pl = O.face_connector(m.K, 4, el["Nfp"], el["Np"], vP, m.EToP)
+    assert all(np.array_equal(pk[k], fc.pick[k]) for k in pk)
+    u = M.uniform(5, 9, m.K * el["Np"], -1, 1).reshape(m.K, el["Np"])
+    u_parts = [u[fc.local_to_global[p]] for p in range(fc.num_partitions)]
+    got = _exchange_gpu(device, fc, u_parts, el["Np"], el["Nfp"])
+    want = _simulate_exchange(fc, u_parts, el["Nfp"])
+    for p in range(fc.num_partitions):
+        assert np.array_equal(got[p], want[p])  # bit-exact copy
+        # and the value placed is u at VmapP (element-ID style check)
+    flat_P = np.zeros((m.K, 4 * el["Nfp"]))
+    for p in range(fc.num_partitions):
+        flat_P[fc.local_to_global[p]] = got[p].reshape(-1, 4 * el["Nfp"])
+    assert np.array_equal(flat_P.reshape(-1), u.reshape(-1)[vP])
+
+
+@pytest.mark.parametrize("N,n,per,parts", [(2, 6, True, [300, 500, 496]), (3, 5, True, [750]),
+                                           (3, 4, False, [100, 284])])
+def test_trace_plus_matches_face_exchange(device, N, n, per, parts):
+    """The fused stage's neighbour gather equals the FaceConnector exchange, bit for bit."""
+    m = M.cube_mesh(n, n, n, periodic=per)
+    st = dg.make_setup(m, N, Kp=parts)
+    mesh = dg.DGMesh(device, st)
+    ref = st.ref
+    u = M.uniform(6, 1, m.K * ref.Np, -1, 1).reshape(m.K, ref.Np)
+    du = device.upload(st.to_pooled(u))
+    dP = device.zeros(m.K * 4 * ref.Nfp)
+    mesh.trace_plus(du, dP)
+    device.sync()
+    got = dP.to_host()
+    vm, vp = st.vmaps()
+    EToP = M.etop_from_counts(st.Kp)
+    fc = M.new_face_connector(m.K, 4, ref.Nfp, ref.Np, vp, EToP)
+    u_parts = [u[fc.local_to_global[p]] for p in range(fc.num_partitions)]
+    ex = _simulate_exchange(fc, u_parts, ref.Nfp)
+    assert np.array_equal(got, np.concatenate(ex))
+    # and the oracle's coordinate-matched VmapP agrees
+    o = O.Setup.from_mesh(N, m)
+    assert np.array_equal(o.imap("vmapP").reshape(-1), vp)
+
+
+def test_geometry_and_hmin_vs_oracle(device):
+    N = 3
+    m = M.cube_mesh(5, 4, 3, periodic=True, jitter=0.1, seed=8)
+    st = dg.make_setup(m, N, Kp=[100, 260])
+    mesh = dg.DGMesh(device, st)
+    geo = mesh.geometry()
+    o = O.Setup.from_mesh(N, m)
+    for q, k in enumerate(METRICS):
+        ref = o.field(k)
+        assert np.max(np.abs(geo[q][:, None] - ref)) <= 1e-12 * np.max(np.abs(ref))
+    J = o.field("J")
+    assert np.max(np.abs(geo[9][:, None] - J)) <= 1e-12 * J.max()
+    Fs = o.field("Fscale").reshape(m.K, 4, -1)
+    for f in range(4):
+        assert np.max(np.abs(geo[13 + 4 * f][:, None] - Fs[:, f, :])) <= 1e-12 * Fs.max()
+        for d, nm in enumerate(("nx", "ny", "nz")):
+            nn = o.field(nm).reshape(m.K, 4, -1)[:, f, :]
+            assert np.max(np.abs(geo[10 + 4 * f + d][:, None] - nn)) <= 1e-13
+    assert abs(mesh.min_node_distance() - o.min_node_distance()) <= 1e-15
+
+
+@pytest.mark.parametrize("N,n,parts", [(2, 8, [768] * 4), (3, 5, [200, 350, 200]), (1, 6, [1296]),
+                                       (4, 3, [100, 62])])
+def test_burgers_rhs_vs_oracle(device, N, n, parts):
+    m = M.cube_mesh(n, n, n, periodic=True)
+    st = dg.make_setup(m, N, Kp=parts)
+    u = smooth_u0(st, noise=1e-3)
+    o = O.Setup.from_mesh(N, m)
+    want = o.burgers_rhs(u)
+    solver = dg.BurgersSolver(device, st)
+    got = solver.rhs(u)
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_config1_burgers_100_steps_vs_oracle(device):
+    """Config 1: 8^3 hexes x 6, P=2, 4 slabs, LF + SSP-RK3, CFL 0.3, 100 steps."""
+    m = M.cube_mesh(8, 8, 8, periodic=True)
+    st = dg.make_setup(m, 2, Kp=M.slab_partition(m.K, 4))
+    u0 = smooth_u0(st, amp=0.3, noise=1e-3, seed=1)
+    o = O.Setup.from_mesh(2, m)
+    hmin = o.min_node_distance()
+    want, dts = o.ssprk3(u0, 100, 0.3, hmin)
+    solver = dg.BurgersSolver(device, st, cfl=0.3, hmin=hmin)
+    solver.set_state(u0)
+    solver.step(100)
+    got = solver.state()
+    assert rel_l2(got, want) <= 1e-12
+    assert np.max(np.abs(got - want)) <= 1e-11
+    # dt follows max|u|, which agrees to the state's round-off
+    assert abs(solver.dt() - dts[-1]) <= 1e-12 * dts[-1]
+
+
+def test_graph_replay_matches_eager(device):
+    m = M.cube_mesh(6, 6, 6, periodic=True)
+    st = dg.make_setup(m, 3, Kp=[400, 896])
+    u0 = smooth_u0(st)
+    a = dg.BurgersSolver(device, st)
+    a.set_state(u0)
+    b = dg.BurgersSolver(device, st, hmin=a.hmin)
+    b.set_state(u0)
+    for rep in range(3):
+        a.step(2)
+        b.step(2, graph=True)
+        device.sync()
+        assert np.array_equal(a.state(), b.state()), f"replay {rep}"
+
+
+@pytest.mark.parametrize("tb,bpc", [(64, 0), (128, 0), (256, 0), (64, 3)])
+def test_tile_shapes_agree(device, tb, bpc):
+    m = M.cube_mesh(6, 5, 4, periodic=True)
+    st = dg.make_setup(m, 3, Kp=[333, 387])
+    u0 = smooth_u0(st)
+    ref = dg.BurgersSolver(device, st)
+    ref.set_state(u0)
+    ref.step(3)
+    s = dg.BurgersSolver(device, st, hmin=ref.hmin)
+    s.mesh.set_tile(tb, bpc)
+    s.set_state(u0)
+    s.step(3)
+    assert np.array_equal(ref.state(), s.state())
