@@ -152,24 +152,28 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Keeps a pointer opaque to the optimizer so that loads through it are not
-// hoisted out of the tile loop (which would pin the whole operator in
-// registers and spill); the value stays in SGPRs.
-template <typename T>
-__device__ __forceinline__ const T* launder_sgpr(const T* p) {
-  asm volatile("" : "+s"(p));
-  return p;
-}
+// Pointer into the constant address space: uniform loads through it become
+// scalar loads (SGPR operands) even after the kernel has stored to memory.
+typedef const __attribute__((address_space(4))) double* cdouble_ptr;
 
 template <int N, int TB>
 __global__ void __launch_bounds__(TB) burgers_stage_kernel(StageParams p) {
   constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP;
   constexpr int S = NP + 1;  // padded LDS row: odd stride in doubles spreads banks
   constexpr FmaskTable<N> FM{};
-  __shared__ double sU[TB * S];
-  __shared__ double sV[TB * S];
+  constexpr int NOP = 3 * NP * NP + NP * NF;  // Dr, Ds, Dt, LIFT
+  __shared__ __attribute__((aligned(16))) double sOp[NOP];
+  __shared__ __attribute__((aligned(16))) double sU[TB * S];
   __shared__ uint8_t sFN[4 * 24 * NFP];
+  // The operators stay in LDS for the block's lifetime; every read below has
+  // a wave-uniform address, so it is a broadcast (one LDS access per wave).
   for (int i = threadIdx.x; i < 4 * 24 * NFP; i += TB) sFN[i] = p.face_nodes[i];
+  for (int i = threadIdx.x; i < NP * NP; i += TB) {
+    sOp[i] = p.Dr[i];
+    sOp[NP * NP + i] = p.Ds[i];
+    sOp[2 * NP * NP + i] = p.Dt[i];
+  }
+  for (int i = threadIdx.x; i < NP * NF; i += TB) sOp[3 * NP * NP + i] = p.LIFT[i];
   const double dt = p.mode == 0 ? *p.dt : 0.0;
   const int64_t K = p.K;
   double lmax = 0.0;
@@ -217,10 +221,14 @@ __global__ void __launch_bounds__(TB) burgers_stage_kernel(StageParams p) {
             p.out[g * NF + f * NFP + fp] = p.u[nb[f] + sFN[(f * 24 + code) * NFP + fp]];
         }
       } else {
-        const double* Dr = launder_sgpr(p.Dr);
-        const double* Ds = launder_sgpr(p.Ds);
-        const double* Dt = launder_sgpr(p.Dt);
-        const double* LIFT = launder_sgpr(p.LIFT);
+        // Operators through scalar loads (wave-uniform addresses). The zero
+        // offset depends on this tile's record, so the loads stay inside the
+        // tile loop instead of being hoisted into registers for all tiles.
+        const int64_t z = (int64_t)(__builtin_amdgcn_readfirstlane(cnt) >> 30);
+        const cdouble_ptr Dr = (cdouble_ptr)(p.Dr + z);
+        const cdouble_ptr Ds = (cdouble_ptr)(p.Ds + z);
+        const cdouble_ptr Dt = (cdouble_ptr)(p.Dt + z);
+        const cdouble_ptr LIFT = (cdouble_ptr)(p.LIFT + z);
         double v[NP];
         {
           const double ga = p.bgeo[0 * K + g], gb = p.bgeo[1 * K + g], gc = p.bgeo[2 * K + g];
@@ -262,18 +270,25 @@ __global__ void __launch_bounds__(TB) burgers_stage_kernel(StageParams p) {
             v[i] = acc;
           }
         }
-        double* sv = sV + e * S;
+        // Only this thread reads its own row of sU (uM above), so the row is
+        // overwritten in place with u + dt*L(u) (or L(u) alone).
+        double* sv = sU + e * S;
+        if (p.mode == 0) {
 #pragma unroll
-        for (int n = 0; n < NP; ++n) sv[n] = v[n];
+          for (int n = 0; n < NP; ++n) sv[n] = fma(dt, v[n], sv[n]);
+        } else {
+#pragma unroll
+          for (int n = 0; n < NP; ++n) sv[n] = v[n];
+        }
       }
     }
     if (p.mode == 2) continue;
     __syncthreads();
     for (int i = threadIdx.x; i < cnt * NP; i += TB) {
       int ee = i / NP, n = i - ee * NP;
-      double o = sV[ee * S + n];
+      double o = sU[ee * S + n];
       if (p.mode == 0) {
-        o = p.a1 * fma(dt, o, sU[ee * S + n]);
+        o = p.a1 * o;
         if (p.a0 != 0.0) o = fma(p.a0, p.u0[tile.vbase + i], o);
       }
       p.out[tile.vbase + i] = o;
@@ -284,6 +299,200 @@ __global__ void __launch_bounds__(TB) burgers_stage_kernel(StageParams p) {
     lmax = wave_max(lmax);
     if ((threadIdx.x & 63) == 0 && lmax > 0.0)
       atomicMax(p.maxabs, (unsigned long long)__double_as_longlong(lmax));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA variant of the stage (FP64 matrix cores, v_mfma_f64_16x16x4_f64).
+//
+// A block of 4 waves owns a tile of 64 elements, 16 per wave (the MFMA N
+// dimension). Per wave, in FP64 throughout:
+//   C_vol (3Np x 16)  = [Dr; Ds; Dt] (3Np x Np) * F (Np x 16),  F = u^2/2
+//   C_lift (Np x 16)  = LIFT (Np x 4Nfp) * G (4Nfp x 16),      G = flux jumps
+// The operator fragments (A operands) are loaded once per block and stay in
+// registers for every tile the block processes; F and G (B operands) are
+// formed lane by lane from the LDS copy of the tile and the neighbour
+// gather. Both products land in LDS, where the chain rule with the folded
+// affine metrics and the RK combination read them back.
+//
+// Operand layout of the 16x16x4 FP64 MFMA (wave64), checked on gfx950: lane
+// l provides A[l%16][l/16] and B[l/16][l%16] of the 16x4 / 4x16 operands and
+// holds D[l/16 + 4r][l%16], r = 0..3, of the 16x16 result (rows interleaved
+// by 4, unlike the FP32 16x16 layouts).
+// ---------------------------------------------------------------------------
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+template <int N>
+struct MfmaShape {
+  static constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP;
+  static constexpr int KSV = (NP + 3) / 4;           // K steps of the volume product
+  static constexpr int MTV = (3 * NP + 15) / 16;     // 16-row tiles of [Dr; Ds; Dt]
+  static constexpr int MTL = (NP + 15) / 16;         // 16-row tiles of LIFT
+  static constexpr int KSL = NF / 4;                 // K steps of the lift product
+  static constexpr int S = NP + 1;                   // LDS row stride of the u tile
+  static constexpr int CROWS = 4 * NP;               // C_vol (3Np) + C_lift (Np) rows kept
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) burgers_stage_mfma_kernel(StageParams p) {
+  using SH = MfmaShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, S = SH::S;
+  constexpr int KSV = SH::KSV, MTV = SH::MTV, MTL = SH::MTL, KSL = SH::KSL;
+  constexpr int E = 64;
+  constexpr FmaskTable<N> FM{};
+  __shared__ __attribute__((aligned(16))) double sU[E * S];
+  __shared__ __attribute__((aligned(16))) double sC[4][SH::CROWS * 16];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  __shared__ uint8_t sFM[NF];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, gq = lane >> 4;
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += 256) sFN[i] = p.face_nodes[i];
+  if (threadIdx.x < NF) sFM[threadIdx.x] = (uint8_t)FM.v[threadIdx.x / NFP][threadIdx.x % NFP];
+
+  // Operator fragments, resident for the block's lifetime.
+  double Av[MTV][KSV], Al[MTL][KSL];
+#pragma unroll
+  for (int t = 0; t < MTV; ++t)
+#pragma unroll
+    for (int s = 0; s < KSV; ++s) {
+      const int row = 16 * t + c, k = 4 * s + gq;
+      double v = 0.0;
+      if (k < NP) {
+        if (row < NP) v = p.Dr[row * NP + k];
+        else if (row < 2 * NP) v = p.Ds[(row - NP) * NP + k];
+        else if (row < 3 * NP) v = p.Dt[(row - 2 * NP) * NP + k];
+      }
+      Av[t][s] = v;
+    }
+#pragma unroll
+  for (int t = 0; t < MTL; ++t)
+#pragma unroll
+    for (int s = 0; s < KSL; ++s) {
+      const int row = 16 * t + c, k = 4 * s + gq;
+      Al[t][s] = row < NP ? p.LIFT[row * NF + k] : 0.0;
+    }
+
+  const double dt = p.mode == 0 ? *p.dt : 0.0;
+  const int64_t K = p.K;
+  double lmax = 0.0;
+  const int G = gridDim.x, b = blockIdx.x;
+  const int vb = (p.chunked && G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  const int t0 = p.chunked ? (int)((int64_t)p.ntiles * vb / G) : b;
+  const int t1 = p.chunked ? (int)((int64_t)p.ntiles * (vb + 1) / G) : p.ntiles;
+  const int ts = p.chunked ? 1 : G;
+  double* sCw = sC[w];
+  for (int t = t0; t < t1; t += ts) {
+    const Tile tile = p.tiles[t];
+    const int cnt = tile.count;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt * NP; i += 256) {
+      int e = i / NP, n = i - e * NP;
+      sU[e * S + n] = p.u[tile.vbase + i];
+    }
+    __syncthreads();
+    const int el = 16 * w + c;  // element of this lane's MFMA column
+    const bool valid = el < cnt;
+    const int64_t g = tile.gbase + (valid ? el : 0);
+    const double* su = sU + el * S;
+    // Neighbour data of the lane's element (replicated across the 4 lane groups).
+    const int packed = valid ? p.codes[g] : 0;
+    int32_t nb[4];
+    double sn[4], fs[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      nb[f] = valid ? p.nbr[f * K + g] : 0;
+      sn[f] = valid ? p.bgeo[(3 + f) * K + g] : 0.0;
+      fs[f] = valid ? p.bgeo[(7 + f) * K + g] : 0.0;
+    }
+    // Gather u+ for this lane's face points m = gq + 4s.
+    double uP[KSL];
+#pragma unroll
+    for (int s = 0; s < KSL; ++s) {
+      const int m = gq + 4 * s, f = m / NFP, fp = m - f * NFP;
+      const int code = (packed >> (5 * f)) & 31;
+      const int32_t base = f == 0 ? nb[0] : f == 1 ? nb[1] : f == 2 ? nb[2] : nb[3];
+      uP[s] = valid ? p.u[base + sFN[(f * 24 + code) * NFP + fp]] : 0.0;
+    }
+    // Volume product.
+    double4_t accV[MTV];
+#pragma unroll
+    for (int tt = 0; tt < MTV; ++tt) accV[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KSV; ++s) {
+      const int k = 4 * s + gq;
+      double Fk = 0.0;
+      if (valid && k < NP) {
+        const double uk = su[k];
+        Fk = 0.5 * uk * uk;
+      }
+#pragma unroll
+      for (int tt = 0; tt < MTV; ++tt)
+        accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Av[tt][s], Fk, accV[tt], 0, 0, 0);
+    }
+    // Flux jumps and the lift product.
+    double4_t accL[MTL];
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt) accL[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KSL; ++s) {
+      const int m = gq + 4 * s, f = m / NFP;
+      const double snf = f == 0 ? sn[0] : f == 1 ? sn[1] : f == 2 ? sn[2] : sn[3];
+      const double fsf = f == 0 ? fs[0] : f == 1 ? fs[1] : f == 2 ? fs[2] : fs[3];
+      double gm = 0.0;
+      if (valid) {
+        const double uM = su[sFM[m]], up = uP[s];
+        const double fM = 0.5 * uM * uM, fP = 0.5 * up * up;
+        const double al = fmax(fabs(uM), fabs(up));
+        gm = fsf * (0.5 * (fM - fP) * snf + 0.5 * al * fabs(snf) * (up - uM));
+      }
+#pragma unroll
+      for (int tt = 0; tt < MTL; ++tt)
+        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Al[tt][s], gm, accL[tt], 0, 0, 0);
+    }
+    // Results to LDS: rows 0..3Np-1 volume, 3Np..4Np-1 lift; column = element.
+#pragma unroll
+    for (int tt = 0; tt < MTV; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * tt + gq + 4 * r;
+        if (row < 3 * NP) sCw[row * 16 + c] = accV[tt][r];
+      }
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * tt + gq + 4 * r;
+        if (row < NP) sCw[(3 * NP + row) * 16 + c] = accL[tt][r];
+      }
+    __syncthreads();
+    if (valid) {
+      const double ga = p.bgeo[0 * K + g], gb = p.bgeo[1 * K + g], gc = p.bgeo[2 * K + g];
+#pragma unroll
+      for (int q = 0; q < (NP + 3) / 4; ++q) {
+        const int i = gq + 4 * q;
+        if (i < NP) {
+          const double rhs = -(ga * sCw[i * 16 + c] + gb * sCw[(NP + i) * 16 + c] +
+                               gc * sCw[(2 * NP + i) * 16 + c]) +
+                             sCw[(3 * NP + i) * 16 + c];
+          sU[el * S + i] = p.mode == 0 ? fma(dt, rhs, su[i]) : rhs;
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt * NP; i += 256) {
+      int ee = i / NP, n = i - ee * NP;
+      double o = sU[ee * S + n];
+      if (p.mode == 0) {
+        o = p.a1 * o;
+        if (p.a0 != 0.0) o = fma(p.a0, p.u0[tile.vbase + i], o);
+      }
+      p.out[tile.vbase + i] = o;
+      lmax = fmax(lmax, fabs(o));
+    }
+  }
+  if (p.maxabs) {
+    lmax = wave_max(lmax);
+    if (lane == 0 && lmax > 0.0) atomicMax(p.maxabs, (unsigned long long)__double_as_longlong(lmax));
   }
 }
 
@@ -348,32 +557,41 @@ struct dgk_dgmesh_s {
   uint8_t* face_nodes = nullptr;
   double *vtx = nullptr, *geo = nullptr, *bgeo = nullptr;
   int32_t *nbr = nullptr, *codes = nullptr;
-  Tile* tiles = nullptr;
+  Tile* tiles = nullptr;      // tiles of tb elements (VALU kernel)
   int ntiles = 0;
+  Tile* tiles64 = nullptr;    // tiles of 64 elements (MFMA kernel)
+  int ntiles64 = 0;
   int tb = 64, blocks_per_cu = 0, chunked = 1;
+  int variant = 0;            // 0 auto, 1 VALU, 2 MFMA
   unsigned long long* scratch = nullptr;  // reduction slot for synchronous helpers
 };
 
 namespace {
 
-int build_tiles(dgk_dgmesh m) {
+int build_tile_list(dgk_dgmesh m, int per_tile, Tile** out, int* nout) {
   std::vector<Tile> tl;
   int64_t g = 0;
   for (int p = 0; p < m->npart; ++p) {
-    for (int64_t f = 0; f < m->Kp[p]; f += m->tb) {
+    for (int64_t f = 0; f < m->Kp[p]; f += per_tile) {
       Tile t;
       t.vbase = m->state_off[p] + f * m->Np;
       t.gbase = (int32_t)(g + f);
-      t.count = (int32_t)std::min<int64_t>(m->tb, m->Kp[p] - f);
+      t.count = (int32_t)std::min<int64_t>(per_tile, m->Kp[p] - f);
       tl.push_back(t);
     }
     g += m->Kp[p];
   }
-  if (m->tiles) dgk_free(m->dev, m->tiles);
-  m->tiles = nullptr;
-  m->ntiles = (int)tl.size();
+  if (*out) dgk_free(m->dev, *out);
+  *out = nullptr;
+  *nout = (int)tl.size();
   if (tl.empty()) return DGK_OK;
-  return dgk_malloc(m->dev, tl.size() * sizeof(Tile), tl.data(), (void**)&m->tiles);
+  return dgk_malloc(m->dev, tl.size() * sizeof(Tile), tl.data(), (void**)out);
+}
+
+int build_tiles(dgk_dgmesh m) {
+  int rc = build_tile_list(m, m->tb, &m->tiles, &m->ntiles);
+  if (rc == DGK_OK) rc = build_tile_list(m, 64, &m->tiles64, &m->ntiles64);
+  return rc;
 }
 
 template <int N, int TB>
@@ -401,6 +619,24 @@ int launch_stage_n(dgk_dgmesh m, const StageParams& p) {
   }
 }
 
+template <int N>
+int launch_stage_mfma(dgk_dgmesh m, const StageParams& p) {
+  int grid = m->ntiles64;
+  int bpc = m->blocks_per_cu > 0 ? m->blocks_per_cu : 3;
+  int cap = m->dev->cu_count * bpc;
+  if (grid > cap) grid = cap;
+  if (grid < 1) return DGK_OK;
+  if (p.chunked && grid >= 8) grid -= grid % 8;
+  hipLaunchKernelGGL((burgers_stage_mfma_kernel<N>), dim3(grid), dim3(256), 0, m->dev->stream, p);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+bool use_mfma(dgk_dgmesh m, int mode) {
+  if (mode == 2 || m->N > 3) return false;
+  return m->variant != 1;
+}
+
 int launch_stage(dgk_dgmesh m, StageParams p) {
   p.Dr = m->Dr; p.Ds = m->Ds; p.Dt = m->Dt; p.LIFT = m->LIFT;
   p.face_nodes = m->face_nodes;
@@ -411,6 +647,15 @@ int launch_stage(dgk_dgmesh m, StageParams p) {
   p.ntiles = m->ntiles;
   p.chunked = m->chunked;
   p.K = m->K;
+  if (use_mfma(m, p.mode)) {
+    p.tiles = m->tiles64;
+    p.ntiles = m->ntiles64;
+    switch (m->N) {
+      case 1: return launch_stage_mfma<1>(m, p);
+      case 2: return launch_stage_mfma<2>(m, p);
+      case 3: return launch_stage_mfma<3>(m, p);
+    }
+  }
   switch (m->N) {
     case 1: return launch_stage_n<1>(m, p);
     case 2: return launch_stage_n<2>(m, p);
@@ -528,7 +773,7 @@ extern "C" int dgk_dgmesh_destroy(dgk_dgmesh m) {
   if (!m) return DGK_OK;
   dgk_device dev = m->dev;
   void* ptrs[] = {m->Dr, m->Ds, m->Dt, m->LIFT, m->rst, m->face_nodes, m->vtx, m->geo,
-                  m->bgeo, m->nbr, m->codes, m->tiles, m->scratch};
+                  m->bgeo, m->nbr, m->codes, m->tiles, m->tiles64, m->scratch};
   for (void* q : ptrs) dgk_free(dev, q);
   delete m;
   return DGK_OK;
@@ -625,6 +870,15 @@ extern "C" int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double
   DGK_REQUIRE(dev && maxabs_dev && dt_dev, "dgk_cfl_dt: NULL argument");
   hipLaunchKernelGGL(cfl_dt_kernel, dim3(1), dim3(64), 0, dev->stream, maxabs_dev, cfl, hmin, dt_dev);
   DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+extern "C" int dgk_burgers_set_variant(dgk_dgmesh m, int variant) {
+  dgk::clear_error();
+  DGK_REQUIRE(m, "dgk_burgers_set_variant: NULL mesh");
+  DGK_REQUIRE(variant >= 0 && variant <= 2, "dgk_burgers_set_variant: variant %d", variant);
+  DGK_REQUIRE(variant != 2 || m->N <= 3, "dgk_burgers_set_variant: MFMA stage built for N <= 3");
+  m->variant = variant;
   return DGK_OK;
 }
 

@@ -237,6 +237,36 @@ matmul_kernel(PartTiles pt, int per_tile, int rows, int cols, const double* __re
   }
 }
 
+// OUT[e*rows + i] = s[e*rows + i] * sum_j A(i,j) IN[e*cols + j], A column-major
+// (the runner's device matrix layout); s may be NULL (plain MATMUL).
+template <typename OFF, typename ST>
+__global__ void __launch_bounds__(256)
+scaled_matmul_kernel(PartTiles pt, int per_tile, int rows, int cols, const double* __restrict__ A,
+                     const double* __restrict__ in, const OFF* __restrict__ in_off,
+                     const ST* __restrict__ sc, const OFF* __restrict__ sc_off,
+                     double* __restrict__ out, const OFF* __restrict__ out_off) {
+  int part, first, count;
+  find_tile(pt, blockIdx.x, per_tile, part, first, count);
+  int64_t ib = (int64_t)in_off[part] + (int64_t)first * cols;
+  int64_t ob = (int64_t)out_off[part] + (int64_t)first * rows;
+  int64_t sb = sc ? (int64_t)sc_off[part] + (int64_t)first * rows : 0;
+  for (int w = threadIdx.x; w < count * rows; w += blockDim.x) {
+    int e = w / rows, i = w - e * rows;
+    double s = 0.0;
+    for (int j = 0; j < cols; ++j) s = fma(A[(int64_t)j * rows + i], in[ib + (int64_t)e * cols + j], s);
+    if (sc) s *= (double)sc[sb + w];
+    out[ob + w] = s;
+  }
+}
+
+__global__ void transpose_kernel(int rows, int cols, const double* __restrict__ in_colmajor,
+                                 double* __restrict__ out_rowmajor) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= rows * cols) return;
+  int i = w / cols, j = w - i * cols;
+  out_rowmajor[w] = in_colmajor[(int64_t)j * rows + i];
+}
+
 }  // namespace
 
 namespace dgk {
@@ -289,6 +319,39 @@ int matmul_any(dgk_device dev, int rows, int cols, const double* A, int npart, c
   return DGK_OK;
 }
 
+int scaled_matmul_any(dgk_device dev, int rows, int cols, const double* A_colmajor, int npart,
+                      const int64_t* K, const double* in, const void* in_off, const void* sc,
+                      int sc_type, const void* sc_off, double* out, const void* out_off,
+                      int off_size) {
+  DGK_REQUIRE(dev && A_colmajor && K && in && in_off && out && out_off, "matmul: NULL argument");
+  int per_tile = rows >= 256 ? 1 : 256 / rows;
+  PartTiles pt;
+  int ntiles = 0;
+  int rc = build_part_tiles(npart, K, per_tile, pt, ntiles);
+  if (rc) return rc;
+  if (!ntiles) return DGK_OK;
+#define DGK_GO(OFF, ST)                                                                          \
+  hipLaunchKernelGGL((scaled_matmul_kernel<OFF, ST>), dim3(ntiles), dim3(256), 0, dev->stream, pt, \
+                     per_tile, rows, cols, A_colmajor, in, (const OFF*)in_off, (const ST*)sc,      \
+                     (const OFF*)sc_off, out, (const OFF*)out_off)
+  if (off_size == 8) {
+    if (sc_type == DGK_F32) DGK_GO(int64_t, float); else DGK_GO(int64_t, double);
+  } else {
+    if (sc_type == DGK_F32) DGK_GO(int32_t, float); else DGK_GO(int32_t, double);
+  }
+#undef DGK_GO
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+int transpose_colmajor(dgk_device dev, int rows, int cols, const double* in, double* out) {
+  int n = rows * cols;
+  hipLaunchKernelGGL(transpose_kernel, dim3((n + 255) / 256), dim3(256), 0, dev->stream, rows, cols,
+                     in, out);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
 }  // namespace dgk
 
 extern "C" int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int64_t* K_host,
@@ -311,4 +374,13 @@ extern "C" int dgk_matmul(dgk_device dev, int rows, int cols, const double* A_co
   dgk::clear_error();
   return dgk::matmul_any(dev, rows, cols, A_colmajor_dev, npart, K_host, in_dev, in_off_dev, out_dev,
                          out_off_dev, add, 8);
+}
+
+extern "C" int dgk_phys_deriv_affine(dgk_device dev, int N, int64_t K, const double* Dr_dev,
+                                     const double* Ds_dev, const double* Dt_dev, const double* u_dev,
+                                     const double* geo_dev, void* dudx_dev, void* dudy_dev,
+                                     void* dudz_dev, int prec) {
+  dgk::clear_error();
+  dgk::set_error("dgk_phys_deriv_affine: not built yet");
+  return DGK_ERR_UNSUPPORTED;
 }

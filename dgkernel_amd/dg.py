@@ -157,6 +157,10 @@ class DGMesh:
     def set_tile(self, elems_per_block: int, blocks_per_cu: int = 0) -> None:
         _lib.call("dgk_burgers_set_tile", self.h, elems_per_block, blocks_per_cu)
 
+    def set_variant(self, variant: int) -> None:
+        """0 automatic, 1 VALU stage kernel, 2 FP64 MFMA stage kernel."""
+        _lib.call("dgk_burgers_set_variant", self.h, variant)
+
     def trace_plus(self, u: _lib.DeviceBuffer, out: _lib.DeviceBuffer) -> None:
         _lib.call("dgk_dgmesh_trace_plus", self.h, u.ptr, out.ptr)
 

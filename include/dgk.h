@@ -122,6 +122,11 @@ typedef struct {
 int dgk_kernel_list(char* buf, size_t size);
 int dgk_kernel_get(dgk_device dev, const char* name, const char* signature,
                    const dgk_kernel_consts* consts, dgk_kernel* out);
+/* A static matrix the runner embeds in the OKL preamble instead of passing it
+ * (runner/builder/builder.go:215-277, MATMUL_<name> with a static matrix):
+ * the shim hands its values over once, rows x cols, row-major. */
+int dgk_kernel_set_matrix(dgk_kernel k, const char* name, int rows, int cols,
+                          const double* values_rowmajor);
 /* Enqueue one launch. host_K: the partition sizes (the runner's K array,
  * npart values of int_size bytes) -- used to size the launch without a
  * device read. args: in signature order. */
@@ -262,9 +267,13 @@ int dgk_burgers_rhs(dgk_dgmesh m, const double* u_dev, double* rhs_dev);
 int dgk_max_abs(dgk_dgmesh m, const double* u_dev, double* maxabs_dev);
 /* dt = cfl * hmin / *maxabs_dev into *dt_dev (FP64), then *maxabs_dev = 0. */
 int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double hmin, double* dt_dev);
-/* Tile shape of the fused stage: elements per block (64, 128 or 256) and
- * blocks per CU of the persistent grid. 0 restores the tuned default. */
+/* Tile shape of the fused stage: elements per block (64, 128 or 256; VALU
+ * kernel) and blocks per CU of the persistent grid (both kernels). 0 restores
+ * the tuned default. */
 int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu);
+/* Stage kernel: 0 automatic (FP64 MFMA for N <= 3, else VALU), 1 VALU (one
+ * element per thread, FP64 FMA with scalar operator operands), 2 FP64 MFMA. */
+int dgk_burgers_set_variant(dgk_dgmesh m, int variant);
 
 #ifdef __cplusplus
 }

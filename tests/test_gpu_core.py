@@ -236,20 +236,26 @@ def test_geometry_and_hmin_vs_oracle(device):
     assert abs(mesh.min_node_distance() - o.min_node_distance()) <= 1e-15
 
 
+@pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("N,n,parts", [(2, 8, [768] * 4), (3, 5, [200, 350, 200]), (1, 6, [1296]),
-                                       (4, 3, [100, 62])])
-def test_burgers_rhs_vs_oracle(device, N, n, parts):
+                                       (4, 3, [100, 62]), (3, 4, [1, 63, 64, 65, 191])])
+def test_burgers_rhs_vs_oracle(device, N, n, parts, variant):
+    """VALU (1) and FP64-MFMA (2) stage kernels against the oracle's L(u)."""
+    if variant == 2 and N > 3:
+        pytest.skip("MFMA stage kernel is built for N <= 3")
     m = M.cube_mesh(n, n, n, periodic=True)
     st = dg.make_setup(m, N, Kp=parts)
     u = smooth_u0(st, noise=1e-3)
     o = O.Setup.from_mesh(N, m)
     want = o.burgers_rhs(u)
     solver = dg.BurgersSolver(device, st)
+    solver.mesh.set_variant(variant)
     got = solver.rhs(u)
     assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
 
 
-def test_config1_burgers_100_steps_vs_oracle(device):
+@pytest.mark.parametrize("variant", [1, 2])
+def test_config1_burgers_100_steps_vs_oracle(device, variant):
     """Config 1: 8^3 hexes x 6, P=2, 4 slabs, LF + SSP-RK3, CFL 0.3, 100 steps."""
     m = M.cube_mesh(8, 8, 8, periodic=True)
     st = dg.make_setup(m, 2, Kp=M.slab_partition(m.K, 4))
@@ -258,6 +264,7 @@ def test_config1_burgers_100_steps_vs_oracle(device):
     hmin = o.min_node_distance()
     want, dts = o.ssprk3(u0, 100, 0.3, hmin)
     solver = dg.BurgersSolver(device, st, cfl=0.3, hmin=hmin)
+    solver.mesh.set_variant(variant)
     solver.set_state(u0)
     solver.step(100)
     got = solver.state()
@@ -282,16 +289,34 @@ def test_graph_replay_matches_eager(device):
         assert np.array_equal(a.state(), b.state()), f"replay {rep}"
 
 
-@pytest.mark.parametrize("tb,bpc", [(64, 0), (128, 0), (256, 0), (64, 3)])
-def test_tile_shapes_agree(device, tb, bpc):
+@pytest.mark.parametrize("variant,tb,bpc", [(1, 64, 0), (1, 128, 0), (1, 256, 0), (1, 64, 3),
+                                            (2, 64, 1), (2, 64, 0), (2, 64, 5)])
+def test_tile_shapes_agree(device, variant, tb, bpc):
+    """Tile shape and grid size do not change the result of a kernel variant."""
     m = M.cube_mesh(6, 5, 4, periodic=True)
     st = dg.make_setup(m, 3, Kp=[333, 387])
     u0 = smooth_u0(st)
     ref = dg.BurgersSolver(device, st)
+    ref.mesh.set_variant(variant)
     ref.set_state(u0)
     ref.step(3)
     s = dg.BurgersSolver(device, st, hmin=ref.hmin)
+    s.mesh.set_variant(variant)
     s.mesh.set_tile(tb, bpc)
     s.set_state(u0)
     s.step(3)
     assert np.array_equal(ref.state(), s.state())
+
+
+def test_mfma_and_valu_stage_agree(device):
+    m = M.cube_mesh(6, 6, 6, periodic=True)
+    st = dg.make_setup(m, 3, Kp=M.slab_partition(m.K, 5, 0.4, seed=3))
+    u0 = smooth_u0(st)
+    out = []
+    for v in (1, 2):
+        s = dg.BurgersSolver(device, st)
+        s.mesh.set_variant(v)
+        s.set_state(u0)
+        s.step(10)
+        out.append(s.state())
+    assert rel_l2(out[1], out[0]) <= 1e-14
