@@ -62,11 +62,13 @@ def read_traffic(name: str):
     return None
 
 
-def build_c3(n: int, N: int, nparts: int, seed: int):
+def build_c3(n: int, N: int, nparts: int, seed: int, morton: bool = False):
     from dgkernel_amd import dg
     from dgkernel_amd import mesh as M
     m = M.cube_mesh(n, n, n, periodic=True)
-    Kp = M.slab_partition(m.K, nparts, 0.4, seed=seed) if nparts > 1 else None
+    Kp = M.slab_partition(m.K, nparts, 0.4, seed=seed) if nparts > 1 else np.array([m.K])
+    if morton:
+        m = M.morton_within_partitions(m, Kp)
     st = dg.make_setup(m, N, Kp=Kp)
     x, y, z = st.node_coords()
     u0 = 0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
@@ -120,6 +122,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (PMC calibration)")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -128,6 +132,15 @@ def main(argv=None) -> int:
         return 2
 
     from dgkernel_amd import _lib, dg
+
+    if a.copy_calib:
+        dev = _lib.Device()
+        src, dst = dev.zeros(a.copy_calib // 8), dev.zeros(a.copy_calib // 8)
+        for _ in range(3):
+            src.copy_to(dst)
+        dev.sync()
+        print(json.dumps({"copy_calib_bytes": a.copy_calib}), flush=True)
+        return 0
 
     t_setup = time.perf_counter()
     m, st, u0 = build_c3(a.n, a.order, a.parts, a.seed)

@@ -307,94 +307,136 @@ __global__ void __launch_bounds__(TB) burgers_stage_kernel(StageParams p) {
 //
 // A block of 4 waves owns a tile of 64 elements, 16 per wave (the MFMA N
 // dimension). Per wave, in FP64 throughout:
-//   C_vol (3Np x 16)  = [Dr; Ds; Dt] (3Np x Np) * F (Np x 16),  F = u^2/2
-//   C_lift (Np x 16)  = LIFT (Np x 4Nfp) * G (4Nfp x 16),      G = flux jumps
-// The operator fragments (A operands) are loaded once per block and stay in
-// registers for every tile the block processes; F and G (B operands) are
-// formed lane by lane from the LDS copy of the tile and the neighbour
-// gather. Both products land in LDS, where the chain rule with the folded
-// affine metrics and the RK combination read them back.
+//   C_vol  = [Dr; Ds; Dt] * F,  F = u^2/2       (3 NPR x 16, K = Np)
+//   C_lift = LIFT * G,          G = flux jumps  (Np x 16,    K = 4 Nfp)
+// where the three derivative blocks are stacked at row offsets 0, NPR, 2 NPR
+// with NPR = Np rounded up to a multiple of 4.
 //
 // Operand layout of the 16x16x4 FP64 MFMA (wave64), checked on gfx950: lane
 // l provides A[l%16][l/16] and B[l/16][l%16] of the 16x4 / 4x16 operands and
 // holds D[l/16 + 4r][l%16], r = 0..3, of the 16x16 result (rows interleaved
-// by 4, unlike the FP32 16x16 layouts).
+// by 4). With blocks stacked at multiples of 4 rows, the lane that holds row
+// i of Dr F also holds rows i of Ds F, Dt F and LIFT G for the same element,
+// so the chain rule with the folded affine metrics and the RK combination
+// happen in registers.
+//
+// The operator fragments are written to LDS once per block, in per-lane
+// order, and read back with one conflict-free ds_read_b64 per MFMA. The next
+// tile's nodal values and the current tile's u0 are loaded while the
+// current tile computes.
 // ---------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 template <int N>
 struct MfmaShape {
   static constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP;
+  static constexpr int NPR = (NP + 3) / 4 * 4;       // row offset between Dr, Ds, Dt blocks
   static constexpr int KSV = (NP + 3) / 4;           // K steps of the volume product
-  static constexpr int MTV = (3 * NP + 15) / 16;     // 16-row tiles of [Dr; Ds; Dt]
+  static constexpr int MTV = (3 * NPR + 15) / 16;    // 16-row tiles of the stacked operator
   static constexpr int MTL = (NP + 15) / 16;         // 16-row tiles of LIFT
   static constexpr int KSL = NF / 4;                 // K steps of the lift product
+  static constexpr int QN = NPR / 4;                 // result rows per lane
   static constexpr int S = NP + 1;                   // LDS row stride of the u tile
-  static constexpr int CROWS = 4 * NP;               // C_vol (3Np) + C_lift (Np) rows kept
+  static constexpr int NA = (MTV * KSV + MTL * KSL) * 64;  // operator fragments (doubles)
 };
 
-template <int N>
-__global__ void __launch_bounds__(256) burgers_stage_mfma_kernel(StageParams p) {
+// LDS writes of one lane made visible to the other lanes of the same wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Each wave works through its own sequence of 16-element tiles; the waves of a
+// block only share the operator fragments (written once, one block barrier).
+template <int N, int WPB>
+__global__ void __launch_bounds__(64 * WPB) burgers_stage_mfma_kernel(StageParams p) {
   using SH = MfmaShape<N>;
-  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, S = SH::S;
-  constexpr int KSV = SH::KSV, MTV = SH::MTV, MTL = SH::MTL, KSL = SH::KSL;
-  constexpr int E = 64;
+  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, S = SH::S, NPR = SH::NPR;
+  constexpr int KSV = SH::KSV, MTV = SH::MTV, MTL = SH::MTL, KSL = SH::KSL, QN = SH::QN;
+  constexpr int NT = 64 * WPB;               // threads per block
+  constexpr int E = 16;                      // elements per wave tile
+  constexpr int PV = (E * NP + 63) / 64;     // tile values per lane
   constexpr FmaskTable<N> FM{};
-  __shared__ __attribute__((aligned(16))) double sU[E * S];
-  __shared__ __attribute__((aligned(16))) double sC[4][SH::CROWS * 16];
+  __shared__ __attribute__((aligned(16))) double sA[SH::NA];
+  __shared__ __attribute__((aligned(16))) double sUall[WPB][E * S];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[NF];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int c = lane & 15, gq = lane >> 4;
-  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += 256) sFN[i] = p.face_nodes[i];
-  if (threadIdx.x < NF) sFM[threadIdx.x] = (uint8_t)FM.v[threadIdx.x / NFP][threadIdx.x % NFP];
-
-  // Operator fragments, resident for the block's lifetime.
-  double Av[MTV][KSV], Al[MTL][KSL];
-#pragma unroll
-  for (int t = 0; t < MTV; ++t)
-#pragma unroll
-    for (int s = 0; s < KSV; ++s) {
-      const int row = 16 * t + c, k = 4 * s + gq;
-      double v = 0.0;
-      if (k < NP) {
-        if (row < NP) v = p.Dr[row * NP + k];
-        else if (row < 2 * NP) v = p.Ds[(row - NP) * NP + k];
-        else if (row < 3 * NP) v = p.Dt[(row - 2 * NP) * NP + k];
-      }
-      Av[t][s] = v;
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) sFN[i] = p.face_nodes[i];
+  for (int i = threadIdx.x; i < NF; i += NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
+  // Operator fragments: entry (tile, kstep, lane) in lane order.
+  for (int q = threadIdx.x; q < SH::NA; q += NT) {
+    const int l = q & 63, ts_ = q >> 6, cl = l & 15, gl = l >> 4;
+    double v = 0.0;
+    if (ts_ < MTV * KSV) {
+      const int tt = ts_ / KSV, s = ts_ % KSV;
+      const int row = 16 * tt + cl, k = 4 * s + gl, blk = row / NPR, i = row % NPR;
+      if (k < NP && blk < 3 && i < NP) v = (blk == 0 ? p.Dr : blk == 1 ? p.Ds : p.Dt)[i * NP + k];
+    } else {
+      const int r2 = ts_ - MTV * KSV, tt = r2 / KSL, s = r2 % KSL;
+      const int row = 16 * tt + cl, k = 4 * s + gl;
+      if (row < NP) v = p.LIFT[row * NF + k];
     }
-#pragma unroll
-  for (int t = 0; t < MTL; ++t)
-#pragma unroll
-    for (int s = 0; s < KSL; ++s) {
-      const int row = 16 * t + c, k = 4 * s + gq;
-      Al[t][s] = row < NP ? p.LIFT[row * NF + k] : 0.0;
-    }
-
+    sA[q] = v;
+  }
+  __syncthreads();
   const double dt = p.mode == 0 ? *p.dt : 0.0;
   const int64_t K = p.K;
   double lmax = 0.0;
-  const int G = gridDim.x, b = blockIdx.x;
-  const int vb = (p.chunked && G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
-  const int t0 = p.chunked ? (int)((int64_t)p.ntiles * vb / G) : b;
-  const int t1 = p.chunked ? (int)((int64_t)p.ntiles * (vb + 1) / G) : p.ntiles;
-  const int ts = p.chunked ? 1 : G;
-  double* sCw = sC[w];
+  // Wave-level persistent schedule.
+  const int NW = gridDim.x * WPB;
+  const int gw = blockIdx.x * WPB + w;
+  const int vw = (p.chunked && NW % 8 == 0) ? (gw % 8) * (NW / 8) + gw / 8 : gw;
+  const int t0 = p.chunked ? (int)((int64_t)p.ntiles * vw / NW) : gw;
+  const int t1 = p.chunked ? (int)((int64_t)p.ntiles * (vw + 1) / NW) : p.ntiles;
+  const int ts = p.chunked ? 1 : NW;
+  const double* sAv = sA;
+  const double* sAl = sA + MTV * KSV * 64;
+  double* sU = sUall[w];
+
+  Tile tile;
+  tile.vbase = 0; tile.gbase = 0; tile.count = 0;
+  if (t0 < t1) tile = p.tiles[t0];
+  double pre[PV];
+#pragma unroll
+  for (int j = 0; j < PV; ++j) {
+    const int i = lane + 64 * j;
+    pre[j] = i < tile.count * NP ? p.u[tile.vbase + i] : 0.0;
+  }
   for (int t = t0; t < t1; t += ts) {
-    const Tile tile = p.tiles[t];
-    const int cnt = tile.count;
-    __syncthreads();
-    for (int i = threadIdx.x; i < cnt * NP; i += 256) {
-      int e = i / NP, n = i - e * NP;
-      sU[e * S + n] = p.u[tile.vbase + i];
+    const Tile cur = tile;
+    const int cnt = cur.count;
+    wave_sync();  // this wave's previous readers of sU are done
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      if (i < cnt * NP) {
+        const int e = i / NP, n = i - e * NP;
+        sU[e * S + n] = pre[j];
+      }
     }
-    __syncthreads();
-    const int el = 16 * w + c;  // element of this lane's MFMA column
-    const bool valid = el < cnt;
-    const int64_t g = tile.gbase + (valid ? el : 0);
-    const double* su = sU + el * S;
-    // Neighbour data of the lane's element (replicated across the 4 lane groups).
+    // u0 of this tile and the next tile's u: in flight while this tile computes.
+    double u0v[PV];
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      u0v[j] = (p.mode == 0 && p.a0 != 0.0 && i < cnt * NP)
+                   ? __builtin_nontemporal_load(p.u0 + cur.vbase + i)
+                   : 0.0;
+    }
+    {
+      const int tn = t + ts;
+      if (tn < t1) tile = p.tiles[tn]; else tile.count = 0;
+#pragma unroll
+      for (int j = 0; j < PV; ++j) {
+        const int i = lane + 64 * j;
+        pre[j] = i < tile.count * NP ? p.u[tile.vbase + i] : 0.0;
+      }
+    }
+    const bool valid = c < cnt;
+    const int64_t g = cur.gbase + (valid ? c : 0);
     const int packed = valid ? p.codes[g] : 0;
     int32_t nb[4];
     double sn[4], fs[4];
@@ -404,7 +446,9 @@ __global__ void __launch_bounds__(256) burgers_stage_mfma_kernel(StageParams p) 
       sn[f] = valid ? p.bgeo[(3 + f) * K + g] : 0.0;
       fs[f] = valid ? p.bgeo[(7 + f) * K + g] : 0.0;
     }
-    // Gather u+ for this lane's face points m = gq + 4s.
+    const double ga = valid ? p.bgeo[0 * K + g] : 0.0;
+    const double gb = valid ? p.bgeo[1 * K + g] : 0.0;
+    const double gc = valid ? p.bgeo[2 * K + g] : 0.0;
     double uP[KSL];
 #pragma unroll
     for (int s = 0; s < KSL; ++s) {
@@ -413,7 +457,8 @@ __global__ void __launch_bounds__(256) burgers_stage_mfma_kernel(StageParams p) 
       const int32_t base = f == 0 ? nb[0] : f == 1 ? nb[1] : f == 2 ? nb[2] : nb[3];
       uP[s] = valid ? p.u[base + sFN[(f * 24 + code) * NFP + fp]] : 0.0;
     }
-    // Volume product.
+    wave_sync();  // sU holds this tile
+    const double* su = sU + c * S;
     double4_t accV[MTV];
 #pragma unroll
     for (int tt = 0; tt < MTV; ++tt) accV[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
@@ -427,9 +472,9 @@ __global__ void __launch_bounds__(256) burgers_stage_mfma_kernel(StageParams p) 
       }
 #pragma unroll
       for (int tt = 0; tt < MTV; ++tt)
-        accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Av[tt][s], Fk, accV[tt], 0, 0, 0);
+        accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAv[(tt * KSV + s) * 64 + lane], Fk,
+                                                         accV[tt], 0, 0, 0);
     }
-    // Flux jumps and the lift product.
     double4_t accL[MTL];
 #pragma unroll
     for (int tt = 0; tt < MTL; ++tt) accL[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
@@ -447,47 +492,41 @@ __global__ void __launch_bounds__(256) burgers_stage_mfma_kernel(StageParams p) 
       }
 #pragma unroll
       for (int tt = 0; tt < MTL; ++tt)
-        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Al[tt][s], gm, accL[tt], 0, 0, 0);
+        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,
+                                                         accL[tt], 0, 0, 0);
     }
-    // Results to LDS: rows 0..3Np-1 volume, 3Np..4Np-1 lift; column = element.
+    // Rows i = gq + 4q of this lane's element: Dr F at row i, Ds F at NPR + i,
+    // Dt F at 2 NPR + i, LIFT G at i; tile and slot are compile-time.
+    double vout[QN];
 #pragma unroll
-    for (int tt = 0; tt < MTV; ++tt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * tt + gq + 4 * r;
-        if (row < 3 * NP) sCw[row * 16 + c] = accV[tt][r];
-      }
-#pragma unroll
-    for (int tt = 0; tt < MTL; ++tt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * tt + gq + 4 * r;
-        if (row < NP) sCw[(3 * NP + row) * 16 + c] = accL[tt][r];
-      }
-    __syncthreads();
+    for (int q = 0; q < QN; ++q) {
+      const double ur = accV[(4 * q) / 16][((4 * q) % 16) / 4];
+      const double us = accV[(NPR + 4 * q) / 16][((NPR + 4 * q) % 16) / 4];
+      const double ut = accV[(2 * NPR + 4 * q) / 16][((2 * NPR + 4 * q) % 16) / 4];
+      const double lf = (4 * q) / 16 < MTL ? accL[((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;
+      const double rhs = -(ga * ur + gb * us + gc * ut) + lf;
+      const int i = gq + 4 * q;
+      vout[q] = (i < NP && p.mode == 0) ? fma(dt, rhs, su[i < NP ? i : 0]) : rhs;
+    }
+    wave_sync();  // all reads of sU as u are done
     if (valid) {
-      const double ga = p.bgeo[0 * K + g], gb = p.bgeo[1 * K + g], gc = p.bgeo[2 * K + g];
 #pragma unroll
-      for (int q = 0; q < (NP + 3) / 4; ++q) {
+      for (int q = 0; q < QN; ++q) {
         const int i = gq + 4 * q;
-        if (i < NP) {
-          const double rhs = -(ga * sCw[i * 16 + c] + gb * sCw[(NP + i) * 16 + c] +
-                               gc * sCw[(2 * NP + i) * 16 + c]) +
-                             sCw[(3 * NP + i) * 16 + c];
-          sU[el * S + i] = p.mode == 0 ? fma(dt, rhs, su[i]) : rhs;
-        }
+        if (i < NP) sU[c * S + i] = vout[q];
       }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < cnt * NP; i += 256) {
-      int ee = i / NP, n = i - ee * NP;
-      double o = sU[ee * S + n];
-      if (p.mode == 0) {
-        o = p.a1 * o;
-        if (p.a0 != 0.0) o = fma(p.a0, p.u0[tile.vbase + i], o);
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      if (i < cnt * NP) {
+        const int e = i / NP, n = i - e * NP;
+        double o = sU[e * S + n];
+        if (p.mode == 0) o = fma(p.a0, u0v[j], p.a1 * o);
+        __builtin_nontemporal_store(o, p.out + cur.vbase + i);
+        lmax = fmax(lmax, fabs(o));
       }
-      p.out[tile.vbase + i] = o;
-      lmax = fmax(lmax, fabs(o));
     }
   }
   if (p.maxabs) {
@@ -559,9 +598,10 @@ struct dgk_dgmesh_s {
   int32_t *nbr = nullptr, *codes = nullptr;
   Tile* tiles = nullptr;      // tiles of tb elements (VALU kernel)
   int ntiles = 0;
-  Tile* tiles64 = nullptr;    // tiles of 64 elements (MFMA kernel)
+  Tile* tiles64 = nullptr;    // tiles of 16 elements (MFMA kernel, one per wave step)
   int ntiles64 = 0;
-  int tb = 64, blocks_per_cu = 0, chunked = 1;
+  int tbm = 64;               // MFMA block: 16 elements per wave, 1/2/4 waves per block
+  int tb = 64, blocks_per_cu = 0, chunked = 0;
   int variant = 0;            // 0 auto, 1 VALU, 2 MFMA
   unsigned long long* scratch = nullptr;  // reduction slot for synchronous helpers
 };
@@ -590,7 +630,7 @@ int build_tile_list(dgk_dgmesh m, int per_tile, Tile** out, int* nout) {
 
 int build_tiles(dgk_dgmesh m) {
   int rc = build_tile_list(m, m->tb, &m->tiles, &m->ntiles);
-  if (rc == DGK_OK) rc = build_tile_list(m, 64, &m->tiles64, &m->ntiles64);
+  if (rc == DGK_OK) rc = build_tile_list(m, 16, &m->tiles64, &m->ntiles64);
   return rc;
 }
 
@@ -619,21 +659,31 @@ int launch_stage_n(dgk_dgmesh m, const StageParams& p) {
   }
 }
 
-template <int N>
-int launch_stage_mfma(dgk_dgmesh m, const StageParams& p) {
-  int grid = m->ntiles64;
-  int bpc = m->blocks_per_cu > 0 ? m->blocks_per_cu : 3;
+template <int N, int WPB>
+int launch_stage_mfma_w(dgk_dgmesh m, const StageParams& p) {
+  int grid = (p.ntiles + WPB - 1) / WPB;
+  int bpc = m->blocks_per_cu > 0 ? m->blocks_per_cu : (WPB == 4 ? 3 : WPB == 2 ? 6 : 8);
   int cap = m->dev->cu_count * bpc;
   if (grid > cap) grid = cap;
   if (grid < 1) return DGK_OK;
   if (p.chunked && grid >= 8) grid -= grid % 8;
-  hipLaunchKernelGGL((burgers_stage_mfma_kernel<N>), dim3(grid), dim3(256), 0, m->dev->stream, p);
+  hipLaunchKernelGGL((burgers_stage_mfma_kernel<N, WPB>), dim3(grid), dim3(64 * WPB), 0,
+                     m->dev->stream, p);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
+template <int N>
+int launch_stage_mfma(dgk_dgmesh m, const StageParams& p) {
+  switch (m->tbm) {
+    case 16: return launch_stage_mfma_w<N, 1>(m, p);
+    case 32: return launch_stage_mfma_w<N, 2>(m, p);
+    default: return launch_stage_mfma_w<N, 4>(m, p);
+  }
+}
+
 bool use_mfma(dgk_dgmesh m, int mode) {
-  if (mode == 2 || m->N > 3) return false;
+  if (mode == 2 || m->N > 4) return false;
   return m->variant != 1;
 }
 
@@ -654,6 +704,7 @@ int launch_stage(dgk_dgmesh m, StageParams p) {
       case 1: return launch_stage_mfma<1>(m, p);
       case 2: return launch_stage_mfma<2>(m, p);
       case 3: return launch_stage_mfma<3>(m, p);
+      case 4: return launch_stage_mfma<4>(m, p);
     }
   }
   switch (m->N) {
@@ -873,11 +924,18 @@ extern "C" int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double
   return DGK_OK;
 }
 
+extern "C" int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked) {
+  dgk::clear_error();
+  DGK_REQUIRE(m, "dgk_burgers_set_order: NULL mesh");
+  m->chunked = xcd_chunked ? 1 : 0;
+  return DGK_OK;
+}
+
 extern "C" int dgk_burgers_set_variant(dgk_dgmesh m, int variant) {
   dgk::clear_error();
   DGK_REQUIRE(m, "dgk_burgers_set_variant: NULL mesh");
   DGK_REQUIRE(variant >= 0 && variant <= 2, "dgk_burgers_set_variant: variant %d", variant);
-  DGK_REQUIRE(variant != 2 || m->N <= 3, "dgk_burgers_set_variant: MFMA stage built for N <= 3");
+  DGK_REQUIRE(variant != 2 || m->N <= 4, "dgk_burgers_set_variant: MFMA stage built for N <= 4");
   m->variant = variant;
   return DGK_OK;
 }
@@ -885,11 +943,17 @@ extern "C" int dgk_burgers_set_variant(dgk_dgmesh m, int variant) {
 extern "C" int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu) {
   dgk::clear_error();
   DGK_REQUIRE(m, "dgk_burgers_set_tile: NULL mesh");
-  int tb = elems_per_block == 0 ? 64 : elems_per_block;
-  DGK_REQUIRE(tb == 64 || tb == 128 || tb == 256, "dgk_burgers_set_tile: elems_per_block %d", tb);
   DGK_REQUIRE(blocks_per_cu >= 0 && blocks_per_cu <= 64, "dgk_burgers_set_tile: blocks_per_cu %d",
               blocks_per_cu);
   m->blocks_per_cu = blocks_per_cu;
+  if (use_mfma(m, 0)) {
+    int e = elems_per_block == 0 ? 64 : elems_per_block;
+    DGK_REQUIRE(e == 16 || e == 32 || e == 64, "dgk_burgers_set_tile: MFMA tile %d (16, 32, 64)", e);
+    m->tbm = e;
+    return DGK_OK;
+  }
+  int tb = elems_per_block == 0 ? 64 : elems_per_block;
+  DGK_REQUIRE(tb == 64 || tb == 128 || tb == 256, "dgk_burgers_set_tile: elems_per_block %d", tb);
   if (tb != m->tb) {
     m->tb = tb;
     return build_tiles(m);

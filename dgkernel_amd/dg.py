@@ -161,6 +161,9 @@ class DGMesh:
         """0 automatic, 1 VALU stage kernel, 2 FP64 MFMA stage kernel."""
         _lib.call("dgk_burgers_set_variant", self.h, variant)
 
+    def set_order(self, xcd_chunked: bool) -> None:
+        _lib.call("dgk_burgers_set_order", self.h, 1 if xcd_chunked else 0)
+
     def trace_plus(self, u: _lib.DeviceBuffer, out: _lib.DeviceBuffer) -> None:
         _lib.call("dgk_dgmesh_trace_plus", self.h, u.ptr, out.ptr)
 

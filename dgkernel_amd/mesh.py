@@ -396,5 +396,39 @@ def slab_partition(K: int, nparts: int, imbalance: float = 0.0, seed: int = 0) -
     return kp
 
 
+def _spread3(v: np.ndarray) -> np.ndarray:
+    """Interleave the low 21 bits of v with two zero bits (Morton encoding)."""
+    v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+    v = (v | (v << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return v
+
+
+def morton_within_partitions(mesh: TetMesh, Kp: np.ndarray) -> TetMesh:
+    """Renumber elements inside each contiguous partition along a Morton
+    (Z-order) curve of their centroids, keeping every partition's element
+    set. Face neighbours then sit close in memory, so the neighbour gathers
+    of the stage kernel hit in cache. Returns a new TetMesh (same vertices)."""
+    K = mesh.K
+    cx = mesh.VX[mesh.EToV].mean(axis=1)
+    cy = mesh.VY[mesh.EToV].mean(axis=1)
+    cz = mesh.VZ[mesh.EToV].mean(axis=1)
+    scale = float(1 << 20)
+    lo = np.array([cx.min(), cy.min(), cz.min()])
+    span = np.array([np.ptp(cx), np.ptp(cy), np.ptp(cz)]) + 1e-30
+    qx = ((cx - lo[0]) / span[0] * (scale - 1)).astype(np.uint64)
+    qy = ((cy - lo[1]) / span[1] * (scale - 1)).astype(np.uint64)
+    qz = ((cz - lo[2]) / span[2] * (scale - 1)).astype(np.uint64)
+    key = _spread3(qx) | (_spread3(qy) << np.uint64(1)) | (_spread3(qz) << np.uint64(2))
+    part = etop_from_counts(np.asarray(Kp))
+    order = np.lexsort((np.arange(K), key, part))
+    etop = None if mesh.EToP is None else mesh.EToP[order]
+    return TetMesh(mesh.VX, mesh.VY, mesh.VZ, mesh.EToV[order], mesh.EToVc[order], EToP=etop,
+                   period=mesh.period, info=dict(mesh.info, order="morton-within-partitions"))
+
+
 def etop_from_counts(kp: np.ndarray) -> np.ndarray:
     return np.repeat(np.arange(kp.size, dtype=np.int64), kp)

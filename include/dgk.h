@@ -267,13 +267,18 @@ int dgk_burgers_rhs(dgk_dgmesh m, const double* u_dev, double* rhs_dev);
 int dgk_max_abs(dgk_dgmesh m, const double* u_dev, double* maxabs_dev);
 /* dt = cfl * hmin / *maxabs_dev into *dt_dev (FP64), then *maxabs_dev = 0. */
 int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double hmin, double* dt_dev);
-/* Tile shape of the fused stage: elements per block (64, 128 or 256; VALU
- * kernel) and blocks per CU of the persistent grid (both kernels). 0 restores
- * the tuned default. */
+/* Tile shape of the selected stage kernel: elements per block (MFMA kernel:
+ * 16, 32 or 64, i.e. 1, 2 or 4 waves of 16 elements; VALU kernel: 64, 128
+ * or 256) and blocks per CU of the persistent grid. 0 restores the tuned
+ * default. Set the variant first. */
 int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu);
 /* Stage kernel: 0 automatic (FP64 MFMA for N <= 3, else VALU), 1 VALU (one
  * element per thread, FP64 FMA with scalar operator operands), 2 FP64 MFMA. */
 int dgk_burgers_set_variant(dgk_dgmesh m, int variant);
+/* Tile order of the persistent grid: 1 gives each XCD a contiguous range of
+ * tiles (blocks renumbered around the round-robin XCD dispatch), 0 (default)
+ * walks the tile list with a grid stride. */
+int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked);
 
 #ifdef __cplusplus
 }

@@ -241,8 +241,6 @@ def test_geometry_and_hmin_vs_oracle(device):
                                        (4, 3, [100, 62]), (3, 4, [1, 63, 64, 65, 191])])
 def test_burgers_rhs_vs_oracle(device, N, n, parts, variant):
     """VALU (1) and FP64-MFMA (2) stage kernels against the oracle's L(u)."""
-    if variant == 2 and N > 3:
-        pytest.skip("MFMA stage kernel is built for N <= 3")
     m = M.cube_mesh(n, n, n, periodic=True)
     st = dg.make_setup(m, N, Kp=parts)
     u = smooth_u0(st, noise=1e-3)
@@ -289,9 +287,10 @@ def test_graph_replay_matches_eager(device):
         assert np.array_equal(a.state(), b.state()), f"replay {rep}"
 
 
-@pytest.mark.parametrize("variant,tb,bpc", [(1, 64, 0), (1, 128, 0), (1, 256, 0), (1, 64, 3),
-                                            (2, 64, 1), (2, 64, 0), (2, 64, 5)])
-def test_tile_shapes_agree(device, variant, tb, bpc):
+@pytest.mark.parametrize("variant,tb,bpc,chunk", [(1, 64, 0, 1), (1, 128, 0, 1), (1, 256, 0, 0),
+                                                  (1, 64, 3, 1), (2, 64, 1, 1), (2, 64, 0, 0),
+                                                  (2, 16, 5, 1), (2, 32, 0, 0), (2, 16, 0, 0)])
+def test_tile_shapes_agree(device, variant, tb, bpc, chunk):
     """Tile shape and grid size do not change the result of a kernel variant."""
     m = M.cube_mesh(6, 5, 4, periodic=True)
     st = dg.make_setup(m, 3, Kp=[333, 387])
@@ -303,6 +302,7 @@ def test_tile_shapes_agree(device, variant, tb, bpc):
     s = dg.BurgersSolver(device, st, hmin=ref.hmin)
     s.mesh.set_variant(variant)
     s.mesh.set_tile(tb, bpc)
+    s.mesh.set_order(bool(chunk))
     s.set_state(u0)
     s.step(3)
     assert np.array_equal(ref.state(), s.state())
