@@ -235,7 +235,9 @@ def main(argv=None) -> int:
                    "kpart_max": int(st.Kp.max()), "stages_timed": 3 * a.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "burgers_stage_kernel", "bytes_per_elem": bpe,
+                     "kernel": (f"burgers_stage_mfma_kernel<{a.order},4>" if a.order <= 4
+                                else f"burgers_stage_kernel<{a.order},64>"),
+                     "bytes_per_elem": bpe,
                      "stage_ms_mean": float(stage_ms.mean()),
                      "fp64_tflops": flops, "fp64_frac": flops / FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
