@@ -106,6 +106,144 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def _time_ms(dev, fn, reps: int, warm: int = 2) -> float:
+    """Mean duration of ``fn`` (enqueued on the library stream) over ``reps`` calls, HIP events."""
+    for _ in range(warm):
+        fn()
+    dev.sync()
+    t = dev.timer()
+    t.start()
+    for _ in range(reps):
+        fn()
+    t.stop()
+    return t.ms() / reps
+
+
+def _errs(x, ref):
+    x = np.asarray(x, dtype=np.float64)
+    return {"rel_l2": float(np.linalg.norm(x - ref) / np.linalg.norm(ref)),
+            "max_abs": float(np.max(np.abs(x - ref))),
+            "max_rel": float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))}
+
+
+def bench_deriv(a, dev) -> dict:
+    """Physical derivative: per-node (reference layout) and affine MFMA forms.
+
+    c2: first 2^20 tets of a 64x64x43-hex cube, vertices jittered +-10% of h,
+        P=4, u ~ U[-1,1]; FP64 and FP32.
+    c4: 64^3 x 6 = 1,572,864 periodic tets, P=6, smooth field + U(+-1e-6);
+        FP64, FP32 and split BF16x3 (FP32 outputs for the last two).
+    """
+    import ctypes as C
+    from dgkernel_amd import _lib, dg
+    from dgkernel_amd import mesh as M
+    case = a.case
+    if case == "c2":
+        m = M.cube_mesh(64, 64, 43, jitter=0.1, seed=a.seed, kmax=2 ** 20)
+        N = 4
+    else:
+        m = M.cube_mesh(64, 64, 64, periodic=True)
+        N = 6
+    st = dg.make_setup(m, N, Kp=[m.K])
+    K, Np = m.K, st.ref.Np
+    if case == "c2":
+        u = M.uniform(a.seed, 1, K * Np, -1.0, 1.0)
+    else:
+        x, y, z = st.node_coords()
+        u = (0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+             ).reshape(-1) + M.uniform(a.seed, 7, K * Np, -1e-6, 1e-6)
+        del x, y, z
+    mesh = dg.DGMesh(dev, st)
+    geo_ptr = mesh.geometry_dev_ptr()
+    du = dev.upload(u)
+    D = (st.ref.Dr, st.ref.Ds, st.ref.Dt)
+    flops = (6.0 * Np * Np + 15.0 * Np) * K
+    out = {"metric": f"physical derivative ({case})", "tets": K, "order": N, "Np": Np,
+           "reps": a.reps, "kernels": {}}
+    ref64 = None
+    res = {}
+    precs = [("affine_f64", _lib.DGK_PREC_F64), ("affine_f32", _lib.DGK_PREC_F32),
+             ("affine_bf16x3", _lib.DGK_PREC_BF16X3)]
+    for name, prec in precs:
+        plan = dg.DerivPlan(dev, N, D, prec)
+        outs = [dev.empty(K * Np, plan.out_dtype) for _ in range(3)]
+        ms = _time_ms(dev, lambda: plan.apply(K, du, geo_ptr, K, outs), a.reps)
+        ob = 8 if prec == _lib.DGK_PREC_F64 else 4
+        nbytes = (Np * 8 + 3 * Np * ob + 72) * K
+        got = [o.to_host().astype(np.float64) for o in outs]
+        if ref64 is None:
+            ref64 = got
+        e = [_errs(g, r) for g, r in zip(got, ref64)]
+        res[name] = ms
+        out["kernels"][name] = {
+            "ms": ms, "bytes_per_elem": nbytes / K, "GBps": nbytes / ms * 1e-6,
+            "hbm_frac": nbytes / ms * 1e-6 / HBM_PEAK_GBS, "TFLOPs": flops / ms * 1e-9,
+            "err_vs_f64": {"rel_l2": max(q["rel_l2"] for q in e),
+                           "max_abs": max(q["max_abs"] for q in e),
+                           "max_rel": max(q["max_rel"] for q in e)}}
+        log(f"{name}: {ms:.4f} ms, {nbytes / ms * 1e-6:.0f} GB/s, {flops / ms * 1e-9:.1f} TF/s, "
+            f"err {out['kernels'][name]['err_vs_f64']}")
+        del outs, got
+        plan.free()
+    if case == "c2":
+        # Reference layout: per-node metrics and outputs, 13 Np K values per call.
+        geo = mesh.geometry()[:9]
+        Dm = [dev.upload(np.ascontiguousarray(d)) for d in D]
+        off = dev.upload(np.array([0, K * Np], dtype=np.int64))
+        Karr = np.array([K], dtype=np.int64)
+        for name, mt in (("nodal_f64", _lib.DGK_F64), ("nodal_f32", _lib.DGK_F32)):
+            mdt = np.float64 if mt == _lib.DGK_F64 else np.float32
+            mb = [dev.upload(np.repeat(geo[q], Np).astype(mdt)) for q in range(9)]
+            mptrs = (C.c_void_p * 9)(*[b.ptr.value for b in mb])
+            outs = [dev.empty(K * Np, mdt) for _ in range(3)]
+            fn = lambda: _lib.call(  # noqa: E731
+                "dgk_phys_deriv_nodal", dev.h, N, 1, Karr.ctypes.data_as(C.POINTER(C.c_int64)),
+                Dm[0].ptr, Dm[1].ptr, Dm[2].ptr, du.ptr, off.ptr, mptrs, mt, off.ptr,
+                outs[0].ptr, outs[1].ptr, outs[2].ptr, mt, off.ptr)
+            ms = _time_ms(dev, fn, a.reps)
+            vb = 8 if mt == _lib.DGK_F64 else 4
+            nbytes = (Np * 8 + 12 * Np * vb) * K
+            got = [o.to_host().astype(np.float64) for o in outs]
+            e = [_errs(g, r) for g, r in zip(got, ref64)]
+            out["kernels"][name] = {
+                "ms": ms, "bytes_per_elem": nbytes / K, "GBps": nbytes / ms * 1e-6,
+                "hbm_frac": nbytes / ms * 1e-6 / HBM_PEAK_GBS, "TFLOPs": flops / ms * 1e-9,
+                "err_vs_affine_f64": {"rel_l2": max(q["rel_l2"] for q in e),
+                                      "max_rel": max(q["max_rel"] for q in e)}}
+            log(f"{name}: {ms:.4f} ms, {nbytes / ms * 1e-6:.0f} GB/s")
+            del mb, outs, got
+        # FaceConnector exchange for 4 slabs, bit-exact maps; bytes: 8 read + 8 write + 2x4 index.
+        _, vp = st.vmaps()
+        Kp = M.slab_partition(K, 4, 0.0, seed=0)
+        etop = M.etop_from_counts(Kp)
+        fc = M.new_face_connector(K, 4, st.ref.Nfp, Np, vp, etop)
+        uoff, _ = dg.aligned_offsets(Kp, Np)
+        Poff, Pbytes = dg.aligned_offsets(Kp, 4 * st.ref.Nfp)
+        counts = np.array([[fc.pick[(s_, d_)].size for d_ in range(4)] for s_ in range(4)],
+                          dtype=np.int64)
+        pick = np.concatenate([fc.pick[(s_, d_)] for s_ in range(4) for d_ in range(4)]).astype(np.int32)
+        place = np.concatenate([fc.place[(d_, s_)] for d_ in range(4) for s_ in range(4)]).astype(np.int32)
+        plan = C.c_void_p()
+        _lib.call("dgk_face_plan_create", dev.h, 4, counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                  pick.ctypes.data_as(C.POINTER(C.c_int32)), place.ctypes.data_as(C.POINTER(C.c_int32)),
+                  C.byref(plan))
+        up = dev.empty(int(uoff[-1]))
+        dP = dev.empty(Pbytes // 8)
+        duo, dPo = dev.upload(uoff), dev.upload(Poff)
+        ms = _time_ms(dev, lambda: _lib.call("dgk_face_exchange", plan, up.ptr, duo.ptr, dP.ptr,
+                                             dPo.ptr), a.reps)
+        nbytes = pick.size * 24
+        out["kernels"]["face_exchange_4slabs"] = {"ms": ms, "GBps": nbytes / ms * 1e-6,
+                                                  "hbm_frac": nbytes / ms * 1e-6 / HBM_PEAK_GBS,
+                                                  "points": int(pick.size)}
+        log(f"face exchange: {ms:.4f} ms, {nbytes / ms * 1e-6:.0f} GB/s")
+        _lib.call("dgk_face_plan_destroy", plan)
+    else:
+        out["speedup_vs_f64"] = {k: res["affine_f64"] / v for k, v in res.items()}
+    out["device"] = dev.name
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +260,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--mode", choices=["stage", "deriv"], default="stage")
+    ap.add_argument("--case", choices=["c2", "c4"], default="c4", help="derivative workload")
+    ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (PMC calibration)")
     a = ap.parse_args(argv)
@@ -140,6 +281,10 @@ def main(argv=None) -> int:
             src.copy_to(dst)
         dev.sync()
         print(json.dumps({"copy_calib_bytes": a.copy_calib}), flush=True)
+        return 0
+
+    if a.mode == "deriv":
+        print(json.dumps(bench_deriv(a, _lib.Device())), flush=True)
         return 0
 
     t_setup = time.perf_counter()

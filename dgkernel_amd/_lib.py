@@ -87,8 +87,9 @@ _SIGS = {
     "dgk_timer_destroy": (_i32, [_vp]),
     "dgk_phys_deriv_nodal": (_i32, [_vp, _i32, _i32, _pi64, _vp, _vp, _vp, _vp, _vp,
                                     C.POINTER(_vp), _i32, _vp, _vp, _vp, _vp, _i32, _vp]),
-    "dgk_phys_deriv_affine": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                     _i32]),
+    "dgk_deriv_plan_create": (_i32, [_vp, _i32, _pf64, _pf64, _pf64, _i32, C.POINTER(_vp)]),
+    "dgk_deriv_plan_destroy": (_i32, [_vp]),
+    "dgk_phys_deriv_affine": (_i32, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
     "dgk_matmul": (_i32, [_vp, _i32, _i32, _vp, _i32, _pi64, _vp, _vp, _vp, _vp, _i32]),
     "dgk_face_plan_create": (_i32, [_vp, _i32, _pi64, _pi32, _pi32, C.POINTER(_vp)]),
     "dgk_face_plan_destroy": (_i32, [_vp]),

@@ -375,12 +375,3 @@ extern "C" int dgk_matmul(dgk_device dev, int rows, int cols, const double* A_co
   return dgk::matmul_any(dev, rows, cols, A_colmajor_dev, npart, K_host, in_dev, in_off_dev, out_dev,
                          out_off_dev, add, 8);
 }
-
-extern "C" int dgk_phys_deriv_affine(dgk_device dev, int N, int64_t K, const double* Dr_dev,
-                                     const double* Ds_dev, const double* Dt_dev, const double* u_dev,
-                                     const double* geo_dev, void* dudx_dev, void* dudy_dev,
-                                     void* dudz_dev, int prec) {
-  dgk::clear_error();
-  dgk::set_error("dgk_phys_deriv_affine: not built yet");
-  return DGK_ERR_UNSUPPORTED;
-}

@@ -237,3 +237,42 @@ class BurgersSolver:
         self.u1.copy_from(self.st.to_pooled(u_dense))
         _lib.call("dgk_burgers_rhs", self.mesh.h, self.u1.ptr, self.u2.ptr)
         return self.st.from_pooled(self.u2.to_host()).reshape(self.st.K, self.st.ref.Np)
+
+
+class DerivPlan:
+    """Affine physical derivative on matrix cores (dgk_deriv_plan).
+
+    ``prec`` is one of ``_lib.DGK_PREC_F64``, ``DGK_PREC_F32``,
+    ``DGK_PREC_BF16X3``; outputs are FP64 for the first and FP32 otherwise.
+    ``D`` is (Dr, Ds, Dt), Np x Np row-major.
+    """
+
+    def __init__(self, dev: _lib.Device, N: int, D, prec: int = _lib.DGK_PREC_F64):
+        self.dev, self.N, self.prec = dev, N, prec
+        D = [np.ascontiguousarray(m, dtype=np.float64) for m in D]
+        h = C.c_void_p()
+        P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        _lib.call("dgk_deriv_plan_create", dev.h, N, *(P(m) for m in D), prec, C.byref(h))
+        self.h = h
+        self.out_dtype = np.float64 if prec == _lib.DGK_PREC_F64 else np.float32
+
+    @classmethod
+    def from_ref(cls, dev: _lib.Device, ref: RefTet, prec: int = _lib.DGK_PREC_F64):
+        return cls(dev, ref.N, (ref.Dr, ref.Ds, ref.Dt), prec)
+
+    def apply(self, K: int, u, geo_ptr, geo_stride: int, outs) -> None:
+        """Enqueue d/dx, d/dy, d/dz of dense [K, Np] ``u`` (device buffer) into ``outs``."""
+        _lib.call("dgk_phys_deriv_affine", self.h, K, u.ptr, geo_ptr, geo_stride,
+                  outs[0].ptr, outs[1].ptr, outs[2].ptr)
+
+    def free(self) -> None:
+        if self.h:
+            _lib.call("dgk_deriv_plan_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib.alive(self.dev):
+                _lib.load().dgk_deriv_plan_destroy(self.h)
+        except Exception:
+            pass

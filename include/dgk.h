@@ -171,16 +171,25 @@ int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int64_t* K_host
                          const int64_t* m_off_dev, void* dudx_dev, void* dudy_dev, void* dudz_dev,
                          int otype, const int64_t* o_off_dev);
 
-/* Affine elements (native layout): metrics are one 3x3 block per element,
- * geo_dev[q*K + k] for q = 0..8 (rx ry rz sx sy sz tx ty tz), FP64.
- * u/out: dense element-major [k*Np + n]. prec: DGK_F64 (FP64 FMA),
- * DGK_F32 (FP32 math, FP32 outputs), or DGK_PREC_BF16X3 (split-bf16 MFMA,
- * FP32 accumulate, FP32 outputs). */
+/* Affine elements (native layout): the operator is prepared once per order
+ * and precision (MFMA fragments), then applied to K elements:
+ *   u_dev: dense element-major [k*Np + n], FP64;
+ *   geo_dev: SoA rows rx ry rz sx sy sz tx ty tz at geo_dev[q*geo_stride + k]
+ *            (e.g. the first 9 rows of dgk_dgmesh_geometry_dev, stride K);
+ *   outputs dense element-major, FP64 for DGK_PREC_F64, FP32 otherwise.
+ * DGK_PREC_F64: FP64 MFMA (v_mfma_f64_16x16x4), FP64 everywhere.
+ * DGK_PREC_F32: operator and u rounded to FP32, FP32 MFMA, FP32 accumulate.
+ * DGK_PREC_BF16X3: operator and u split into bf16 hi + lo parts, three bf16
+ *   MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulate (Henry, Tang &
+ *   Heinecke, ARITH 2019). Metrics are applied in FP32 for the FP32 outputs. */
 enum { DGK_PREC_F64 = 0, DGK_PREC_F32 = 1, DGK_PREC_BF16X3 = 2 };
-int dgk_phys_deriv_affine(dgk_device dev, int N, int64_t K, const double* Dr_dev,
-                          const double* Ds_dev, const double* Dt_dev, const double* u_dev,
-                          const double* geo_dev, void* dudx_dev, void* dudy_dev, void* dudz_dev,
-                          int prec);
+typedef struct dgk_deriv_plan_s* dgk_deriv_plan;
+int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr_host, const double* Ds_host,
+                          const double* Dt_host, int prec, dgk_deriv_plan* out);
+int dgk_deriv_plan_destroy(dgk_deriv_plan plan);
+int dgk_phys_deriv_affine(dgk_deriv_plan plan, int64_t K, const double* u_dev,
+                          const double* geo_dev, int64_t geo_stride, void* dudx_dev,
+                          void* dudy_dev, void* dudz_dev);
 
 /* MATMUL_<X>(IN, OUT, K[part]) / MATMUL_ADD_<X> of
  * runner/builder/builder.go:355-410 over all partitions in one launch:

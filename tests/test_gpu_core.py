@@ -320,3 +320,23 @@ def test_mfma_and_valu_stage_agree(device):
         s.step(10)
         out.append(s.state())
     assert rel_l2(out[1], out[0]) <= 1e-14
+
+
+def test_face_exchange_c2_size_four_slabs_bit_exact(device):
+    """2^20 jittered tets, P=4, 4 slabs: the device exchange equals the
+    host simulation value for value and places u at VmapP."""
+    m = M.cube_mesh(64, 64, 43, jitter=0.1, seed=5, kmax=2 ** 20)
+    st = dg.make_setup(m, 4, Kp=[m.K])
+    Np, Nfp = st.ref.Np, st.ref.Nfp
+    _, vp = st.vmaps()
+    etop = M.etop_from_counts(M.slab_partition(m.K, 4, 0.0, seed=0))
+    fc = M.new_face_connector(m.K, 4, Nfp, Np, vp, etop)
+    u = M.uniform(6, 9, m.K * Np, -1, 1).reshape(m.K, Np)
+    u_parts = [u[fc.local_to_global[p]] for p in range(4)]
+    got = _exchange_gpu(device, fc, u_parts, Np, Nfp)
+    want = _simulate_exchange(fc, u_parts, Nfp)
+    flat = np.empty((m.K, 4 * Nfp))
+    for p in range(4):
+        assert np.array_equal(got[p], want[p])
+        flat[fc.local_to_global[p]] = got[p].reshape(-1, 4 * Nfp)
+    assert np.array_equal(flat.reshape(-1), u.reshape(-1)[vp])

@@ -146,3 +146,21 @@ def test_pooled_layout_roundtrip():
     assert pool.size * 8 == st.state_bytes
     assert np.array_equal(st.from_pooled(pool).reshape(u.shape), u)
     assert all((o * 8) % dg.WARP_ALIGN == 0 for o in st.state_off)
+
+
+def test_face_connector_c2_size_one_and_four_slabs():
+    """2^20 jittered tets at P=4 (first 2^20 tets of a 64x64x43-hex cube):
+    pick/place maps for 1 partition and for 4 slabs equal the oracle's."""
+    m = M.cube_mesh(64, 64, 43, jitter=0.1, seed=5, kmax=2 ** 20)
+    st = dg.make_setup(m, 4, Kp=[m.K])
+    _, vp = st.vmaps()
+    for parts in (1, 4):
+        Kp = M.slab_partition(m.K, parts, 0.0, seed=0) if parts > 1 else np.array([m.K])
+        etop = M.etop_from_counts(Kp)
+        fc = M.new_face_connector(m.K, 4, st.ref.Nfp, st.ref.Np, vp, etop)
+        NP, counts, pk, pl = O.face_connector(m.K, 4, st.ref.Nfp, st.ref.Np, vp, etop)
+        assert NP == parts == fc.num_partitions
+        assert sum(v.size for v in fc.pick.values()) == m.K * 4 * st.ref.Nfp
+        for k in pk:
+            assert np.array_equal(pk[k], fc.pick[k])
+            assert np.array_equal(pl[(k[1], k[0])], fc.place[(k[1], k[0])])
