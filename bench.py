@@ -132,7 +132,7 @@ def bench_deriv(a, dev) -> dict:
     c2: first 2^20 tets of a 64x64x43-hex cube, vertices jittered +-10% of h,
         P=4, u ~ U[-1,1]; FP64 and FP32.
     c4: 64^3 x 6 = 1,572,864 periodic tets, P=6, smooth field + U(+-1e-6);
-        FP64, FP32 and split BF16x3 (FP32 outputs for the last two).
+        FP64, FP32, split BF16x3 and BF16x6 (FP32 outputs below FP64).
     """
     import ctypes as C
     from dgkernel_amd import _lib, dg
@@ -163,9 +163,13 @@ def bench_deriv(a, dev) -> dict:
     ref64 = None
     res = {}
     precs = [("affine_f64", _lib.DGK_PREC_F64), ("affine_f32", _lib.DGK_PREC_F32),
-             ("affine_bf16x3", _lib.DGK_PREC_BF16X3)]
+             ("affine_bf16x3", _lib.DGK_PREC_BF16X3), ("affine_bf16x6", _lib.DGK_PREC_BF16X6)]
+    rst = (st.ref.r, st.ref.s, st.ref.t)
+    if a.precs:
+        keep = set(a.precs.split(","))
+        precs = [pp for pp in precs if pp[0].split("_", 1)[1] in keep or pp[0] == "affine_f64"]
     for name, prec in precs:
-        plan = dg.DerivPlan(dev, N, D, prec)
+        plan = dg.DerivPlan(dev, N, D, prec, rst)
         outs = [dev.empty(K * Np, plan.out_dtype) for _ in range(3)]
         ms = _time_ms(dev, lambda: plan.apply(K, du, geo_ptr, K, outs), a.reps)
         ob = 8 if prec == _lib.DGK_PREC_F64 else 4
@@ -185,7 +189,7 @@ def bench_deriv(a, dev) -> dict:
             f"err {out['kernels'][name]['err_vs_f64']}")
         del outs, got
         plan.free()
-    if case == "c2":
+    if case == "c2" and not a.precs:
         # Reference layout: per-node metrics and outputs, 13 Np K values per call.
         geo = mesh.geometry()[:9]
         Dm = [dev.upload(np.ascontiguousarray(d)) for d in D]
@@ -263,6 +267,7 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["stage", "deriv"], default="stage")
     ap.add_argument("--case", choices=["c2", "c4"], default="c4", help="derivative workload")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--precs", default="", help="derivative precisions to run (f32,bf16x3,...)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (PMC calibration)")
     a = ap.parse_args(argv)

@@ -23,7 +23,7 @@ _pf64 = C.POINTER(C.c_double)
 
 DGK_OK = 0
 DGK_F64, DGK_F32 = 0, 1
-DGK_PREC_F64, DGK_PREC_F32, DGK_PREC_BF16X3 = 0, 1, 2
+DGK_PREC_F64, DGK_PREC_F32, DGK_PREC_BF16X3, DGK_PREC_BF16X6 = 0, 1, 2, 3
 ARG_MEM, ARG_F64, ARG_F32, ARG_I64, ARG_I32 = 0, 1, 2, 3, 4
 
 
@@ -87,7 +87,8 @@ _SIGS = {
     "dgk_timer_destroy": (_i32, [_vp]),
     "dgk_phys_deriv_nodal": (_i32, [_vp, _i32, _i32, _pi64, _vp, _vp, _vp, _vp, _vp,
                                     C.POINTER(_vp), _i32, _vp, _vp, _vp, _vp, _i32, _vp]),
-    "dgk_deriv_plan_create": (_i32, [_vp, _i32, _pf64, _pf64, _pf64, _i32, C.POINTER(_vp)]),
+    "dgk_deriv_plan_create": (_i32, [_vp, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pf64, _i32,
+                                     C.POINTER(_vp)]),
     "dgk_deriv_plan_destroy": (_i32, [_vp]),
     "dgk_phys_deriv_affine": (_i32, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
     "dgk_matmul": (_i32, [_vp, _i32, _i32, _vp, _i32, _pi64, _vp, _vp, _vp, _vp, _i32]),

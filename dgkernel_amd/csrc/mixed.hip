@@ -1,5 +1,5 @@
 // Physical derivative on affine tetrahedra with matrix cores, in FP64, FP32
-// and split BF16x3.
+// and split BF16 (three- or six-product) arithmetic.
 //
 // For an affine element the metrics are constant, so
 //   du/dx = rx (Dr u) + sx (Ds u) + tx (Dt u)   (likewise y, z),
@@ -7,229 +7,453 @@
 // (tetnudg/tet_nudg_mesh_physical_derivative_test.go:344-367) with one
 // metric value per element instead of one per node.
 //
-// Per wave, 16 elements are the N dimension of the MFMA. The stacked operator
-// [Dr; Ds; Dt] (blocks at row offsets 0, SO, 2 SO) is the A operand, u the B
-// operand, read straight from HBM in the lane order the MFMA wants. SO is
-// chosen so that the lane holding row i of Dr u also holds rows i of Ds u
-// and Dt u (FP64 result rows are interleaved by 4, FP32 rows are blocked by
-// 4 within 16-row tiles), so the chain rule happens in registers.
+// Work decomposition. The stacked operator A = [Dr; Ds; Dt] (3 Np rows,
+// padded to MT tiles of 16 rows) is the MFMA A operand; 16 elements of u are
+// the B operand (N dimension). A block of G waves shares one 16-element
+// tile at a time; wave w owns row tiles [w TPW, (w+1) TPW) and keeps their
+// operator fragments in registers for the whole launch, so the operator is
+// read from memory once per wave, not once per tile. Per element tile:
+//   1. u (16 Np contiguous doubles) and the 9 metrics are staged in LDS with
+//      coalesced 16-byte loads; the next tile's loads are issued before the
+//      matrix phase;
+//   2. low-precision modes subtract each element's linear interpolant through
+//      its four vertex nodes (w = u - L u, in FP64); D L u is the constant
+//      (u_v1 - u_v0, u_v2 - u_v0, u_v3 - u_v0)/2 in (r, s, t) and is added
+//      back in FP64 in the epilogue. Rounding errors then scale with
+//      |u - L u| = O(h^2 |u''|) instead of |u|;
+//   3. every wave runs its MFMAs with B read from LDS, and parks the results
+//      in LDS;
+//   4. the epilogue applies the chain rule in FP64 and writes the three
+//      outputs with contiguous stores.
 //
-//   F64:    v_mfma_f64_16x16x4_f64, operator fragments streamed from L2.
-//   F32:    v_mfma_f32_16x16x4_f32 on FP32-rounded operator and u.
-//   BF16X3: v_mfma_f32_16x16x32_bf16 on hi/lo splits, hi*hi + hi*lo + lo*hi,
-//           FP32 accumulate (Henry, Tang & Heinecke, ARITH 2019).
-// F32 and BF16X3 keep the operator fragments in LDS for the block's life.
+//   F64:     v_mfma_f64_16x16x4_f64.
+//   F32:     v_mfma_f32_16x16x4_f32 on FP32-rounded operator and w.
+//   BF16X3:  v_mfma_f32_16x16x32_bf16 on two-part bf16 splits,
+//            hi*hi + hi*lo + lo*hi (Henry, Tang & Heinecke, ARITH 2019);
+//            ~16 significant bits per operand.
+//   BF16X6:  three-part splits, hi*hi + hi*mid + mid*hi + mid*mid + hi*lo +
+//            lo*hi; ~24 bits, FP32-level accuracy.
+// All low-precision modes accumulate in FP32 and write FP32 outputs.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "dgk_internal.h"
 
 namespace {
 
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+typedef unsigned int uintx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kF64 = 0, kF32 = 1, kBF = 2;
+constexpr int kF64 = 0, kF32 = 1, kBF3 = 2, kBF6 = 3;
+
+constexpr int parts_of(int prec) { return prec == kBF3 ? 2 : (prec == kBF6 ? 3 : 1); }
+constexpr bool is_bf(int prec) { return prec == kBF3 || prec == kBF6; }
+constexpr int freg_of(int prec) { return prec == kF64 ? 2 : (prec == kF32 ? 1 : 4); }
 
 template <int N, int PREC>
-struct DShape {
+struct ACfg {
   static constexpr int NP = (N + 1) * (N + 2) * (N + 3) / 6;
-  static constexpr int SO = PREC == kF64 ? (NP + 3) / 4 * 4 : (NP + 15) / 16 * 16;
-  static constexpr int MT = (2 * SO + NP + 15) / 16;
-  static constexpr int KSTEP = PREC == kBF ? 32 : 4;
+  static constexpr int ROWS = 3 * NP;
+  static constexpr int MT = (ROWS + 15) / 16;
+  static constexpr int KSTEP = is_bf(PREC) ? 32 : 4;
   static constexpr int KS = (NP + KSTEP - 1) / KSTEP;
-  static constexpr int NFRAG = MT * KS * 64;  // fragments (per part for BF16)
+  static constexpr int PARTS = parts_of(PREC);
+  static constexpr int FRAG_REGS = MT * KS * PARTS * freg_of(PREC);
+  // G matrix waves hold the operator; G memory waves stage, convert and
+  // write back. Four matrix waves (one per SIMD) keep long MFMA chains; the
+  // low-precision kernels are held to 128 registers (WPE = 4 waves per
+  // SIMD) so that two blocks share a CU and one block's matrix phase
+  // overlaps the other's memory phase. FP64 needs ~170 fragment and
+  // accumulator registers per matrix wave and runs one block per CU.
+  static constexpr int G = MT < 4 ? MT : (FRAG_REGS / 4 > 100 && PREC != kF64 ? 8 : 4);
+  static constexpr int WPE = PREC == kF64 ? 2 : 4;
+  static constexpr int TPW = (MT + G - 1) / G;
+  static constexpr int NT = 128 * G;                     // threads per block
+  static constexpr int NTM = 64 * G;                     // memory threads
+  static constexpr int NPU = NP | 1;                     // LDS row stride of u, in doubles
+  static constexpr int NPF = NP | 1;                     // LDS row stride of FP32 w, in floats
+  static constexpr int KPB = KS * 32 + 8;                // LDS row stride of bf16 w parts
+  static constexpr int ROWSP = MT * 16 + 1;              // LDS row stride of results
+  static constexpr int NV2 = 8 * NP;                     // double2 per 16-element tile
+  static constexpr int NL = (NV2 + NTM - 1) / NTM;       // double2 loads per memory thread
+  static constexpr int NIT = (16 * NP + NTM - 1) / NTM;  // values per memory thread
+  static constexpr size_t NFRAG = (size_t)MT * KS * 64;  // fragments per part
 };
 
-int shape_query(int N, int prec, int& NP, int& SO, int& MT, int& KSTEP, int& KS) {
-  NP = dgk::np_of(N);
-  SO = prec == kF64 ? (NP + 3) / 4 * 4 : (NP + 15) / 16 * 16;
-  MT = (2 * SO + NP + 15) / 16;
-  KSTEP = prec == kBF ? 32 : 4;
-  KS = (NP + KSTEP - 1) / KSTEP;
-  return DGK_OK;
+// Host-side copy of the fragment shape (same formulas as ACfg).
+struct HostCfg {
+  int NP, MT, KSTEP, KS, PARTS;
+};
+HostCfg host_cfg(int N, int prec) {
+  HostCfg c;
+  c.NP = dgk::np_of(N);
+  c.MT = (3 * c.NP + 15) / 16;
+  c.KSTEP = is_bf(prec) ? 32 : 4;
+  c.KS = (c.NP + c.KSTEP - 1) / c.KSTEP;
+  c.PARTS = parts_of(prec);
+  return c;
 }
 
-// Node groups: the rows are processed in NG passes over K, each pass
-// accumulating only the 16-row tiles that hold its nodes' Dr u, Ds u, Dt u,
-// which bounds the live accumulators (the B operand is re-read from cache).
-template <int N, int PREC, int NG>
-struct Groups {
-  using SH = DShape<N, PREC>;
-  static constexpr int QN = PREC == kF64 ? SH::SO / 4 : SH::SO / 16;  // node slots per lane
-  static constexpr int QG = (QN + NG - 1) / NG;
-  // Does group G use 16-row tile tt?
-  static constexpr bool needs(int G, int tt) {
-    for (int q = G * QG; q < (G + 1) * QG && q < QN; ++q)
-      for (int b = 0; b < 3; ++b) {
-        const int row0 = PREC == kF64 ? b * SH::SO + 4 * q : b * SH::SO + 16 * q;
-        if (row0 / 16 == tt) return true;
-      }
-    return false;
-  }
+template <int PREC>
+struct Frag;
+template <>
+struct Frag<kF64> {
+  double v;
+};
+template <>
+struct Frag<kF32> {
+  float v;
+};
+template <>
+struct Frag<kBF3> {
+  bf16x8 v[2];
+};
+template <>
+struct Frag<kBF6> {
+  bf16x8 v[3];
 };
 
-template <int N, int PREC, int NG, int G>
-__device__ __forceinline__ void deriv_group(const unsigned char* sA, const void* frag, const double* ue,
-                                            bool valid, int lane, int g, const double* m64,
-                                            void* ox, void* oy, void* oz, int64_t ge) {
-  using SH = DShape<N, PREC>;
-  using GR = Groups<N, PREC, NG>;
-  constexpr int NP = SH::NP, SO = SH::SO, MT = SH::MT, KS = SH::KS, NFRAG = SH::NFRAG;
-  constexpr int QN = GR::QN, QG = GR::QG;
-  if constexpr (PREC == kF64) {
-    const double* A = (const double*)frag;
-    doublex4 acc[MT];
-#pragma unroll
-    for (int tt = 0; tt < MT; ++tt) acc[tt] = (doublex4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll 3
-    for (int s = 0; s < KS; ++s) {
-      const int k = 4 * s + g;
-      const double b = (valid && k < NP) ? ue[k] : 0.0;
-#pragma unroll
-      for (int tt = 0; tt < MT; ++tt)
-        if (GR::needs(G, tt))
-          acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(tt * KS + s) * 64 + lane], b, acc[tt],
-                                                          0, 0, 0);
+template <int PREC>
+using AccT = typename std::conditional<PREC == kF64, doublex4, floatx4>::type;
+template <int PREC>
+using ResT = typename std::conditional<PREC == kF64, double, float>::type;
+
+struct AffineArgs {
+  int64_t K;
+  const void* frag;    // operator fragments, [part][MT][KS][64]
+  const double* lam;   // [3][NP] barycentric weights l1, l2, l3 of each node (mixed modes)
+  int vtx[4];          // indices of the vertex nodes (mixed modes)
+  const double* u;
+  const double* geo;
+  int64_t gs;
+  void* ox;
+  void* oy;
+  void* oz;
+};
+
+template <int N, int PREC>
+__global__ void __launch_bounds__((ACfg<N, PREC>::NT))
+__attribute__((amdgpu_waves_per_eu(ACfg<N, PREC>::WPE)))
+deriv_affine_kernel(AffineArgs a) {
+  using C = ACfg<N, PREC>;
+  constexpr int NP = C::NP, MT = C::MT, KS = C::KS, TPW = C::TPW, G = C::G, NTM = C::NTM;
+  constexpr int NPU = C::NPU, NPF = C::NPF, KPB = C::KPB, ROWSP = C::ROWSP;
+  constexpr int NV2 = C::NV2, NL = C::NL, NIT = C::NIT, PARTS = C::PARTS;
+  constexpr bool MIXED = PREC != kF64;
+  using R = ResT<PREC>;
+  // Software pipeline over the block's tiles n = 0, 1, ... (tile index
+  // blockIdx.x + n gridDim.x). Between two barriers, phase n runs
+  //   memory waves: stage(n + LEAD)  prefetched registers -> sU, sM
+  //                 lin(n + 1)       (below FP64) sU -> w = u - Lu in the
+  //                                  matrix precision; slopes of Lu -> sM
+  //                 epi(n - 1)       sR, sM -> outputs
+  //   matrix waves: mma(n)           w (or sU) -> sR
+  // so the matrix cores of one tile overlap the memory and vector work of
+  // its neighbours. Buffers are indexed by tile parity (sU, w, sR) or by
+  // n mod 4 (sM), which keeps every live tile in its own buffer.
+  constexpr int LEAD = MIXED ? 2 : 1;
+  __shared__ double sU[2][16 * NPU];
+  __shared__ float sWf[PREC == kF32 ? 2 : 1][PREC == kF32 ? 16 * NPF : 1];
+  __shared__ __attribute__((aligned(16))) __bf16 sWb[is_bf(PREC) ? 2 : 1][PARTS]
+                                                    [is_bf(PREC) ? 16 * KPB : 1];
+  __shared__ R sR[2][16 * ROWSP];
+  // Per element: 9 metrics rx..tz, then (below FP64) half the linear part's
+  // r, s, t slopes, padded to 12 doubles so the epilogue reads it with six
+  // 16-byte loads.
+  __shared__ __attribute__((aligned(16))) double sM[4][16 * 12];
+  __shared__ double sLam[MIXED ? 3 * NP : 1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const bool is_mma = w < G;
+  const int mt = tid - NTM;  // memory-thread index (valid when !is_mma)
+  const int c = lane & 15, g = lane >> 4;
+
+  // All setup loads complete here, so that the wait counters inside the
+  // tile loop only ever cover the tile prefetch.
+  __builtin_amdgcn_s_waitcnt(0);
+
+  const int64_t K = a.K;
+  const int64_t ntiles = (K + 15) / 16;
+  const int64_t nval = K * NP;
+  const int64_t Gd = gridDim.x;
+  const int64_t b0 = blockIdx.x;
+  const int64_t nb = b0 < ntiles ? (ntiles - b0 + Gd - 1) / Gd : 0;  // tiles of this block
+  constexpr int PD = 2;  // tiles whose loads are in flight per block
+  if (nb == 0) return;
+
+  // Tile loads through a buffer resource based at the tile: the per-thread
+  // offsets are loop-invariant registers, and reads past the end of u return
+  // zero. Values of a pair that straddles or passes the end are fixed up when
+  // staged.
+  const int64_t last_pair = nval / 2 - 1;
+  auto prefetch = [&](int64_t n, doublex2 (&P)[NL], double& PM) {
+    const int64_t t = b0 + n * Gd;
+    if (mt < 9 * 16) {
+      const int64_t e = t * 16 + (mt & 15);
+      PM = a.geo[(mt >> 4) * a.gs + (e < K ? e : K - 1)];
     }
-    if (valid) {
-      double* px = (double*)ox + ge * NP;
-      double* py = (double*)oy + ge * NP;
-      double* pz = (double*)oz + ge * NP;
+    const int64_t first = t * (16 * NP);  // first value of the tile
+    const int64_t left = (nval - first) * 8;
+    const int bytes = (int)(left < 16 * NP * 8 ? left : 16 * NP * 8);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(a.u + first), 0, bytes, 0x00020000);
 #pragma unroll
-      for (int q = G * QG; q < (G + 1) * QG && q < QN; ++q) {
-        const int i = g + 4 * q;
-        const double ur = acc[(4 * q) / 16][((4 * q) % 16) / 4];
-        const double us = acc[(SO + 4 * q) / 16][((SO + 4 * q) % 16) / 4];
-        const double ut = acc[(2 * SO + 4 * q) / 16][((2 * SO + 4 * q) % 16) / 4];
-        if (i < NP) {
-          px[i] = m64[0] * ur + m64[3] * us + m64[6] * ut;
-          py[i] = m64[1] * ur + m64[4] * us + m64[7] * ut;
-          pz[i] = m64[2] * ur + m64[5] * us + m64[8] * ut;
+    for (int q = 0; q < NL; ++q) {
+      const int v = mt + q * NTM;
+      P[q] = __builtin_bit_cast(doublex2, __builtin_amdgcn_raw_buffer_load_b128(rs, 16 * v, 0, 0));
+    }
+  };
+
+  // stage(n): registers of slot n % PD -> sU[n % 2], sM[n % 4]; refills the slot.
+  auto stage = [&](int64_t n, int bu, int bm, doublex2 (&P)[NL], double& PM) {
+    const int64_t t = b0 + n * Gd;
+    double* U = sU[bu];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      const int v = mt + q * NTM;
+      if (v < NV2) {
+        const int64_t p = t * (8 * NP) + v;  // pair index in u
+        double x0 = P[q].x, x1 = P[q].y;
+        if (p > last_pair) {
+          x0 = (p == last_pair + 1 && (nval & 1)) ? a.u[nval - 1] : 0.0;
+          x1 = 0.0;
         }
+        const int i0 = 2 * v, i1 = 2 * v + 1;
+        U[(i0 / NP) * NPU + i0 % NP] = x0;
+        U[(i1 / NP) * NPU + i1 % NP] = x1;
+      }
+    }
+    if constexpr (NTM >= 9 * 16) {
+      if (mt < 9 * 16) sM[bm][(mt & 15) * 12 + (mt >> 4)] = PM;
+    } else {
+      const int ne = (int)((K - t * 16) < 16 ? (K - t * 16) : 16);
+      for (int v = mt; v < 9 * 16; v += NTM) {
+        const int q = v >> 4, e = v & 15;
+        sM[bm][e * 12 + q] = e < ne ? a.geo[q * a.gs + t * 16 + e] : 0.0;
+      }
+    }
+    if (n + PD < nb) prefetch(n + PD, P, PM);
+  };
+
+  // lin: w = u minus the linear interpolant through the vertex nodes, in the
+  // matrix precision (FP32, or bf16 parts with zero padding up to KS*32).
+  auto lin = [&](int bu, int bl) {
+    const double* U = sU[bu];
+    constexpr int RW = is_bf(PREC) ? KS * 32 : NP;  // values per element row
+    constexpr int NV = 16 * RW;
+    for (int v = mt; v < NV; v += NTM) {
+      const int e = v / RW, i = v - e * RW;
+      double x = 0.0;
+      if (i < NP) {
+        const double* ue = U + e * NPU;
+        const double a0 = ue[a.vtx[0]];
+        const double a1 = ue[a.vtx[1]] - a0, a2 = ue[a.vtx[2]] - a0, a3 = ue[a.vtx[3]] - a0;
+        x = ue[i] - (a0 + a1 * sLam[i] + a2 * sLam[NP + i] + a3 * sLam[2 * NP + i]);
+        if (i == a.vtx[0]) {
+          sM[bl][e * 12 + 9] = 0.5 * a1;
+          sM[bl][e * 12 + 10] = 0.5 * a2;
+          sM[bl][e * 12 + 11] = 0.5 * a3;
+        }
+      }
+      if constexpr (PREC == kF32) {
+        sWf[bu][e * NPF + i] = (float)x;
+      } else if constexpr (is_bf(PREC)) {
+        const __bf16 h = (__bf16)(float)x;
+        const double r1 = x - (double)(float)h;
+        const __bf16 m = (__bf16)(float)r1;
+        sWb[bu][0][e * KPB + i] = h;
+        sWb[bu][1][e * KPB + i] = m;
+        if constexpr (PREC == kBF6) sWb[bu][2][e * KPB + i] = (__bf16)(float)(r1 - (double)(float)m);
+      }
+    }
+  };
+
+  // mma: this wave's row tiles of [Dr; Ds; Dt] times the tile -> sR[n % 2].
+  auto mma = [&](int bu, const Frag<PREC> (&fa)[TPW][KS]) {
+    AccT<PREC> acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[j][r] = 0;
+    if constexpr (PREC == kF64 || PREC == kF32) {
+      using BT = typename std::conditional<PREC == kF64, double, float>::type;
+      const BT* ub;
+      if constexpr (PREC == kF64) ub = sU[bu] + c * NPU;
+      else ub = sWf[bu] + c * NPF;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        const BT b = k < NP ? ub[k] : (BT)0;
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+          if constexpr (PREC == kF64)
+            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[j][s].v, b, acc[j], 0, 0, 0);
+          else
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[j][s].v, b, acc[j], 0, 0, 0);
+        }
+      }
+    } else {
+      bf16x8 bb[KS][PARTS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p)
+          bb[s][p] = *(const bf16x8*)(sWb[bu][p] + c * KPB + 32 * s + 8 * g);
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const bf16x8* b = bb[s];
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+          if constexpr (PREC == kBF3) {
+            // parts: [0] = hi, [1] = lo; smallest products first
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[1], b[0], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[0], b[1], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[0], b[0], acc[j], 0, 0, 0);
+          } else {
+            // parts: [0] = hi, [1] = mid, [2] = lo; smallest products first
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[2], b[0], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[0], b[2], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[1], b[1], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[1], b[0], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[0], b[1], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[j][s].v[0], b[0], acc[j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // Results to LDS, [element][row]. FP64 D lanes hold rows g + 4r of a tile,
+    // FP32 D lanes hold rows 4g + r.
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+      const int tt = w * TPW + j;
+      if (tt < MT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = PREC == kF64 ? g + 4 * r : 4 * g + r;
+          sR[bu][c * ROWSP + 16 * tt + row] = acc[j][r];
+        }
+      }
+    }
+  };
+
+  // epi(n): chain rule in FP64 and contiguous stores.
+  auto epi = [&](int64_t n, int br, int bm) {
+    const int64_t t = b0 + n * Gd;
+    const int64_t e0 = t * 16;
+    const int ne = (int)((K - e0) < 16 ? (K - e0) : 16);
+    const int64_t obase = e0 * NP;
+#pragma unroll
+    for (int q = 0; q < NIT; ++q) {
+      const int v = mt + q * NTM;
+      if (v < ne * NP) {
+        const int e = v / NP, i = v - e * NP;
+        const R* re = sR[br] + e * ROWSP;
+        const doublex2* M2 = (const doublex2*)(sM[bm] + e * 12);
+        const doublex2 m01 = M2[0], m23 = M2[1], m45 = M2[2], m67 = M2[3], m8l = M2[4];
+        double ur = (double)re[i], us = (double)re[NP + i], ut = (double)re[2 * NP + i];
+        if constexpr (MIXED) {
+          const doublex2 l23 = M2[5];
+          ur += m8l.y;
+          us += l23.x;
+          ut += l23.y;
+        }
+        const double dx = m01.x * ur + m23.y * us + m67.x * ut;
+        const double dy = m01.y * ur + m45.x * us + m67.y * ut;
+        const double dz = m23.x * ur + m45.y * us + m8l.x * ut;
+        if constexpr (PREC == kF64) {
+          __builtin_nontemporal_store(dx, (double*)a.ox + obase + v);
+          __builtin_nontemporal_store(dy, (double*)a.oy + obase + v);
+          __builtin_nontemporal_store(dz, (double*)a.oz + obase + v);
+        } else {
+          __builtin_nontemporal_store((float)dx, (float*)a.ox + obase + v);
+          __builtin_nontemporal_store((float)dy, (float*)a.oy + obase + v);
+          __builtin_nontemporal_store((float)dz, (float*)a.oz + obase + v);
+        }
+      }
+    }
+  };
+
+  // The two roles run separate loops with the same barrier sequence, so that
+  // the operator registers of the matrix waves and the prefetch registers of
+  // the memory waves are never live together.
+  if (is_mma) {
+    // Operator fragments of this wave's row tiles, resident for the launch.
+    Frag<PREC> fa[TPW][KS];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+      const int tt = w * TPW + j;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const size_t q = ((size_t)tt * KS + s) * 64 + lane;
+        if constexpr (PREC == kF64) {
+          fa[j][s].v = tt < MT ? ((const double*)a.frag)[q] : 0.0;
+        } else if constexpr (PREC == kF32) {
+          fa[j][s].v = tt < MT ? ((const float*)a.frag)[q] : 0.f;
+        } else {
+#pragma unroll
+          for (int p = 0; p < PARTS; ++p) {
+            const bf16x8* F = (const bf16x8*)a.frag + (size_t)p * C::NFRAG;
+            bf16x8 z;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.0f;
+            fa[j][s].v[p] = tt < MT ? F[q] : z;
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if constexpr (MIXED) __syncthreads();
+    for (int64_t n0 = 0; n0 <= nb; n0 += 2) {
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const int64_t n = n0 + d;
+        if (n > nb) break;
+        if (n < nb) mma(d & 1, fa);
+        __syncthreads();
       }
     }
   } else {
-    floatx4 acc[MT];
-#pragma unroll
-    for (int tt = 0; tt < MT; ++tt) acc[tt] = (floatx4){0.f, 0.f, 0.f, 0.f};
-    if constexpr (PREC == kF32) {
-      const float* A = (const float*)sA;
-#pragma unroll 3
-      for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + g;
-        const float b = (valid && k < NP) ? (float)ue[k] : 0.f;
-#pragma unroll
-        for (int tt = 0; tt < MT; ++tt)
-          if (GR::needs(G, tt))
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[(tt * KS + s) * 64 + lane], b, acc[tt],
-                                                            0, 0, 0);
-      }
-    } else {
-      const bf16x8* Ahi = (const bf16x8*)sA;
-      const bf16x8* Alo = Ahi + NFRAG;
-#pragma unroll 1
-      for (int s = 0; s < KS; ++s) {
-        bf16x8 bhi, blo;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int k = 32 * s + 8 * g + e;
-          const double x = (valid && k < NP) ? ue[k] : 0.0;
-          const __bf16 h = (__bf16)(float)x;
-          bhi[e] = h;
-          blo[e] = (__bf16)(float)(x - (double)(float)h);
-        }
-#pragma unroll
-        for (int tt = 0; tt < MT; ++tt)
-          if (GR::needs(G, tt)) {
-            const bf16x8 ah = Ahi[(tt * KS + s) * 64 + lane];
-            const bf16x8 al = Alo[(tt * KS + s) * 64 + lane];
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bhi, acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, blo, acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bhi, acc[tt], 0, 0, 0);
-          }
-      }
+    if constexpr (MIXED) {
+      for (int v = mt; v < 3 * NP; v += NTM) sLam[v] = a.lam[v];
     }
-    if (valid) {
-      float m[9];
+    doublex2 pre[PD][NL];
+    double pm[PD];  // metrics: memory thread v < 144 holds row v / 16 of element v % 16
 #pragma unroll
-      for (int q = 0; q < 9; ++q) m[q] = (float)m64[q];
-      float* px = (float*)ox + ge * NP;
-      float* py = (float*)oy + ge * NP;
-      float* pz = (float*)oz + ge * NP;
+    for (int d = 0; d < PD; ++d)
+      if (d < nb) prefetch(d, pre[d], pm[d]);
+    // prologue: tiles 0 .. LEAD-1 staged, tile 0 linearised below FP64
 #pragma unroll
-      for (int tq = G * QG; tq < (G + 1) * QG && tq < QN; ++tq) {
-        floatx4 vx, vy, vz;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float ur = acc[tq][r], us = acc[SO / 16 + tq][r], ut = acc[2 * SO / 16 + tq][r];
-          vx[r] = m[0] * ur + m[3] * us + m[6] * ut;
-          vy[r] = m[1] * ur + m[4] * us + m[7] * ut;
-          vz[r] = m[2] * ur + m[5] * us + m[8] * ut;
-        }
-        const int i0 = 16 * tq + 4 * g;
-        if constexpr (NP % 4 == 0) {
-          if (i0 < NP) {
-            *(floatx4*)(px + i0) = vx;
-            *(floatx4*)(py + i0) = vy;
-            *(floatx4*)(pz + i0) = vz;
-          }
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (i0 + r < NP) {
-              px[i0 + r] = vx[r];
-              py[i0 + r] = vy[r];
-              pz[i0 + r] = vz[r];
-            }
-        }
-      }
-    }
-  }
-}
-
-template <int N, int PREC, int NG, int G>
-__device__ __forceinline__ void deriv_groups(const unsigned char* sA, const void* frag,
-                                             const double* ue, bool valid, int lane, int g,
-                                             const double* m64, void* ox, void* oy, void* oz,
-                                             int64_t ge) {
-  if constexpr (G < NG) {
-    deriv_group<N, PREC, NG, G>(sA, frag, ue, valid, lane, g, m64, ox, oy, oz, ge);
-    deriv_groups<N, PREC, NG, G + 1>(sA, frag, ue, valid, lane, g, m64, ox, oy, oz, ge);
-  }
-}
-
-template <int N, int PREC, int WPB, int NG>
-__global__ void __launch_bounds__(64 * WPB)
-deriv_affine_kernel(int64_t K, const void* __restrict__ frag, const double* __restrict__ u,
-                    const double* __restrict__ geo, int64_t gs, void* __restrict__ ox,
-                    void* __restrict__ oy, void* __restrict__ oz) {
-  using SH = DShape<N, PREC>;
-  constexpr int NP = SH::NP, NFRAG = SH::NFRAG;
-  constexpr int NT = 64 * WPB;
-  // LDS copy of the fragments (F32: floats; BF16: hi then lo, 8 bf16 each).
-  constexpr int LDS_BYTES = PREC == kF32 ? NFRAG * 4 : (PREC == kBF ? NFRAG * 32 : 16);
-  __shared__ __attribute__((aligned(16))) unsigned char sA[LDS_BYTES];
-  if constexpr (PREC != kF64) {
-    const uint4* src = (const uint4*)frag;
-    uint4* dst = (uint4*)sA;
-    for (int i = threadIdx.x; i < LDS_BYTES / 16; i += NT) dst[i] = src[i];
+    for (int d = 0; d < LEAD; ++d)
+      if (d < nb) stage(d, d & 1, d, pre[d & 1], pm[d & 1]);
     __syncthreads();
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int c = lane & 15, g = lane >> 4;
-  const int64_t ntiles = (K + 15) / 16;
-  const int64_t NW = (int64_t)gridDim.x * WPB;
-  for (int64_t t = (int64_t)blockIdx.x * WPB + w; t < ntiles; t += NW) {
-    const int64_t e0 = t * 16;
-    const bool valid = e0 + c < K;
-    const int64_t ge = valid ? e0 + c : 0;
-    double m64[9];
+    if constexpr (MIXED) {
+      lin(0, 0);
+      __syncthreads();
+    }
+    // Phase n; unrolled by 2 so that the prefetch slot and the parity-indexed
+    // buffers are compile-time constants (PD = 2).
+    for (int64_t n0 = 0; n0 <= nb; n0 += 2) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) m64[q] = valid ? geo[q * gs + ge] : 0.0;
-    deriv_groups<N, PREC, NG, 0>(sA, frag, u + ge * NP, valid, lane, g, m64, ox, oy, oz, ge);
+      for (int d = 0; d < 2; ++d) {
+        const int64_t n = n0 + d;
+        if (n > nb) break;
+        const int n4 = (int)(n & 3);
+        if (n + LEAD < nb)
+          stage(n + LEAD, (d + LEAD) & 1, (n4 + LEAD) & 3, pre[(d + LEAD) & 1], pm[(d + LEAD) & 1]);
+        if constexpr (MIXED) {
+          if (n + 1 < nb) lin((d + 1) & 1, (n4 + 1) & 3);
+        }
+        if (n >= 1) epi(n - 1, (d + 1) & 1, (n4 + 3) & 3);
+        __syncthreads();
+      }
+    }
   }
 }
 
@@ -239,6 +463,9 @@ struct dgk_deriv_plan_s {
   dgk_device dev = nullptr;
   int N = 0, prec = 0;
   void* frag = nullptr;
+  double* lam = nullptr;
+  int vtx[4] = {0, 0, 0, 0};
+  int blocks = 0;  // persistent grid size
 };
 
 namespace {
@@ -260,82 +487,138 @@ float bf16_to_float(uint16_t b) {
   return f;
 }
 
-template <int N, int PREC, int WPB, int NG>
-int launch_affine_t(dgk_deriv_plan pl, int64_t K, const double* u, const double* geo, int64_t gs,
-                    void* ox, void* oy, void* oz) {
-  int64_t ntiles = (K + 15) / 16;
-  int64_t blocks = (ntiles + WPB - 1) / WPB;
-  int bpc = PREC == kF64 ? 4 : 1;
-  int64_t cap = (int64_t)pl->dev->cu_count * bpc;
-  if (blocks > cap) blocks = cap;
+template <int N, int PREC>
+int occupancy_blocks(dgk_deriv_plan pl) {
+  int per_cu = 0;
+  DGK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, deriv_affine_kernel<N, PREC>,
+                                                       ACfg<N, PREC>::NT, 0));
+  if (per_cu < 1) per_cu = 1;
+  pl->blocks = pl->dev->cu_count * per_cu;
+  return DGK_OK;
+}
+
+template <int N, int PREC>
+int launch_affine_t(dgk_deriv_plan pl, const AffineArgs& a) {
+  const int64_t ntiles = (a.K + 15) / 16;
+  const int64_t blocks = pl->blocks < ntiles ? pl->blocks : ntiles;
   if (blocks < 1) return DGK_OK;
-  hipLaunchKernelGGL((deriv_affine_kernel<N, PREC, WPB, NG>), dim3((unsigned)blocks),
-                     dim3(64 * WPB), 0, pl->dev->stream, K, pl->frag, u, geo, gs, ox, oy, oz);
+  hipLaunchKernelGGL((deriv_affine_kernel<N, PREC>), dim3((unsigned)blocks),
+                     dim3(ACfg<N, PREC>::NT), 0, pl->dev->stream, a);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
+// occupancy == true: size the persistent grid; otherwise launch with *a.
 template <int N>
-int launch_affine_n(dgk_deriv_plan pl, int64_t K, const double* u, const double* geo, int64_t gs,
-                    void* ox, void* oy, void* oz) {
-  constexpr int NG = N >= 5 ? 2 : 1;
+int dispatch_n(dgk_deriv_plan pl, const AffineArgs* a, bool occupancy) {
   switch (pl->prec) {
-    case kF64: return launch_affine_t<N, kF64, 4, NG>(pl, K, u, geo, gs, ox, oy, oz);
-    case kF32: return launch_affine_t<N, kF32, 8, 1>(pl, K, u, geo, gs, ox, oy, oz);
-    default: return launch_affine_t<N, kBF, 8, NG>(pl, K, u, geo, gs, ox, oy, oz);
+    case kF64: return occupancy ? occupancy_blocks<N, kF64>(pl) : launch_affine_t<N, kF64>(pl, *a);
+    case kF32: return occupancy ? occupancy_blocks<N, kF32>(pl) : launch_affine_t<N, kF32>(pl, *a);
+    case kBF3: return occupancy ? occupancy_blocks<N, kBF3>(pl) : launch_affine_t<N, kBF3>(pl, *a);
+    default: return occupancy ? occupancy_blocks<N, kBF6>(pl) : launch_affine_t<N, kBF6>(pl, *a);
   }
+}
+
+int dispatch(dgk_deriv_plan pl, const AffineArgs* a, bool occupancy) {
+  switch (pl->N) {
+    case 1: return dispatch_n<1>(pl, a, occupancy);
+    case 2: return dispatch_n<2>(pl, a, occupancy);
+    case 3: return dispatch_n<3>(pl, a, occupancy);
+    case 4: return dispatch_n<4>(pl, a, occupancy);
+    case 5: return dispatch_n<5>(pl, a, occupancy);
+    case 6: return dispatch_n<6>(pl, a, occupancy);
+  }
+  dgk::set_error("dgk_phys_deriv_affine: N=%d", pl->N);
+  return DGK_ERR_UNSUPPORTED;
 }
 
 }  // namespace
 
 extern "C" int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr, const double* Ds,
-                                     const double* Dt, int prec, dgk_deriv_plan* out) {
+                                     const double* Dt, const double* r, const double* s,
+                                     const double* t, int prec, dgk_deriv_plan* out) {
   dgk::clear_error();
   DGK_REQUIRE(dev && Dr && Ds && Dt && out, "dgk_deriv_plan_create: NULL argument");
   DGK_REQUIRE(N >= 1 && N <= 6, "dgk_deriv_plan_create: N=%d outside 1..6", N);
-  DGK_REQUIRE(prec == kF64 || prec == kF32 || prec == kBF, "dgk_deriv_plan_create: prec %d", prec);
+  DGK_REQUIRE(prec >= kF64 && prec <= kBF6, "dgk_deriv_plan_create: precision %d", prec);
+  DGK_REQUIRE(prec == kF64 || (r && s && t),
+              "dgk_deriv_plan_create: node coordinates r, s, t are required below FP64");
   *out = nullptr;
-  int NP, SO, MT, KSTEP, KS;
-  shape_query(N, prec, NP, SO, MT, KSTEP, KS);
+  const HostCfg hc = host_cfg(N, prec);
+  const int NP = hc.NP;
   const double* D[3] = {Dr, Ds, Dt};
   auto val = [&](int row, int k) -> double {
-    int blk = row / SO, i = row % SO;
-    if (blk >= 3 || i >= NP || k >= NP) return 0.0;
+    const int blk = row / NP, i = row % NP;
+    if (blk >= 3 || k >= NP) return 0.0;
     return D[blk][i * NP + k];
   };
-  const size_t nfrag = (size_t)MT * KS * 64;
+  const size_t nfrag = (size_t)hc.MT * hc.KS * 64;
   std::vector<unsigned char> buf;
   if (prec == kF64 || prec == kF32) {
     buf.resize(nfrag * (prec == kF64 ? 8 : 4));
-    for (int tt = 0; tt < MT; ++tt)
-      for (int s = 0; s < KS; ++s)
+    for (int tt = 0; tt < hc.MT; ++tt)
+      for (int ks = 0; ks < hc.KS; ++ks)
         for (int l = 0; l < 64; ++l) {
-          double v = val(16 * tt + (l & 15), 4 * s + (l >> 4));
-          size_t q = ((size_t)tt * KS + s) * 64 + l;
+          const double v = val(16 * tt + (l & 15), 4 * ks + (l >> 4));
+          const size_t q = ((size_t)tt * hc.KS + ks) * 64 + l;
           if (prec == kF64) ((double*)buf.data())[q] = v;
           else ((float*)buf.data())[q] = (float)v;
         }
   } else {
-    buf.resize(nfrag * 32);
-    uint16_t* hi = (uint16_t*)buf.data();
-    uint16_t* lo = hi + nfrag * 8;
-    for (int tt = 0; tt < MT; ++tt)
-      for (int s = 0; s < KS; ++s)
+    buf.resize(nfrag * 16 * hc.PARTS);
+    uint16_t* P0 = (uint16_t*)buf.data();
+    for (int tt = 0; tt < hc.MT; ++tt)
+      for (int ks = 0; ks < hc.KS; ++ks)
         for (int l = 0; l < 64; ++l)
           for (int e = 0; e < 8; ++e) {
-            double v = val(16 * tt + (l & 15), 32 * s + 8 * (l >> 4) + e);
-            uint16_t h = bf16_bits((float)v);
-            size_t q = (((size_t)tt * KS + s) * 64 + l) * 8 + e;
-            hi[q] = h;
-            lo[q] = bf16_bits((float)(v - (double)bf16_to_float(h)));
+            const double v = val(16 * tt + (l & 15), 32 * ks + 8 * (l >> 4) + e);
+            const size_t q = (((size_t)tt * hc.KS + ks) * 64 + l) * 8 + e;
+            const uint16_t h = bf16_bits((float)v);
+            const double r1 = v - (double)bf16_to_float(h);
+            const uint16_t m = bf16_bits((float)r1);
+            P0[q] = h;
+            P0[nfrag * 8 + q] = m;
+            if (hc.PARTS == 3) {
+              const double r2 = r1 - (double)bf16_to_float(m);
+              P0[2 * nfrag * 8 + q] = bf16_bits((float)r2);
+            }
           }
+  }
+  // Vertex nodes and barycentric weights (reference tet (-1,-1,-1), (1,-1,-1),
+  // (-1,1,-1), (-1,-1,1)) for the linear-part subtraction.
+  std::vector<double> lam;
+  int vtx[4] = {0, 0, 0, 0};
+  if (prec != kF64) {
+    const double V[4][3] = {{-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+    for (int v = 0; v < 4; ++v) {
+      int found = -1;
+      for (int i = 0; i < NP; ++i)
+        if (std::fabs(r[i] - V[v][0]) < 1e-10 && std::fabs(s[i] - V[v][1]) < 1e-10 &&
+            std::fabs(t[i] - V[v][2]) < 1e-10)
+          found = i;
+      DGK_REQUIRE(found >= 0, "dgk_deriv_plan_create: vertex %d of the reference tet is not a node",
+                  v);
+      vtx[v] = found;
+    }
+    lam.resize(3 * NP);
+    for (int i = 0; i < NP; ++i) {
+      lam[i] = 0.5 * (1.0 + r[i]);
+      lam[NP + i] = 0.5 * (1.0 + s[i]);
+      lam[2 * NP + i] = 0.5 * (1.0 + t[i]);
+    }
   }
   dgk_deriv_plan p = new dgk_deriv_plan_s();
   p->dev = dev;
   p->N = N;
   p->prec = prec;
+  std::memcpy(p->vtx, vtx, sizeof vtx);
   int rc = dgk_malloc(dev, buf.size(), buf.data(), &p->frag);
+  if (rc == DGK_OK && !lam.empty())
+    rc = dgk_malloc(dev, lam.size() * sizeof(double), lam.data(), (void**)&p->lam);
+  if (rc == DGK_OK) rc = dispatch(p, nullptr, true);
   if (rc != DGK_OK) {
+    if (p->frag) dgk_free(dev, p->frag);
+    if (p->lam) dgk_free(dev, p->lam);
     delete p;
     return rc;
   }
@@ -346,6 +629,7 @@ extern "C" int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr, co
 extern "C" int dgk_deriv_plan_destroy(dgk_deriv_plan p) {
   if (!p) return DGK_OK;
   dgk_free(p->dev, p->frag);
+  if (p->lam) dgk_free(p->dev, p->lam);
   delete p;
   return DGK_OK;
 }
@@ -358,13 +642,18 @@ extern "C" int dgk_phys_deriv_affine(dgk_deriv_plan p, int64_t K, const double* 
               "dgk_phys_deriv_affine: NULL argument");
   DGK_REQUIRE(K >= 0 && geo_stride >= K, "dgk_phys_deriv_affine: K=%lld stride=%lld", (long long)K,
               (long long)geo_stride);
+  DGK_REQUIRE(((uintptr_t)u_dev & 15) == 0, "dgk_phys_deriv_affine: u must be 16-byte aligned");
   if (K == 0) return DGK_OK;
-  switch (p->N) {
-#define DGK_CASE(n) \
-  case n: return launch_affine_n<n>(p, K, u_dev, geo_dev, geo_stride, dudx_dev, dudy_dev, dudz_dev);
-    DGK_CASE(1) DGK_CASE(2) DGK_CASE(3) DGK_CASE(4) DGK_CASE(5) DGK_CASE(6)
-#undef DGK_CASE
-  }
-  dgk::set_error("dgk_phys_deriv_affine: N=%d", p->N);
-  return DGK_ERR_UNSUPPORTED;
+  AffineArgs a;
+  a.K = K;
+  a.frag = p->frag;
+  a.lam = p->lam;
+  std::memcpy(a.vtx, p->vtx, sizeof a.vtx);
+  a.u = u_dev;
+  a.geo = geo_dev;
+  a.gs = geo_stride;
+  a.ox = dudx_dev;
+  a.oy = dudy_dev;
+  a.oz = dudz_dev;
+  return dispatch(p, &a, false);
 }

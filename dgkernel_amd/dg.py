@@ -243,22 +243,25 @@ class DerivPlan:
     """Affine physical derivative on matrix cores (dgk_deriv_plan).
 
     ``prec`` is one of ``_lib.DGK_PREC_F64``, ``DGK_PREC_F32``,
-    ``DGK_PREC_BF16X3``; outputs are FP64 for the first and FP32 otherwise.
-    ``D`` is (Dr, Ds, Dt), Np x Np row-major.
+    ``DGK_PREC_BF16X3``, ``DGK_PREC_BF16X6``; outputs are FP64 for the first
+    and FP32 otherwise. ``D`` is (Dr, Ds, Dt), Np x Np row-major; ``rst`` the
+    reference node coordinates (required below FP64).
     """
 
-    def __init__(self, dev: _lib.Device, N: int, D, prec: int = _lib.DGK_PREC_F64):
+    def __init__(self, dev: _lib.Device, N: int, D, prec: int = _lib.DGK_PREC_F64, rst=None):
         self.dev, self.N, self.prec = dev, N, prec
         D = [np.ascontiguousarray(m, dtype=np.float64) for m in D]
+        R = [None] * 3 if rst is None else [np.ascontiguousarray(v, dtype=np.float64) for v in rst]
         h = C.c_void_p()
-        P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
-        _lib.call("dgk_deriv_plan_create", dev.h, N, *(P(m) for m in D), prec, C.byref(h))
+        P = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        _lib.call("dgk_deriv_plan_create", dev.h, N, *(P(m) for m in D), *(P(v) for v in R), prec,
+                  C.byref(h))
         self.h = h
         self.out_dtype = np.float64 if prec == _lib.DGK_PREC_F64 else np.float32
 
     @classmethod
     def from_ref(cls, dev: _lib.Device, ref: RefTet, prec: int = _lib.DGK_PREC_F64):
-        return cls(dev, ref.N, (ref.Dr, ref.Ds, ref.Dt), prec)
+        return cls(dev, ref.N, (ref.Dr, ref.Ds, ref.Dt), prec, (ref.r, ref.s, ref.t))
 
     def apply(self, K: int, u, geo_ptr, geo_stride: int, outs) -> None:
         """Enqueue d/dx, d/dy, d/dz of dense [K, Np] ``u`` (device buffer) into ``outs``."""

@@ -173,19 +173,26 @@ int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int64_t* K_host
 
 /* Affine elements (native layout): the operator is prepared once per order
  * and precision (MFMA fragments), then applied to K elements:
- *   u_dev: dense element-major [k*Np + n], FP64;
+ *   u_dev: dense element-major [k*Np + n], FP64, 16-byte aligned;
  *   geo_dev: SoA rows rx ry rz sx sy sz tx ty tz at geo_dev[q*geo_stride + k]
  *            (e.g. the first 9 rows of dgk_dgmesh_geometry_dev, stride K);
  *   outputs dense element-major, FP64 for DGK_PREC_F64, FP32 otherwise.
  * DGK_PREC_F64: FP64 MFMA (v_mfma_f64_16x16x4), FP64 everywhere.
- * DGK_PREC_F32: operator and u rounded to FP32, FP32 MFMA, FP32 accumulate.
- * DGK_PREC_BF16X3: operator and u split into bf16 hi + lo parts, three bf16
- *   MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulate (Henry, Tang &
- *   Heinecke, ARITH 2019). Metrics are applied in FP32 for the FP32 outputs. */
-enum { DGK_PREC_F64 = 0, DGK_PREC_F32 = 1, DGK_PREC_BF16X3 = 2 };
+ * Below FP64 each element's linear interpolant through its vertex nodes is
+ * removed in FP64 before rounding and its (exact) derivative added back in
+ * FP64, so rounding scales with |u - Lu| rather than |u|; r, s, t (the
+ * reference nodes) locate the vertex nodes and are required.
+ * DGK_PREC_F32: operator and remainder rounded to FP32, FP32 MFMA.
+ * DGK_PREC_BF16X3: two-part bf16 splits, hi*hi + hi*lo + lo*hi
+ *   (Henry, Tang & Heinecke, ARITH 2019), ~16 bits per operand.
+ * DGK_PREC_BF16X6: three-part bf16 splits, six products, ~24 bits.
+ * Low-precision modes accumulate in FP32 and write FP32 outputs; metrics
+ * are applied in FP64. */
+enum { DGK_PREC_F64 = 0, DGK_PREC_F32 = 1, DGK_PREC_BF16X3 = 2, DGK_PREC_BF16X6 = 3 };
 typedef struct dgk_deriv_plan_s* dgk_deriv_plan;
 int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr_host, const double* Ds_host,
-                          const double* Dt_host, int prec, dgk_deriv_plan* out);
+                          const double* Dt_host, const double* r_host, const double* s_host,
+                          const double* t_host, int prec, dgk_deriv_plan* out);
 int dgk_deriv_plan_destroy(dgk_deriv_plan plan);
 int dgk_phys_deriv_affine(dgk_deriv_plan plan, int64_t K, const double* u_dev,
                           const double* geo_dev, int64_t geo_stride, void* dudx_dev,

@@ -13,18 +13,19 @@ from tests.helpers import rel_l2
 pytestmark = pytest.mark.gpu
 
 METRICS = ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz")
-PRECS = {"f64": _lib.DGK_PREC_F64, "f32": _lib.DGK_PREC_F32, "bf16x3": _lib.DGK_PREC_BF16X3}
+PRECS = {"f64": _lib.DGK_PREC_F64, "f32": _lib.DGK_PREC_F32, "bf16x3": _lib.DGK_PREC_BF16X3,
+         "bf16x6": _lib.DGK_PREC_BF16X6}
 # Bounds on the error against FP64 (relative L2, max error scaled by max |du|)
 # for FP32-accumulated products of length Np <= 84 on O(1) data. FP32
 # operands carry 24 bits (measured ~1e-7); the two-part bf16 split carries
 # 16 bits per operand and drops lo*lo, so BF16x3 sits near 2^-16 / sqrt(Np)
-# (measured ~2e-6).
-BOUND = {"f32": (1e-6, 1e-5), "bf16x3": (1e-5, 5e-5)}
+# (measured ~2e-6); the three-part split (six products) is at FP32 level.
+BOUND = {"f32": (1e-6, 1e-5), "bf16x3": (1e-5, 5e-5), "bf16x6": (1e-6, 1e-5)}
 
 
-def _run(device, N, D, U, geo, prec):
+def _run(device, N, D, U, geo, prec, rst):
     K, Np = U.shape
-    plan = dg.DerivPlan(device, N, D, prec)
+    plan = dg.DerivPlan(device, N, D, prec, rst)
     du = device.upload(np.ascontiguousarray(U))
     dg_ = device.upload(np.ascontiguousarray(geo))
     outs = [device.zeros(K * Np, plan.out_dtype) for _ in range(3)]
@@ -33,6 +34,10 @@ def _run(device, N, D, U, geo, prec):
     res = [o.to_host().reshape(K, Np).astype(np.float64) for o in outs]
     plan.free()
     return res
+
+
+def _rst(el):
+    return (el["r"], el["s"], el["t"])
 
 
 def _mesh_case(N, seed=21):
@@ -55,17 +60,17 @@ def _mesh_case(N, seed=21):
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_affine_fp64_vs_oracle(device, N):
     el, U, geo, host = _mesh_case(N)
-    got = _run(device, N, (el["Dr"], el["Ds"], el["Dt"]), U, geo, PRECS["f64"])
+    got = _run(device, N, (el["Dr"], el["Ds"], el["Dt"]), U, geo, PRECS["f64"], _rst(el))
     for d in range(3):
         scale = np.max(np.abs(host[d]))
         assert np.max(np.abs(got[d] - host[d])) <= 1e-13 * scale
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
-@pytest.mark.parametrize("N", [1, 3, 4, 6])
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_affine_mixed_error_bounds(device, N, prec):
     el, U, geo, host = _mesh_case(N, seed=22)
-    got = _run(device, N, (el["Dr"], el["Ds"], el["Dt"]), U, geo, PRECS[prec])
+    got = _run(device, N, (el["Dr"], el["Ds"], el["Dt"]), U, geo, PRECS[prec], _rst(el))
     l2b, maxb = BOUND[prec]
     for d in range(3):
         e2 = rel_l2(got[d], host[d])
@@ -76,10 +81,11 @@ def test_affine_mixed_error_bounds(device, N, prec):
         assert e2 > 0.0
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
-def test_affine_grid_stride_and_ragged(device, prec):
-    """K above one sweep of the persistent grid, K % 16 != 0, random metrics."""
-    N = 4
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("N", [4, 6])
+def test_affine_grid_stride_and_ragged(device, prec, N):
+    """K above one sweep of the persistent grid, K % 16 != 0 (K Np odd at N=4),
+    random metrics."""
     el = O.element(N)
     Np = el["Np"]
     K = 70001
@@ -88,7 +94,7 @@ def test_affine_grid_stride_and_ragged(device, prec):
     stride = K + 37
     geo_s = np.zeros((9, stride))
     geo_s[:, :K] = geo
-    got = _run(device, N, (el["Dr"], el["Ds"], el["Dt"]), U, geo_s, PRECS[prec])
+    got = _run(device, N, (el["Dr"], el["Ds"], el["Dt"]), U, geo_s, PRECS[prec], _rst(el))
     mets = [np.repeat(geo[q][:, None], Np, axis=1) for q in range(9)]
     host = O.phys_deriv(el["Dr"], el["Ds"], el["Dt"], U, mets)
     for d in range(3):
@@ -100,6 +106,8 @@ def test_affine_grid_stride_and_ragged(device, prec):
 def test_affine_empty_and_errors(device):
     el = O.element(2)
     plan = dg.DerivPlan(device, 2, (el["Dr"], el["Ds"], el["Dt"]))
+    with pytest.raises(_lib.DgkError):  # node coordinates required below FP64
+        dg.DerivPlan(device, 2, (el["Dr"], el["Ds"], el["Dt"]), _lib.DGK_PREC_F32)
     buf = device.zeros(16)
     plan.apply(0, buf, buf.ptr, 0, [buf, buf, buf])  # K = 0 is a no-op
     with pytest.raises(_lib.DgkError):
