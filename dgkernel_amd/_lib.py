@@ -107,6 +107,7 @@ _SIGS = {
     "dgk_cfl_dt": (_i32, [_vp, _vp, _f64, _f64, _vp]),
     "dgk_burgers_set_tile": (_i32, [_vp, _i32, _i32]),
     "dgk_burgers_set_variant": (_i32, [_vp, _i32]),
+    "dgk_burgers_set_precision": (_i32, [_vp, _i32]),
     "dgk_burgers_set_order": (_i32, [_vp, _i32]),
     "dgk_kernel_set_matrix": (_i32, [_vp, C.c_char_p, _i32, _i32, _pf64]),
 }

@@ -26,37 +26,16 @@
 //    partition, so uneven partition sizes cost no padding.
 #include <cmath>
 
-#include "dgk_internal.h"
+#include "dgmesh_internal.h"
 
 namespace {
 
-template <int N>
-struct Tet {
-  static constexpr int Np = (N + 1) * (N + 2) * (N + 3) / 6;
-  static constexpr int Nfp = (N + 1) * (N + 2) / 2;
-};
+using dgk::FmaskTable;
+using dgk::kBGeoRows;
+using dgk::kGeoRows;
+using dgk::Tet;
+using dgk::Tile;
 
-// Face masks of the NUDG node ordering (t=-1, s=-1, r+s+t=-1, r=-1), derived
-// from the lattice indices of EquiNodes3D (element/library/gonudg/nodes3d.go:
-// 191-201); the warp & blend map keeps face nodes on their faces. Checked
-// against the caller's Fmask in dgk_dgmesh_create.
-template <int N>
-struct FmaskTable {
-  int v[4][Tet<N>::Nfp];
-  constexpr FmaskTable() : v{} {
-    int c[4] = {0, 0, 0, 0};
-    int sk = 0;
-    for (int n = 0; n <= N; ++n)
-      for (int m = 0; m <= N - n; ++m)
-        for (int l = 0; l <= N - n - m; ++l) {
-          if (n == 0) v[0][c[0]++] = sk;
-          if (m == 0) v[1][c[1]++] = sk;
-          if (l + m + n == N) v[2][c[2]++] = sk;
-          if (l == 0) v[3][c[3]++] = sk;
-          ++sk;
-        }
-  }
-};
 
 std::vector<int> host_fmask(int N) {
   std::vector<int> v;
@@ -75,14 +54,7 @@ std::vector<int> host_fmask(int N) {
   return v;
 }
 
-struct Tile {
-  int64_t vbase;  // value offset of the tile's first element in state arrays
-  int32_t gbase;  // dense global index of the tile's first element
-  int32_t count;  // elements in the tile
-};
 
-constexpr int kGeoRows = 26;   // rx..tz, J, 4 x (nx, ny, nz, Fscale)
-constexpr int kBGeoRows = 11;  // rx+ry+rz, sx+sy+sz, tx+ty+tz, 4 x n.(1,1,1), 4 x Fscale
 
 // FP64 affine geometry per element from its four vertices: the
 // GeometricFactors3D / Normals3D / ComputeFscale formulas
@@ -587,24 +559,6 @@ __global__ void min_node_distance_kernel(int64_t K, const double* __restrict__ v
 
 }  // namespace
 
-struct dgk_dgmesh_s {
-  dgk_device dev = nullptr;
-  int N = 0, Np = 0, Nfp = 0, npart = 0;
-  int64_t K = 0;
-  std::vector<int64_t> Kp, state_off;
-  double *Dr = nullptr, *Ds = nullptr, *Dt = nullptr, *LIFT = nullptr, *rst = nullptr;
-  uint8_t* face_nodes = nullptr;
-  double *vtx = nullptr, *geo = nullptr, *bgeo = nullptr;
-  int32_t *nbr = nullptr, *codes = nullptr;
-  Tile* tiles = nullptr;      // tiles of tb elements (VALU kernel)
-  int ntiles = 0;
-  Tile* tiles64 = nullptr;    // tiles of 16 elements (MFMA kernel, one per wave step)
-  int ntiles64 = 0;
-  int tbm = 64;               // MFMA block: 16 elements per wave, 1/2/4 waves per block
-  int tb = 64, blocks_per_cu = 0, chunked = 0;
-  int variant = 0;            // 0 auto, 1 VALU, 2 MFMA
-  unsigned long long* scratch = nullptr;  // reduction slot for synchronous helpers
-};
 
 namespace {
 
@@ -684,10 +638,16 @@ int launch_stage_mfma(dgk_dgmesh m, const StageParams& p) {
 
 bool use_mfma(dgk_dgmesh m, int mode) {
   if (mode == 2 || m->N > 4) return false;
-  return m->variant != 1;
+  return m->variant != 1 && m->variant != 3;
+}
+
+bool use_split(dgk_dgmesh m, int mode) {
+  return mode != 2 && (m->N > 4 || m->prec != 0 || m->variant == 3);
 }
 
 int launch_stage(dgk_dgmesh m, StageParams p) {
+  if (use_split(m, p.mode))
+    return dgk::split_stage(m, p.u, p.u0, p.out, p.a0, p.a1, p.dt, p.maxabs, p.mode);
   p.Dr = m->Dr; p.Ds = m->Ds; p.Dt = m->Dt; p.LIFT = m->LIFT;
   p.face_nodes = m->face_nodes;
   p.bgeo = m->bgeo;
@@ -713,7 +673,8 @@ int launch_stage(dgk_dgmesh m, StageParams p) {
     case 3: return launch_stage_n<3>(m, p);
     case 4: return launch_stage_n<4>(m, p);
     default:
-      dgk::set_error("burgers stage: order N=%d not built (1..4)", m->N);
+      dgk::set_error("burgers stage: the fused kernel and the trace are built for N <= 4 (N=%d)",
+                     m->N);
       return DGK_ERR_UNSUPPORTED;
   }
 }
@@ -785,6 +746,13 @@ extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_d
   }
   std::vector<double> rst(3 * Np);
   for (int i = 0; i < Np; ++i) { rst[i] = d->r[i]; rst[Np + i] = d->s[i]; rst[2 * Np + i] = d->t[i]; }
+  m->Dr_h.assign(d->Dr, d->Dr + (size_t)Np * Np);
+  m->Ds_h.assign(d->Ds, d->Ds + (size_t)Np * Np);
+  m->Dt_h.assign(d->Dt, d->Dt + (size_t)Np * Np);
+  m->LIFT_h.assign(d->LIFT, d->LIFT + (size_t)Np * 4 * Nfp);
+  m->r_h.assign(d->r, d->r + Np);
+  m->s_h.assign(d->s, d->s + Np);
+  m->t_h.assign(d->t, d->t + Np);
   int rc = DGK_OK;
   rc |= dgk_malloc(dev, (size_t)Np * Np * 8, d->Dr, (void**)&m->Dr);
   rc |= dgk_malloc(dev, (size_t)Np * Np * 8, d->Ds, (void**)&m->Ds);
@@ -816,6 +784,13 @@ extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_d
     dgk::set_error("dgk_dgmesh_create: %d elements with non-positive Jacobian", nbad);
     return DGK_ERR_NUMERIC;
   }
+  if (use_split(m, 0)) {
+    rc = dgk::split_prepare(m);
+    if (rc != DGK_OK) {
+      dgk_dgmesh_destroy(m);
+      return rc;
+    }
+  }
   *out = m;
   return DGK_OK;
 }
@@ -823,6 +798,7 @@ extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_d
 extern "C" int dgk_dgmesh_destroy(dgk_dgmesh m) {
   if (!m) return DGK_OK;
   dgk_device dev = m->dev;
+  dgk::split_release(m);
   void* ptrs[] = {m->Dr, m->Ds, m->Dt, m->LIFT, m->rst, m->face_nodes, m->vtx, m->geo,
                   m->bgeo, m->nbr, m->codes, m->tiles, m->tiles64, m->scratch};
   for (void* q : ptrs) dgk_free(dev, q);
@@ -934,10 +910,10 @@ extern "C" int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked) {
 extern "C" int dgk_burgers_set_variant(dgk_dgmesh m, int variant) {
   dgk::clear_error();
   DGK_REQUIRE(m, "dgk_burgers_set_variant: NULL mesh");
-  DGK_REQUIRE(variant >= 0 && variant <= 2, "dgk_burgers_set_variant: variant %d", variant);
+  DGK_REQUIRE(variant >= 0 && variant <= 3, "dgk_burgers_set_variant: variant %d", variant);
   DGK_REQUIRE(variant != 2 || m->N <= 4, "dgk_burgers_set_variant: MFMA stage built for N <= 4");
   m->variant = variant;
-  return DGK_OK;
+  return use_split(m, 0) ? dgk::split_prepare(m) : DGK_OK;
 }
 
 extern "C" int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu) {
@@ -959,4 +935,15 @@ extern "C" int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int block
     return build_tiles(m);
   }
   return DGK_OK;
+}
+
+extern "C" int dgk_burgers_set_precision(dgk_dgmesh m, int prec) {
+  dgk::clear_error();
+  DGK_REQUIRE(m, "dgk_burgers_set_precision: NULL mesh");
+  DGK_REQUIRE(prec >= 0 && prec <= 3, "dgk_burgers_set_precision: precision %d", prec);
+  if (prec != m->prec) {
+    dgk::split_release(m);
+    m->prec = prec;
+  }
+  return use_split(m, 0) ? dgk::split_prepare(m) : DGK_OK;
 }

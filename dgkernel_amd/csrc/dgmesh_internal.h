@@ -1,0 +1,98 @@
+// Device-mesh internals shared by the fused stage (burgers.hip) and the
+// split stage (split.hip): element shape constants, face masks, element
+// tiles and the dgk_dgmesh_s state.
+#pragma once
+
+#include <vector>
+
+#include "dgk_internal.h"
+
+namespace dgk {
+
+template <int N>
+struct Tet {
+  static constexpr int Np = (N + 1) * (N + 2) * (N + 3) / 6;
+  static constexpr int Nfp = (N + 1) * (N + 2) / 2;
+};
+
+// Face masks of the NUDG node ordering (t=-1, s=-1, r+s+t=-1, r=-1), derived
+// from the lattice indices of EquiNodes3D (element/library/gonudg/nodes3d.go:
+// 191-201); the warp & blend map keeps face nodes on their faces. Checked
+// against the caller's Fmask in dgk_dgmesh_create.
+template <int N>
+struct FmaskTable {
+  int v[4][Tet<N>::Nfp];
+  constexpr FmaskTable() : v{} {
+    int c[4] = {0, 0, 0, 0};
+    int sk = 0;
+    for (int n = 0; n <= N; ++n)
+      for (int m = 0; m <= N - n; ++m)
+        for (int l = 0; l <= N - n - m; ++l) {
+          if (n == 0) v[0][c[0]++] = sk;
+          if (m == 0) v[1][c[1]++] = sk;
+          if (l + m + n == N) v[2][c[2]++] = sk;
+          if (l == 0) v[3][c[3]++] = sk;
+          ++sk;
+        }
+  }
+};
+
+struct Tile {
+  int64_t vbase;  // value offset of the tile's first element in state arrays
+  int32_t gbase;  // dense global index of the tile's first element
+  int32_t count;  // elements in the tile
+};
+
+constexpr int kGeoRows = 26;   // rx..tz, J, 4 x (nx, ny, nz, Fscale)
+constexpr int kBGeoRows = 11;  // rx+ry+rz, sx+sy+sz, tx+ty+tz, 4 x n.(1,1,1), 4 x Fscale
+
+}  // namespace dgk
+
+struct dgk_dgmesh_s {
+  dgk_device dev = nullptr;
+  int N = 0, Np = 0, Nfp = 0, npart = 0;
+  int64_t K = 0;
+  std::vector<int64_t> Kp, state_off;
+  double *Dr = nullptr, *Ds = nullptr, *Dt = nullptr, *LIFT = nullptr, *rst = nullptr;
+  uint8_t* face_nodes = nullptr;
+  double *vtx = nullptr, *geo = nullptr, *bgeo = nullptr;
+  int32_t *nbr = nullptr, *codes = nullptr;
+  dgk::Tile* tiles = nullptr;      // tiles of tb elements (VALU kernel)
+  int ntiles = 0;
+  dgk::Tile* tiles64 = nullptr;    // tiles of 16 elements (MFMA kernel, one per wave step)
+  int ntiles64 = 0;
+  int tbm = 64;               // MFMA block: 16 elements per wave, 1/2/4 waves per block
+  int tb = 64, blocks_per_cu = 0, chunked = 0;
+  int variant = 0;            // 0 auto, 1 VALU, 2 MFMA
+  unsigned long long* scratch = nullptr;  // reduction slot for synchronous helpers
+  // Host copies of the reference operators (row-major) and nodes.
+  std::vector<double> Dr_h, Ds_h, Dt_h, LIFT_h, r_h, s_h, t_h;
+  // Split stage (orders above 4, or contractions below FP64): volume term
+  // through a derivative plan into rhs, then flux + LIFT + RK update.
+  int prec = 0;                          // DGK_PREC_* of the D and LIFT contractions
+  dgk_deriv_plan plan = nullptr;         // built on first use for (N, prec)
+  double* rhs = nullptr;                 // state-sized FP64 volume term
+  void* lift_frag = nullptr;             // LIFT fragments for prec
+  int split_blocks = 0;                  // persistent grid of the surface kernel
+};
+
+namespace dgk {
+
+// Split stage for orders above 4 or contractions below FP64 (split.hip):
+// mode 0: out = a0 u0 + a1 (u + dt L(u)) (max |out| into maxabs if given);
+// mode 1: out = L(u).
+int split_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0, double a1,
+                const double* dt, unsigned long long* maxabs, int mode);
+// Builds the split stage's plan, FP64 volume buffer and LIFT fragments for
+// (N, prec) if not there yet (allocates: call outside graph capture).
+int split_prepare(dgk_dgmesh m);
+void split_release(dgk_dgmesh m);
+
+// Volume term of the Burgers RHS through a derivative plan (mixed.hip):
+// out = -(div f(u)), f(u) = u^2/2 (1,1,1), for the elements of the tiles
+// (16 elements each, runner layout), FP64 output, geo = the 26-row
+// geometry with stride gs.
+int deriv_plan_divergence(dgk_deriv_plan plan, const Tile* tiles, int ntiles, const double* u,
+                          const double* geo, int64_t gs, double* out);
+
+}  // namespace dgk

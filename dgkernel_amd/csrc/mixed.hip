@@ -39,7 +39,7 @@
 #include <type_traits>
 #include <vector>
 
-#include "dgk_internal.h"
+#include "dgmesh_internal.h"
 
 namespace {
 
@@ -124,7 +124,9 @@ template <int PREC>
 using ResT = typename std::conditional<PREC == kF64, double, float>::type;
 
 struct AffineArgs {
-  int64_t K;
+  int64_t K;                // dense mode: elements; tile mode: unused
+  const dgk::Tile* tiles;   // tile mode: 16-element tiles in the runner layout (else dense)
+  int64_t ntiles;
   const void* frag;    // operator fragments, [part][MT][KS][64]
   const double* lam;   // [3][NP] barycentric weights l1, l2, l3 of each node (mixed modes)
   int vtx[4];          // indices of the vertex nodes (mixed modes)
@@ -136,7 +138,11 @@ struct AffineArgs {
   void* oz;
 };
 
-template <int N, int PREC>
+// DIV = false: du/dx, du/dy, du/dz into ox, oy, oz (FP64 or FP32).
+// DIV = true: the Burgers volume term -div f(u), f(u) = u^2/2 (1,1,1), into
+// ox (FP64): the staged values are f, and the epilogue folds the metrics to
+// -( (rx+ry+rz) Dr f + (sx+sy+sz) Ds f + (tx+ty+tz) Dt f ).
+template <int N, int PREC, bool DIV>
 __global__ void __launch_bounds__((ACfg<N, PREC>::NT))
 __attribute__((amdgpu_waves_per_eu(ACfg<N, PREC>::WPE)))
 deriv_affine_kernel(AffineArgs a) {
@@ -179,30 +185,39 @@ deriv_affine_kernel(AffineArgs a) {
   __builtin_amdgcn_s_waitcnt(0);
 
   const int64_t K = a.K;
-  const int64_t ntiles = (K + 15) / 16;
-  const int64_t nval = K * NP;
+  const int64_t ntiles = a.tiles ? a.ntiles : (K + 15) / 16;
   const int64_t Gd = gridDim.x;
   const int64_t b0 = blockIdx.x;
   const int64_t nb = b0 < ntiles ? (ntiles - b0 + Gd - 1) / Gd : 0;  // tiles of this block
   constexpr int PD = 2;  // tiles whose loads are in flight per block
   if (nb == 0) return;
 
-  // Tile loads through a buffer resource based at the tile: the per-thread
-  // offsets are loop-invariant registers, and reads past the end of u return
-  // zero. Values of a pair that straddles or passes the end are fixed up when
-  // staged.
-  const int64_t last_pair = nval / 2 - 1;
-  auto prefetch = [&](int64_t n, doublex2 (&P)[NL], double& PM) {
-    const int64_t t = b0 + n * Gd;
-    if (mt < 9 * 16) {
-      const int64_t e = t * 16 + (mt & 15);
-      PM = a.geo[(mt >> 4) * a.gs + (e < K ? e : K - 1)];
+  // Tile t: value offset of its first element, first element (metrics),
+  // element count.
+  struct TileRef {
+    int64_t vbase, gbase;
+    int count;
+  };
+  auto tile_ref = [&](int64_t t) -> TileRef {
+    if (a.tiles) {
+      const dgk::Tile tl = a.tiles[t];
+      return {tl.vbase, (int64_t)tl.gbase, (int)tl.count};
     }
-    const int64_t first = t * (16 * NP);  // first value of the tile
-    const int64_t left = (nval - first) * 8;
-    const int bytes = (int)(left < 16 * NP * 8 ? left : 16 * NP * 8);
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void*)(a.u + first), 0, bytes, 0x00020000);
+    const int64_t left = K - t * 16;
+    return {t * 16 * NP, t * 16, (int)(left < 16 ? left : 16)};
+  };
+
+  // Tile loads through a buffer resource based at the tile: the per-thread
+  // offsets are loop-invariant registers, and reads past the tile's values
+  // return zero. A pair that straddles the end is fixed up when staged.
+  auto prefetch = [&](int64_t n, doublex2 (&P)[NL], double& PM) {
+    const TileRef tr = tile_ref(b0 + n * Gd);
+    if (mt < 9 * 16) {
+      const int e = mt & 15;
+      PM = a.geo[(mt >> 4) * a.gs + tr.gbase + (e < tr.count ? e : 0)];
+    }
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.u + tr.vbase), 0, tr.count * NP * 8, 0x00020000);
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
       const int v = mt + q * NTM;
@@ -210,21 +225,26 @@ deriv_affine_kernel(AffineArgs a) {
     }
   };
 
-  // stage(n): registers of slot n % PD -> sU[n % 2], sM[n % 4]; refills the slot.
+  // stage(n): registers of slot n % PD -> sU[n % 2] (u, or f = u^2/2 for
+  // DIV), sM[n % 4]; refills the slot.
   auto stage = [&](int64_t n, int bu, int bm, doublex2 (&P)[NL], double& PM) {
-    const int64_t t = b0 + n * Gd;
+    const TileRef tr = tile_ref(b0 + n * Gd);
+    const int lim = tr.count * NP;  // values of this tile
     double* U = sU[bu];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
       const int v = mt + q * NTM;
       if (v < NV2) {
-        const int64_t p = t * (8 * NP) + v;  // pair index in u
+        const int i0 = 2 * v, i1 = 2 * v + 1;
         double x0 = P[q].x, x1 = P[q].y;
-        if (p > last_pair) {
-          x0 = (p == last_pair + 1 && (nval & 1)) ? a.u[nval - 1] : 0.0;
+        if (i1 >= lim) {
+          x0 = i0 < lim ? a.u[tr.vbase + i0] : 0.0;
           x1 = 0.0;
         }
-        const int i0 = 2 * v, i1 = 2 * v + 1;
+        if constexpr (DIV) {
+          x0 = 0.5 * x0 * x0;
+          x1 = 0.5 * x1 * x1;
+        }
         U[(i0 / NP) * NPU + i0 % NP] = x0;
         U[(i1 / NP) * NPU + i1 % NP] = x1;
       }
@@ -232,10 +252,9 @@ deriv_affine_kernel(AffineArgs a) {
     if constexpr (NTM >= 9 * 16) {
       if (mt < 9 * 16) sM[bm][(mt & 15) * 12 + (mt >> 4)] = PM;
     } else {
-      const int ne = (int)((K - t * 16) < 16 ? (K - t * 16) : 16);
       for (int v = mt; v < 9 * 16; v += NTM) {
         const int q = v >> 4, e = v & 15;
-        sM[bm][e * 12 + q] = e < ne ? a.geo[q * a.gs + t * 16 + e] : 0.0;
+        sM[bm][e * 12 + q] = e < tr.count ? a.geo[q * a.gs + tr.gbase + e] : 0.0;
       }
     }
     if (n + PD < nb) prefetch(n + PD, P, PM);
@@ -344,14 +363,12 @@ deriv_affine_kernel(AffineArgs a) {
 
   // epi(n): chain rule in FP64 and contiguous stores.
   auto epi = [&](int64_t n, int br, int bm) {
-    const int64_t t = b0 + n * Gd;
-    const int64_t e0 = t * 16;
-    const int ne = (int)((K - e0) < 16 ? (K - e0) : 16);
-    const int64_t obase = e0 * NP;
+    const TileRef tr = tile_ref(b0 + n * Gd);
+    const int lim = tr.count * NP;
 #pragma unroll
     for (int q = 0; q < NIT; ++q) {
       const int v = mt + q * NTM;
-      if (v < ne * NP) {
+      if (v < lim) {
         const int e = v / NP, i = v - e * NP;
         const R* re = sR[br] + e * ROWSP;
         const doublex2* M2 = (const doublex2*)(sM[bm] + e * 12);
@@ -363,17 +380,23 @@ deriv_affine_kernel(AffineArgs a) {
           us += l23.x;
           ut += l23.y;
         }
-        const double dx = m01.x * ur + m23.y * us + m67.x * ut;
-        const double dy = m01.y * ur + m45.x * us + m67.y * ut;
-        const double dz = m23.x * ur + m45.y * us + m8l.x * ut;
-        if constexpr (PREC == kF64) {
-          __builtin_nontemporal_store(dx, (double*)a.ox + obase + v);
-          __builtin_nontemporal_store(dy, (double*)a.oy + obase + v);
-          __builtin_nontemporal_store(dz, (double*)a.oz + obase + v);
+        if constexpr (DIV) {
+          const double ga = m01.x + m01.y + m23.x, gb = m23.y + m45.x + m45.y,
+                       gc = m67.x + m67.y + m8l.x;
+          __builtin_nontemporal_store(-(ga * ur + gb * us + gc * ut), (double*)a.ox + tr.vbase + v);
         } else {
-          __builtin_nontemporal_store((float)dx, (float*)a.ox + obase + v);
-          __builtin_nontemporal_store((float)dy, (float*)a.oy + obase + v);
-          __builtin_nontemporal_store((float)dz, (float*)a.oz + obase + v);
+          const double dx = m01.x * ur + m23.y * us + m67.x * ut;
+          const double dy = m01.y * ur + m45.x * us + m67.y * ut;
+          const double dz = m23.x * ur + m45.y * us + m8l.x * ut;
+          if constexpr (PREC == kF64) {
+            __builtin_nontemporal_store(dx, (double*)a.ox + tr.vbase + v);
+            __builtin_nontemporal_store(dy, (double*)a.oy + tr.vbase + v);
+            __builtin_nontemporal_store(dz, (double*)a.oz + tr.vbase + v);
+          } else {
+            __builtin_nontemporal_store((float)dx, (float*)a.ox + tr.vbase + v);
+            __builtin_nontemporal_store((float)dy, (float*)a.oy + tr.vbase + v);
+            __builtin_nontemporal_store((float)dz, (float*)a.oz + tr.vbase + v);
+          }
         }
       }
     }
@@ -487,46 +510,54 @@ float bf16_to_float(uint16_t b) {
   return f;
 }
 
+// Persistent grid: blocks resident per CU (the same for both modes) times
+// the CU count.
 template <int N, int PREC>
 int occupancy_blocks(dgk_deriv_plan pl) {
   int per_cu = 0;
-  DGK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, deriv_affine_kernel<N, PREC>,
-                                                       ACfg<N, PREC>::NT, 0));
+  DGK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, deriv_affine_kernel<N, PREC, false>, ACfg<N, PREC>::NT, 0));
   if (per_cu < 1) per_cu = 1;
   pl->blocks = pl->dev->cu_count * per_cu;
   return DGK_OK;
 }
 
-template <int N, int PREC>
+template <int N, int PREC, bool DIV>
 int launch_affine_t(dgk_deriv_plan pl, const AffineArgs& a) {
-  const int64_t ntiles = (a.K + 15) / 16;
+  const int64_t ntiles = a.tiles ? a.ntiles : (a.K + 15) / 16;
   const int64_t blocks = pl->blocks < ntiles ? pl->blocks : ntiles;
   if (blocks < 1) return DGK_OK;
-  hipLaunchKernelGGL((deriv_affine_kernel<N, PREC>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((deriv_affine_kernel<N, PREC, DIV>), dim3((unsigned)blocks),
                      dim3(ACfg<N, PREC>::NT), 0, pl->dev->stream, a);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
-// occupancy == true: size the persistent grid; otherwise launch with *a.
+// what: 0 size the persistent grid, 1 gradient launch, 2 divergence launch.
+template <int N, int PREC>
+int dispatch_p(dgk_deriv_plan pl, const AffineArgs* a, int what) {
+  if (what == 0) return occupancy_blocks<N, PREC>(pl);
+  return what == 1 ? launch_affine_t<N, PREC, false>(pl, *a) : launch_affine_t<N, PREC, true>(pl, *a);
+}
+
 template <int N>
-int dispatch_n(dgk_deriv_plan pl, const AffineArgs* a, bool occupancy) {
+int dispatch_n(dgk_deriv_plan pl, const AffineArgs* a, int what) {
   switch (pl->prec) {
-    case kF64: return occupancy ? occupancy_blocks<N, kF64>(pl) : launch_affine_t<N, kF64>(pl, *a);
-    case kF32: return occupancy ? occupancy_blocks<N, kF32>(pl) : launch_affine_t<N, kF32>(pl, *a);
-    case kBF3: return occupancy ? occupancy_blocks<N, kBF3>(pl) : launch_affine_t<N, kBF3>(pl, *a);
-    default: return occupancy ? occupancy_blocks<N, kBF6>(pl) : launch_affine_t<N, kBF6>(pl, *a);
+    case kF64: return dispatch_p<N, kF64>(pl, a, what);
+    case kF32: return dispatch_p<N, kF32>(pl, a, what);
+    case kBF3: return dispatch_p<N, kBF3>(pl, a, what);
+    default: return dispatch_p<N, kBF6>(pl, a, what);
   }
 }
 
-int dispatch(dgk_deriv_plan pl, const AffineArgs* a, bool occupancy) {
+int dispatch(dgk_deriv_plan pl, const AffineArgs* a, int what) {
   switch (pl->N) {
-    case 1: return dispatch_n<1>(pl, a, occupancy);
-    case 2: return dispatch_n<2>(pl, a, occupancy);
-    case 3: return dispatch_n<3>(pl, a, occupancy);
-    case 4: return dispatch_n<4>(pl, a, occupancy);
-    case 5: return dispatch_n<5>(pl, a, occupancy);
-    case 6: return dispatch_n<6>(pl, a, occupancy);
+    case 1: return dispatch_n<1>(pl, a, what);
+    case 2: return dispatch_n<2>(pl, a, what);
+    case 3: return dispatch_n<3>(pl, a, what);
+    case 4: return dispatch_n<4>(pl, a, what);
+    case 5: return dispatch_n<5>(pl, a, what);
+    case 6: return dispatch_n<6>(pl, a, what);
   }
   dgk::set_error("dgk_phys_deriv_affine: N=%d", pl->N);
   return DGK_ERR_UNSUPPORTED;
@@ -615,7 +646,7 @@ extern "C" int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr, co
   int rc = dgk_malloc(dev, buf.size(), buf.data(), &p->frag);
   if (rc == DGK_OK && !lam.empty())
     rc = dgk_malloc(dev, lam.size() * sizeof(double), lam.data(), (void**)&p->lam);
-  if (rc == DGK_OK) rc = dispatch(p, nullptr, true);
+  if (rc == DGK_OK) rc = dispatch(p, nullptr, 0);
   if (rc != DGK_OK) {
     if (p->frag) dgk_free(dev, p->frag);
     if (p->lam) dgk_free(dev, p->lam);
@@ -646,6 +677,8 @@ extern "C" int dgk_phys_deriv_affine(dgk_deriv_plan p, int64_t K, const double* 
   if (K == 0) return DGK_OK;
   AffineArgs a;
   a.K = K;
+  a.tiles = nullptr;
+  a.ntiles = 0;
   a.frag = p->frag;
   a.lam = p->lam;
   std::memcpy(a.vtx, p->vtx, sizeof a.vtx);
@@ -655,5 +688,24 @@ extern "C" int dgk_phys_deriv_affine(dgk_deriv_plan p, int64_t K, const double* 
   a.ox = dudx_dev;
   a.oy = dudy_dev;
   a.oz = dudz_dev;
-  return dispatch(p, &a, false);
+  return dispatch(p, &a, 1);
+}
+
+int dgk::deriv_plan_divergence(dgk_deriv_plan p, const dgk::Tile* tiles, int ntiles, const double* u,
+                               const double* geo, int64_t gs, double* out) {
+  if (ntiles == 0) return DGK_OK;
+  AffineArgs a;
+  a.K = 0;
+  a.tiles = tiles;
+  a.ntiles = ntiles;
+  a.frag = p->frag;
+  a.lam = p->lam;
+  std::memcpy(a.vtx, p->vtx, sizeof a.vtx);
+  a.u = u;
+  a.geo = geo;
+  a.gs = gs;
+  a.ox = out;
+  a.oy = out;
+  a.oz = out;
+  return dispatch(p, &a, 2);
 }

@@ -157,8 +157,12 @@ class DGMesh:
     def set_tile(self, elems_per_block: int, blocks_per_cu: int = 0) -> None:
         _lib.call("dgk_burgers_set_tile", self.h, elems_per_block, blocks_per_cu)
 
+    def set_precision(self, prec: int) -> None:
+        """DGK_PREC_* of the D and LIFT contractions of the stage (state stays FP64)."""
+        _lib.call("dgk_burgers_set_precision", self.h, prec)
+
     def set_variant(self, variant: int) -> None:
-        """0 automatic, 1 VALU stage kernel, 2 FP64 MFMA stage kernel."""
+        """0 automatic, 1 VALU stage kernel, 2 FP64 MFMA stage kernel, 3 split stage."""
         _lib.call("dgk_burgers_set_variant", self.h, variant)
 
     def set_order(self, xcd_chunked: bool) -> None:

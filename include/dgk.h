@@ -276,7 +276,7 @@ int dgk_dgmesh_trace_plus(dgk_dgmesh m, const double* u_dev, double* uP_dev);
 /* ---------------------------------------------------------------------- */
 int dgk_burgers_stage(dgk_dgmesh m, const double* u_dev, const double* u0_dev, double* out_dev,
                       double a0, double a1, const double* dt_dev, double* maxabs_dev);
-/* L(u) alone into rhs_dev (runner layout). */
+/* L(u) alone into rhs_dev (runner layout). N = 1..6. */
 int dgk_burgers_rhs(dgk_dgmesh m, const double* u_dev, double* rhs_dev);
 /* max|u| over all elements into *maxabs_dev (which must hold 0 or a
  * previous max). */
@@ -288,13 +288,23 @@ int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double hmin, doub
  * or 256) and blocks per CU of the persistent grid. 0 restores the tuned
  * default. Set the variant first. */
 int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu);
-/* Stage kernel: 0 automatic (FP64 MFMA for N <= 3, else VALU), 1 VALU (one
- * element per thread, FP64 FMA with scalar operator operands), 2 FP64 MFMA. */
+/* Stage kernel: 0 automatic (FP64 MFMA for N <= 4, split for N = 5, 6),
+ * 1 VALU (one element per thread, FP64 FMA with scalar operator operands),
+ * 2 FP64 MFMA, 3 split (volume kernel + surface kernel, see
+ * dgk_burgers_set_precision). */
 int dgk_burgers_set_variant(dgk_dgmesh m, int variant);
 /* Tile order of the persistent grid: 1 gives each XCD a contiguous range of
  * tiles (blocks renumbered around the round-robin XCD dispatch), 0 (default)
  * walks the tile list with a grid stride. */
 int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked);
+/* Precision of the Dr/Ds/Dt and LIFT contractions in the stage (DGK_PREC_*,
+ * default DGK_PREC_F64). The state, geometry, flux, RK combination and dt
+ * stay FP64. Below FP64, and for N = 5, 6 at any precision, the stage runs
+ * split: a volume kernel (-div f(u) through a derivative plan, with the
+ * linear-part removal of dgk_phys_deriv_affine) into a mesh-owned FP64
+ * buffer, then a surface kernel (LF flux, LIFT on matrix cores, RK update).
+ * The fused kernels (N <= 4, FP64) are used otherwise. */
+int dgk_burgers_set_precision(dgk_dgmesh m, int prec);
 
 #ifdef __cplusplus
 }

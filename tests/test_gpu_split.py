@@ -1,0 +1,115 @@
+"""Split Burgers stage (volume kernel + surface kernel): orders 5 and 6
+against the oracle, agreement with the fused kernels, and the D/LIFT
+contractions below FP64 against the FP64 GPU run."""
+import numpy as np
+import pytest
+
+from dgkernel_amd import _lib, dg
+from dgkernel_amd import mesh as M
+from oracle import oracle as O
+from tests.helpers import rel_l2, smooth_u0
+
+pytestmark = pytest.mark.gpu
+
+METRICS = ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz")
+
+
+def _metric_spread(o):
+    """Relative spread of the oracle's per-node metrics over an affine element:
+    they come from Dr X etc. and carry round-off ~ cond(V) eps (1e-11 at
+    P=5, 1e-10 at P=6), where the device uses one exact value per element."""
+    return max(float(np.max(np.abs(mm - mm[:, :1])) / np.max(np.abs(mm)))
+               for mm in (o.field(k) for k in METRICS))
+
+PRECS = {"f32": _lib.DGK_PREC_F32, "bf16x3": _lib.DGK_PREC_BF16X3, "bf16x6": _lib.DGK_PREC_BF16X6}
+
+
+@pytest.mark.parametrize("N,n,parts", [(5, 4, [100, 284]), (6, 4, [384]), (6, 5, [250, 1, 499])])
+def test_split_rhs_vs_oracle(device, N, n, parts):
+    m = M.cube_mesh(n, n, n, periodic=True, jitter=0.05, seed=N)
+    st = dg.make_setup(m, N, Kp=parts)
+    u = smooth_u0(st, noise=1e-3)
+    o = O.Setup.from_mesh(N, m)
+    want = o.burgers_rhs(u)
+    solver = dg.BurgersSolver(device, st)
+    got = solver.rhs(u)
+    tol = 1e-12 + 4.0 * _metric_spread(o)
+    assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("N,parts", [(2, [300, 468]), (3, [200, 350, 200]), (4, [163, 1, 186])])
+def test_split_matches_fused(device, N, parts):
+    m = M.cube_mesh(4, 4, 4, periodic=True) if N != 3 else M.cube_mesh(5, 5, 5, periodic=True)
+    parts = list(parts)
+    parts[-1] = m.K - sum(parts[:-1])
+    st = dg.make_setup(m, N, Kp=parts)
+    u0 = smooth_u0(st)
+    a = dg.BurgersSolver(device, st)
+    a.set_state(u0)
+    b = dg.BurgersSolver(device, st, hmin=a.hmin)
+    b.mesh.set_variant(3)
+    b.set_state(u0)
+    ra, rb = a.rhs(u0), b.rhs(u0)
+    assert np.max(np.abs(ra - rb)) <= 1e-12 * np.max(np.abs(ra))
+    a.step(5)
+    b.step(5)
+    assert rel_l2(b.state(), a.state()) <= 1e-13
+
+
+def test_p6_ssprk3_vs_oracle(device):
+    N = 6
+    m = M.cube_mesh(4, 4, 4, periodic=True)
+    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, 2))
+    u0 = smooth_u0(st, amp=0.4, noise=1e-6)
+    o = O.Setup.from_mesh(N, m)
+    hmin = o.min_node_distance()
+    want, dts = o.ssprk3(u0, 4, 0.3, hmin)
+    solver = dg.BurgersSolver(device, st, cfl=0.3, hmin=hmin)
+    solver.set_state(u0)
+    solver.step(4)
+    assert rel_l2(solver.state(), want) <= 1e-12 + _metric_spread(o)
+    assert abs(solver.dt() - dts[-1]) <= 1e-12 * dts[-1]
+
+
+# The contractions round the operators and the linear-part remainders to the
+# matrix precision; the state error after a few steps scales with dt times
+# that rounding. On this coarse mesh (h = 1/4) the remainder u - Lu is not
+# small, so the errors sit near the precision's unit round-off.
+BOUND = {"f32": 5e-5, "bf16x3": 5e-3, "bf16x6": 5e-5}
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
+def test_p6_mixed_precision_vs_fp64(device, prec):
+    N = 6
+    m = M.cube_mesh(4, 4, 4, periodic=True)
+    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, 2))
+    u0 = smooth_u0(st, amp=0.4, noise=1e-6)
+    ref = dg.BurgersSolver(device, st)
+    ref.set_state(u0)
+    ref.step(10)
+    lo = dg.BurgersSolver(device, st, hmin=ref.hmin)
+    lo.mesh.set_precision(PRECS[prec])
+    lo.set_state(u0)
+    lo.step(10)
+    a, b = lo.state(), ref.state()
+    e2 = rel_l2(a, b)
+    emax = float(np.max(np.abs(a - b)))
+    print(f"P=6 {prec} 10 steps: rel L2 {e2:.3e}, max {emax:.3e}")
+    assert 0.0 < e2 <= BOUND[prec]
+
+
+def test_split_graph_replay(device):
+    m = M.cube_mesh(4, 4, 4, periodic=True)
+    st = dg.make_setup(m, 6, Kp=[200, 184])
+    u0 = smooth_u0(st)
+    a = dg.BurgersSolver(device, st)
+    a.mesh.set_precision(_lib.DGK_PREC_BF16X3)
+    a.set_state(u0)
+    b = dg.BurgersSolver(device, st, hmin=a.hmin)
+    b.mesh.set_precision(_lib.DGK_PREC_BF16X3)
+    b.set_state(u0)
+    for rep in range(3):
+        a.step(2)
+        b.step(2, graph=True)
+        device.sync()
+        assert np.array_equal(a.state(), b.state()), f"replay {rep}"
