@@ -248,6 +248,62 @@ def bench_deriv(a, dev) -> dict:
     return out
 
 
+PREC_NAMES = ("f64", "f32", "bf16x3", "bf16x6")
+
+
+def bench_c4_stage(a, dev) -> dict:
+    """Mixed precision at high order: 64^3 x 6 = 1,572,864 periodic tets, P=6,
+    u0 = 0.5 + 0.4 sin(2 pi x) cos(2 pi y) sin(2 pi z) + U(+-1e-6), 2 slab
+    partitions, ``a.steps`` SSP-RK3 steps per precision of the D and LIFT
+    contractions. The state, geometry, flux and dt stay FP64; each precision's
+    final state is compared with the FP64 GPU run."""
+    from dgkernel_amd import _lib, dg
+    from dgkernel_amd import mesh as M
+    n, N = a.n, a.order
+    m = M.cube_mesh(n, n, n, periodic=True)
+    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, max(2, a.parts), 0.0, seed=a.seed))
+    x, y, z = st.node_coords()
+    u0 = (0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+          + M.uniform(a.seed, 7, x.size, -1e-6, 1e-6).reshape(x.shape))
+    del x, y, z
+    K, Np = m.K, st.ref.Np
+    ref = None
+    out = {"metric": "DOF-stage-updates/s per contraction precision (P=6 high-order case)",
+           "config": {"workload": f"Burgers SSP-RK3, {n}^3x6={K} periodic tets, P={N}, "
+                                  f"{st.npart} slab partitions, {a.steps} steps",
+                      "tets": K, "order": N, "Np": Np, "steps": a.steps},
+           "precisions": {}}
+    solver = None
+    hmin = None
+    precs = [p for p in PREC_NAMES if not a.precs or p == "f64" or p in a.precs.split(",")]
+    for name in precs:
+        solver = dg.BurgersSolver(dev, st, cfl=0.3, hmin=hmin)
+        hmin = solver.hmin
+        solver.mesh.set_precision(PREC_NAMES.index(name))
+        solver.set_state(u0)
+        solver.step(1)  # warm-up outside the timed run, restarted below
+        solver.set_state(u0)
+        dev.sync()
+        t = dev.timer()
+        t.start()
+        solver.step(a.steps, graph=not a.no_graph)
+        t.stop()
+        ms = t.ms()
+        u = solver.state()
+        rate = K * Np * 3 * a.steps / (ms * 1e-3)
+        rec = {"ms_per_step": ms / a.steps, "value": rate, "final_dt": solver.dt()}
+        if ref is None:
+            ref = u
+        else:
+            rec["err_vs_f64"] = _errs(u, ref)
+        out["precisions"][name] = rec
+        log(f"c4 stage {name}: {ms / a.steps:.3f} ms/step, {rate:.3e} DOF-stage/s, "
+            f"{rec.get('err_vs_f64', '')}")
+        del solver
+    out["device"] = dev.name
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,12 +315,17 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tile", type=int, default=0, help="elements per block (0 = tuned)")
     ap.add_argument("--bpc", type=int, default=0, help="blocks per CU (0 = tuned)")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="stage kernel (0 auto, 1 VALU, 2 MFMA pipelined, 3 split, 4 MFMA)")
+    ap.add_argument("--xcd", type=int, default=-1, help="1: XCD-swept tile order")
+    ap.add_argument("--morton", action="store_true",
+                    help="renumber elements in Morton order within each partition")
     ap.add_argument("--cpu-n", type=int, default=48, help="hexes per side of the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--mode", choices=["stage", "deriv"], default="stage")
+    ap.add_argument("--mode", choices=["stage", "deriv", "c4"], default="stage")
     ap.add_argument("--case", choices=["c2", "c4"], default="c4", help="derivative workload")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--precs", default="", help="derivative precisions to run (f32,bf16x3,...)")
@@ -291,12 +352,19 @@ def main(argv=None) -> int:
     if a.mode == "deriv":
         print(json.dumps(bench_deriv(a, _lib.Device())), flush=True)
         return 0
+    if a.mode == "c4":
+        print(json.dumps(bench_c4_stage(a, _lib.Device())), flush=True)
+        return 0
 
     t_setup = time.perf_counter()
-    m, st, u0 = build_c3(a.n, a.order, a.parts, a.seed)
+    m, st, u0 = build_c3(a.n, a.order, a.parts, a.seed, morton=a.morton)
     log(f"host setup {time.perf_counter() - t_setup:.1f} s, K={m.K}, parts={st.npart}")
     dev = _lib.Device()
     solver = dg.BurgersSolver(dev, st, cfl=0.3)
+    if a.variant >= 0:
+        solver.mesh.set_variant(a.variant)
+    if a.xcd >= 0:
+        solver.mesh.set_order(bool(a.xcd))
     if a.tile or a.bpc:
         solver.mesh.set_tile(a.tile or 64, a.bpc)
     solver.set_state(u0)

@@ -93,6 +93,9 @@ __global__ void geometry_kernel(int64_t K, const double* __restrict__ vtx,
     geo[(13 + 4 * f) * K + k] = fs;
     bgeo[(3 + f) * K + k] = nx + ny + nz;
     bgeo[(7 + f) * K + k] = fs;
+    // LF jump factored as (u- - u+) (A (u- + u+) - B alpha):
+    bgeo[(11 + f) * K + k] = 0.25 * fs * (nx + ny + nz);
+    bgeo[(15 + f) * K + k] = 0.5 * fs * fabs(nx + ny + nz);
   }
 }
 
@@ -507,6 +510,489 @@ __global__ void __launch_bounds__(64 * WPB) burgers_stage_mfma_kernel(StageParam
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined MFMA stage: the same stage as the kernel above, scheduled for
+// memory-level parallelism and laid out so that each lane serves one face.
+//
+//  * LIFT k ordering: k = 4 fp + f, i.e. the 4 lanes of a column group gq
+//    hold face f = gq, and k step s covers face point s of all four faces.
+//    Each lane therefore needs the normal, Fscale, neighbour offset and
+//    orientation code of a single face, fixed for the whole tile: no
+//    per-step selects and no face padding (4 Nfp / 4 = Nfp k steps).
+//  * Every load is unconditional: addresses of lanes past a partial tile are
+//    clamped into the tile, so the loop body has no divergent branches and
+//    the compiler can count outstanding loads exactly. Only the stores of a
+//    partial tile are predicated; full tiles (all but at most one per
+//    partition) run a loop without any predicate.
+//  * Software pipeline per wave: tile records are loaded two tiles ahead;
+//    the next tile's nodal values and neighbour records are issued after the
+//    current tile's gathers and geometry, so they are in flight while the
+//    current tile computes; u0 is issued after the volume product.
+//  * The -1/2 of -div(u^2/2 (1,1,1)) is folded into the stacked derivative
+//    operator, and the Lax-Friedrichs jump is factored as
+//    (u- - u+) (A (u- + u+) - B alpha), A = Fscale n.(1,1,1)/4,
+//    B = Fscale |n.(1,1,1)|/2, precomputed per face at setup.
+// ---------------------------------------------------------------------------
+template <int N>
+struct PipeShape {
+  static constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP;
+  static constexpr int KSL = NFP;                    // k steps of the lift product
+  static constexpr int NPR = (NP + 3) / 4 * 4;       // row offset between Dr, Ds, Dt blocks
+  static constexpr int KSV = (NP + 3) / 4;           // k steps of the volume product
+  static constexpr int MTV = (3 * NPR + 15) / 16;    // 16-row tiles of the stacked operator
+  static constexpr int MTL = (NP + 15) / 16;         // 16-row tiles of LIFT
+  static constexpr int QN = NPR / 4;                 // result rows per lane
+  static constexpr int S = NP + 1;                   // LDS row stride of the u tile
+  static constexpr int E = 16;                       // elements per wave tile
+  static constexpr int PV = (E * NP + 63) / 64;      // tile values per lane
+  static constexpr int NA = (MTV * KSV + MTL * KSL) * 64;  // operator fragments (doubles)
+};
+
+template <int PV>
+struct TileIn {
+  double u[PV];    // nodal values of the tile, value lane + 64 j
+  int32_t nb;      // neighbour value offset across face lane / 16 of element lane % 16
+  int32_t packed;  // the element's four orientation codes
+};
+
+// A tile record as loaded by every lane (vector load, one address).
+__device__ __forceinline__ int4 load_tile_raw(const Tile* tiles, int t) {
+  return *reinterpret_cast<const int4*>(tiles + t);
+}
+
+// The record in scalar registers (it is wave-uniform).
+__device__ __forceinline__ Tile tile_uniform(int4 v) {
+  Tile d;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane(v.x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane(v.y);
+  d.vbase = (int64_t)(((uint64_t)hi << 32) | lo);
+  d.gbase = __builtin_amdgcn_readfirstlane(v.z);
+  d.count = __builtin_amdgcn_readfirstlane(v.w);
+  return d;
+}
+
+template <int N>
+__device__ __forceinline__ void load_tile_in(const StageParams& p, const Tile& d, int lane,
+                                             TileIn<PipeShape<N>::PV>& in) {
+  using SH = PipeShape<N>;
+  const int last = d.count * SH::NP - 1;
+#pragma unroll
+  for (int j = 0; j < SH::PV; ++j) in.u[j] = p.u[d.vbase + min(lane + 64 * j, last)];
+  const int64_t g = (int64_t)d.gbase + min(lane & 15, d.count - 1);
+  in.packed = p.codes[g];
+  in.nb = p.nbr[(lane >> 4) * p.K + g];
+}
+
+// Packed volume-node indices of face f's points (6 bits each, five per
+// word): word w holds points 5w .. 5w+4.
+template <int N>
+struct FacePack {
+  static constexpr int NFP = Tet<N>::Nfp, NW = (NFP + 4) / 5;
+  uint32_t w[4][NW];
+  constexpr FacePack() : w{} {
+    constexpr FmaskTable<N> FM{};
+    for (int f = 0; f < 4; ++f)
+      for (int q = 0; q < NFP; ++q) w[f][q / 5] |= (uint32_t)FM.v[f][q] << (6 * (q % 5));
+  }
+};
+
+// MODE 0: out = a0 u0 + a1 (u + dt L(u)); 1: out = a1 (u + dt L(u)); 2: out = L(u).
+// REGA: the operator fragments are held in registers for the whole sweep
+// (40 doubles per lane at N = 3) instead of being read from LDS per MFMA.
+template <int N, int MODE, bool FULL, bool REGA>
+__device__ __forceinline__ void pipe_tiles(const StageParams& p, const double* sA, double* sU,
+                                           const uint8_t* sFN, int t0, int t1, int ts,
+                                           double dt, double& lmax) {
+  using SH = PipeShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, S = SH::S, NPR = SH::NPR;
+  constexpr int KSV = SH::KSV, MTV = SH::MTV, MTL = SH::MTL, KSL = SH::KSL, QN = SH::QN;
+  constexpr int E = SH::E, PV = SH::PV;
+  constexpr FacePack<N> FPK{};
+  constexpr int NW = FacePack<N>::NW;
+  if (t0 >= t1) return;
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15, gq = lane >> 4;  // element column, face of this lane
+  const int64_t K = p.K;
+  constexpr int NFR = REGA ? MTV * KSV + MTL * KSL : 1;
+  double ra[NFR];
+  if (REGA) {
+#pragma unroll
+    for (int q = 0; q < NFR; ++q) ra[q] = sA[q * 64 + lane];
+  }
+  const double* sAv = sA;
+  const double* sAl = sA + MTV * KSV * 64;
+  const double* su = sU + c * S;
+  // Own volume nodes of face gq, packed (lane-dependent, loop-invariant).
+  uint32_t own[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w)
+    own[w] = gq == 0 ? FPK.w[0][w] : gq == 1 ? FPK.w[1][w] : gq == 2 ? FPK.w[2][w] : FPK.w[3][w];
+  const uint8_t* sfn_face = sFN + gq * 24 * NFP;
+
+  Tile dcur = tile_uniform(load_tile_raw(p.tiles, t0));
+  Tile dnxt = tile_uniform(load_tile_raw(p.tiles, min(t0 + ts, t1 - 1)));
+  TileIn<PV> in;
+  load_tile_in<N>(p, dcur, lane, in);
+  for (int t = t0; t < t1; t += ts) {
+    const int4 d2 = load_tile_raw(p.tiles, min(t + 2 * ts, t1 - 1));
+    const int cnt = FULL ? E : dcur.count;
+    // Stage this tile's values (rows of S doubles; lanes past the tile write
+    // the pad slot of the last row).
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      const int e = i / NP, n = i - e * NP;
+      const bool ok = (E * NP % 64 == 0 || i < E * NP) && (FULL || i < cnt * NP);
+      sU[ok ? e * S + n : E * S - 1] = in.u[j];
+    }
+    // Neighbour trace across face gq: neighbour node of each face point.
+    double uP[KSL];
+    {
+      const int code = (in.packed >> (5 * gq)) & 31;
+      const uint8_t* fn = sfn_face + code * NFP;
+      int idx[KSL];
+#pragma unroll
+      for (int s = 0; s < KSL; ++s) idx[s] = fn[s];
+#pragma unroll
+      for (int s = 0; s < KSL; ++s) uP[s] = p.u[in.nb + idx[s]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // Geometry of element c (clamped into the tile): folded metrics, and the
+    // flux factors of face gq.
+    const int64_t g = (int64_t)dcur.gbase + (FULL ? c : min(c, cnt - 1));
+    const double fa = p.bgeo[(11 + gq) * K + g];
+    const double fb = p.bgeo[(15 + gq) * K + g];
+    const double ga = p.bgeo[0 * K + g], gb = p.bgeo[1 * K + g], gc = p.bgeo[2 * K + g];
+    // Next tile, in flight while this one computes.
+    TileIn<PV> nxt;
+    load_tile_in<N>(p, dnxt, lane, nxt);
+    __builtin_amdgcn_sched_barrier(0);
+    wave_sync();  // sU holds this tile
+    // Volume: [Dr; Ds; Dt] F, folded at once with the affine metrics.
+    double vout[QN];
+    {
+      double4_t accV[MTV];
+#pragma unroll
+      for (int tt = 0; tt < MTV; ++tt) accV[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KSV; ++s) {
+        const int k = 4 * s + gq;
+        const double uk = su[k < NP ? k : NP - 1];
+        const double Fk = k < NP ? uk * uk : 0.0;  // the operator carries -1/2
+#pragma unroll
+        for (int tt = 0; tt < MTV; ++tt)
+          accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              REGA ? ra[(tt * KSV + s) % NFR] : sAv[(tt * KSV + s) * 64 + lane], Fk,
+                                                           accV[tt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const double ur = accV[(4 * q) / 16][((4 * q) % 16) / 4];
+        const double us = accV[(NPR + 4 * q) / 16][((NPR + 4 * q) % 16) / 4];
+        const double ut = accV[(2 * NPR + 4 * q) / 16][((2 * NPR + 4 * q) % 16) / 4];
+        vout[q] = fma(gc, ut, fma(gb, us, ga * ur));
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double u0v[PV];
+    if (MODE == 0) {
+#pragma unroll
+      for (int j = 0; j < PV; ++j)
+        u0v[j] = __builtin_nontemporal_load(p.u0 + dcur.vbase + min(lane + 64 * j, cnt * NP - 1));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // Surface: Lax-Friedrichs jumps at point s of face gq, then LIFT G.
+    double4_t accL[MTL];
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt) accL[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KSL; ++s) {
+      const int nm = (own[s / 5] >> (6 * (s % 5))) & 63;
+      const double uM = su[nm], up = uP[s];
+      const double al = fmax(fabs(uM), fabs(up));
+      // Fscale (n.f(u-) - n.f*) with f(u) = u^2/2 (1,1,1), factored
+      const double gm = (uM - up) * fma(fa, uM + up, -fb * al);
+#pragma unroll
+      for (int tt = 0; tt < MTL; ++tt)
+        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            REGA ? ra[(MTV * KSV + tt * KSL + s) % NFR] : sAl[(tt * KSL + s) * 64 + lane], gm,
+                                                         accL[tt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const double lf = (4 * q) / 16 < MTL ? accL[((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;
+      const double rhs = vout[q] + lf;
+      const int i = gq + 4 * q;
+      vout[q] = MODE == 2 ? rhs : fma(dt, rhs, su[i < NP ? i : NP - 1]);
+    }
+    wave_sync();  // all reads of sU as u are done
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int i = gq + 4 * q;
+      const bool ok = (NPR == NP || i < NP) && (FULL || c < cnt);
+      sU[ok ? c * S + i : E * S - 1] = vout[q];
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      const int e = i / NP, n = i - e * NP;
+      const bool inb = E * NP % 64 == 0 || i < E * NP;
+      double o = sU[inb ? e * S + n : E * S - 1];
+      if (MODE == 0) o = fma(p.a0, u0v[j], p.a1 * o);
+      if (MODE == 1) o = p.a1 * o;
+      if (FULL ? inb : i < cnt * NP) {
+        __builtin_nontemporal_store(o, p.out + dcur.vbase + i);
+        lmax = fmax(lmax, fabs(o));
+      }
+    }
+    dcur = dnxt;
+    dnxt = tile_uniform(d2);
+#pragma unroll
+    for (int j = 0; j < PV; ++j) in.u[j] = nxt.u[j];
+    in.packed = nxt.packed;
+    in.nb = nxt.nb;
+  }
+}
+
+// Deep variant of the pipeline (two waves per SIMD, twice the registers):
+// every load of tile t+1 -- nodal values, the neighbour gathers, geometry and
+// u0 -- is issued at the start of tile t, with the neighbour records of tile
+// t+2 and the tile record of t+3 behind them, so the whole compute of tile t
+// covers their latency. Same arithmetic as pipe_tiles.
+template <int N, int MODE>
+struct DeepIn {
+  double u[PipeShape<N>::PV];
+  double uP[PipeShape<N>::KSL];
+  double u0[MODE == 0 ? PipeShape<N>::PV : 1];
+  double ga, gb, gc, fa, fb;
+};
+
+template <int N, int MODE>
+__device__ __forceinline__ void deep_issue(const StageParams& p, const Tile& d, int lane,
+                                           int32_t nb, int32_t packed, const uint8_t* sfn_face,
+                                           DeepIn<N, MODE>& x) {
+  using SH = PipeShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, KSL = SH::KSL, PV = SH::PV;
+  const int c = lane & 15, gq = lane >> 4;
+  const int last = d.count * NP - 1;
+#pragma unroll
+  for (int j = 0; j < PV; ++j) x.u[j] = p.u[d.vbase + min(lane + 64 * j, last)];
+  const uint8_t* fn = sfn_face + ((packed >> (5 * gq)) & 31) * NFP;
+  int idx[KSL];
+#pragma unroll
+  for (int s = 0; s < KSL; ++s) idx[s] = fn[s];
+#pragma unroll
+  for (int s = 0; s < KSL; ++s) x.uP[s] = p.u[nb + idx[s]];
+  const int64_t K = p.K;
+  const int64_t g = (int64_t)d.gbase + min(c, d.count - 1);
+  x.fa = p.bgeo[(11 + gq) * K + g];
+  x.fb = p.bgeo[(15 + gq) * K + g];
+  x.ga = p.bgeo[0 * K + g];
+  x.gb = p.bgeo[1 * K + g];
+  x.gc = p.bgeo[2 * K + g];
+  if (MODE == 0) {
+#pragma unroll
+    for (int j = 0; j < PV; ++j)
+      x.u0[j] = __builtin_nontemporal_load(p.u0 + d.vbase + min(lane + 64 * j, last));
+  }
+}
+
+__device__ __forceinline__ void deep_meta(const StageParams& p, const Tile& d, int lane,
+                                          int32_t& nb, int32_t& packed) {
+  const int64_t g = (int64_t)d.gbase + min(lane & 15, d.count - 1);
+  packed = p.codes[g];
+  nb = p.nbr[(lane >> 4) * p.K + g];
+}
+
+template <int N, int MODE, bool FULL>
+__device__ __forceinline__ void deep_tiles(const StageParams& p, const double* sA, double* sU,
+                                           const uint8_t* sFN, int t0, int t1, int ts,
+                                           double dt, double& lmax) {
+  using SH = PipeShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, S = SH::S, NPR = SH::NPR;
+  constexpr int KSV = SH::KSV, MTV = SH::MTV, MTL = SH::MTL, KSL = SH::KSL, QN = SH::QN;
+  constexpr int E = SH::E, PV = SH::PV;
+  constexpr FacePack<N> FPK{};
+  constexpr int NW = FacePack<N>::NW;
+  if (t0 >= t1) return;
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15, gq = lane >> 4;
+  const double* sAv = sA;
+  const double* sAl = sA + MTV * KSV * 64;
+  const double* su = sU + c * S;
+  uint32_t own[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w)
+    own[w] = gq == 0 ? FPK.w[0][w] : gq == 1 ? FPK.w[1][w] : gq == 2 ? FPK.w[2][w] : FPK.w[3][w];
+  const uint8_t* sfn_face = sFN + gq * 24 * NFP;
+
+  Tile dcur = tile_uniform(load_tile_raw(p.tiles, t0));
+  Tile dn1 = tile_uniform(load_tile_raw(p.tiles, min(t0 + ts, t1 - 1)));
+  Tile dn2 = tile_uniform(load_tile_raw(p.tiles, min(t0 + 2 * ts, t1 - 1)));
+  int32_t nb0, pk0, nb1, pk1;
+  deep_meta(p, dcur, lane, nb0, pk0);
+  DeepIn<N, MODE> cur;
+  deep_issue<N, MODE>(p, dcur, lane, nb0, pk0, sfn_face, cur);
+  deep_meta(p, dn1, lane, nb1, pk1);
+  for (int t = t0; t < t1; t += ts) {
+    const int4 d3 = load_tile_raw(p.tiles, min(t + 3 * ts, t1 - 1));
+    const int cnt = FULL ? E : dcur.count;
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      const int e = i / NP, n = i - e * NP;
+      const bool ok = (E * NP % 64 == 0 || i < E * NP) && (FULL || i < cnt * NP);
+      sU[ok ? e * S + n : E * S - 1] = cur.u[j];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    DeepIn<N, MODE> nx;
+    deep_issue<N, MODE>(p, dn1, lane, nb1, pk1, sfn_face, nx);
+    int32_t nb2, pk2;
+    deep_meta(p, dn2, lane, nb2, pk2);
+    __builtin_amdgcn_sched_barrier(0);
+    wave_sync();  // sU holds this tile
+    double vout[QN];
+    {
+      double4_t accV[MTV];
+#pragma unroll
+      for (int tt = 0; tt < MTV; ++tt) accV[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KSV; ++s) {
+        const int k = 4 * s + gq;
+        const double uk = su[k < NP ? k : NP - 1];
+        const double Fk = k < NP ? uk * uk : 0.0;  // the operator carries -1/2
+#pragma unroll
+        for (int tt = 0; tt < MTV; ++tt)
+          accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAv[(tt * KSV + s) * 64 + lane], Fk,
+                                                           accV[tt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const double ur = accV[(4 * q) / 16][((4 * q) % 16) / 4];
+        const double us = accV[(NPR + 4 * q) / 16][((NPR + 4 * q) % 16) / 4];
+        const double ut = accV[(2 * NPR + 4 * q) / 16][((2 * NPR + 4 * q) % 16) / 4];
+        vout[q] = fma(cur.gc, ut, fma(cur.gb, us, cur.ga * ur));
+      }
+    }
+    double4_t accL[MTL];
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt) accL[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KSL; ++s) {
+      const int nm = (own[s / 5] >> (6 * (s % 5))) & 63;
+      const double uM = su[nm], up = cur.uP[s];
+      const double al = fmax(fabs(uM), fabs(up));
+      const double gm = (uM - up) * fma(cur.fa, uM + up, -cur.fb * al);
+#pragma unroll
+      for (int tt = 0; tt < MTL; ++tt)
+        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,
+                                                         accL[tt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const double lf = (4 * q) / 16 < MTL ? accL[((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;
+      const double rhs = vout[q] + lf;
+      const int i = gq + 4 * q;
+      vout[q] = MODE == 2 ? rhs : fma(dt, rhs, su[i < NP ? i : NP - 1]);
+    }
+    wave_sync();  // all reads of sU as u are done
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int i = gq + 4 * q;
+      const bool ok = (NPR == NP || i < NP) && (FULL || c < cnt);
+      sU[ok ? c * S + i : E * S - 1] = vout[q];
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      const int e = i / NP, n = i - e * NP;
+      const bool inb = E * NP % 64 == 0 || i < E * NP;
+      double o = sU[inb ? e * S + n : E * S - 1];
+      if (MODE == 0) o = fma(p.a0, cur.u0[j], p.a1 * o);
+      if (MODE == 1) o = p.a1 * o;
+      if (FULL ? inb : i < cnt * NP) {
+        __builtin_nontemporal_store(o, p.out + dcur.vbase + i);
+        lmax = fmax(lmax, fabs(o));
+      }
+    }
+    cur = nx;
+    nb1 = nb2;
+    pk1 = pk2;
+    dcur = dn1;
+    dn1 = dn2;
+    dn2 = tile_uniform(d3);
+  }
+}
+
+// Tiles [0, nfull) hold 16 elements each; [nfull, ntiles) are the partial
+// last tiles of partitions.
+// KIND 0: pipe_tiles (3 waves per SIMD); 1: deep_tiles (2 waves per SIMD);
+// 2: pipe_tiles with register-resident operator fragments (2 waves per SIMD).
+template <int N, int WPB, int MODE, int KIND>
+__global__ void __launch_bounds__(64 * WPB, KIND == 0 ? 3 : 2)
+    burgers_stage_pipe_kernel(StageParams p, int nfull) {
+  using SH = PipeShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, NPR = SH::NPR;
+  constexpr int KSV = SH::KSV, MTV = SH::MTV, MTL = SH::MTL, KSL = SH::KSL;
+  constexpr int NT = 64 * WPB;
+  __shared__ __attribute__((aligned(16))) double sA[SH::NA];
+  __shared__ __attribute__((aligned(16))) double sUall[WPB][SH::E * SH::S];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) sFN[i] = p.face_nodes[i];
+  // Operator fragments, entry (tile, kstep, lane) in lane order. Volume: the
+  // stacked -1/2 [Dr; Ds; Dt]. LIFT: column k = 4 fp + f of k step fp is
+  // face f, point fp.
+  for (int q = threadIdx.x; q < SH::NA; q += NT) {
+    const int l = q & 63, ts_ = q >> 6, cl = l & 15, gl = l >> 4;
+    double v = 0.0;
+    if (ts_ < MTV * KSV) {
+      const int tt = ts_ / KSV, s = ts_ % KSV;
+      const int row = 16 * tt + cl, k = 4 * s + gl, blk = row / NPR, i = row % NPR;
+      if (k < NP && blk < 3 && i < NP)
+        v = -0.5 * (blk == 0 ? p.Dr : blk == 1 ? p.Ds : p.Dt)[i * NP + k];  // exact scaling
+    } else {
+      const int r2 = ts_ - MTV * KSV, tt = r2 / KSL, s = r2 % KSL;
+      const int row = 16 * tt + cl;
+      if (row < NP) v = p.LIFT[row * NF + gl * NFP + s];
+    }
+    sA[q] = v;
+  }
+  __syncthreads();
+  const double dt = MODE == 2 ? 0.0 : *p.dt;
+  double lmax = 0.0;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int NW = gridDim.x * WPB;
+  const int gw = blockIdx.x * WPB + w;
+  double* sU = sUall[w];
+  int f0 = gw, f1 = nfull, fs = NW;
+  if (p.chunked && gridDim.x % 8 == 0) {
+    // Blocks are dealt round robin over the 8 XCDs (block b on XCD b % 8).
+    // Each XCD takes a contiguous eighth of the full tiles and its waves
+    // sweep it together, so tiles processed at the same time on one XCD are
+    // neighbours in the tile list and their face neighbours (a few tiles to
+    // a few dozen tiles away) are read from that XCD's L2.
+    const int x = blockIdx.x % 8;
+    const int lw = (blockIdx.x / 8) * WPB + w;
+    f0 = (int)((int64_t)nfull * x / 8) + lw;
+    f1 = (int)((int64_t)nfull * (x + 1) / 8);
+    fs = NW / 8;
+  }
+  if (KIND == 1) {
+    deep_tiles<N, MODE, true>(p, sA, sU, sFN, f0, f1, fs, dt, lmax);
+    deep_tiles<N, MODE, false>(p, sA, sU, sFN, nfull + gw, p.ntiles, NW, dt, lmax);
+  } else {
+    pipe_tiles<N, MODE, true, KIND == 2>(p, sA, sU, sFN, f0, f1, fs, dt, lmax);
+    pipe_tiles<N, MODE, false, false>(p, sA, sU, sFN, nfull + gw, p.ntiles, NW, dt, lmax);
+  }
+  if (MODE != 2 && p.maxabs) {
+    lmax = wave_max(lmax);
+    if ((threadIdx.x & 63) == 0 && lmax > 0.0)
+      atomicMax(p.maxabs, (unsigned long long)__double_as_longlong(lmax));
+  }
+}
+
 __global__ void max_abs_kernel(const Tile* __restrict__ tiles, int ntiles, int np,
                                const double* __restrict__ u, unsigned long long* maxabs) {
   double m = 0.0;
@@ -562,8 +1048,10 @@ __global__ void min_node_distance_kernel(int64_t K, const double* __restrict__ v
 
 namespace {
 
-int build_tile_list(dgk_dgmesh m, int per_tile, Tile** out, int* nout) {
-  std::vector<Tile> tl;
+// Tiles of per_tile elements over all partitions, full tiles first (in
+// partition order), then the partial last tiles of partitions.
+int build_tile_list(dgk_dgmesh m, int per_tile, Tile** out, int* nout, int* nfull) {
+  std::vector<Tile> full, part;
   int64_t g = 0;
   for (int p = 0; p < m->npart; ++p) {
     for (int64_t f = 0; f < m->Kp[p]; f += per_tile) {
@@ -571,20 +1059,22 @@ int build_tile_list(dgk_dgmesh m, int per_tile, Tile** out, int* nout) {
       t.vbase = m->state_off[p] + f * m->Np;
       t.gbase = (int32_t)(g + f);
       t.count = (int32_t)std::min<int64_t>(per_tile, m->Kp[p] - f);
-      tl.push_back(t);
+      (t.count == per_tile ? full : part).push_back(t);
     }
     g += m->Kp[p];
   }
+  if (nfull) *nfull = (int)full.size();
+  full.insert(full.end(), part.begin(), part.end());
   if (*out) dgk_free(m->dev, *out);
   *out = nullptr;
-  *nout = (int)tl.size();
-  if (tl.empty()) return DGK_OK;
-  return dgk_malloc(m->dev, tl.size() * sizeof(Tile), tl.data(), (void**)out);
+  *nout = (int)full.size();
+  if (full.empty()) return DGK_OK;
+  return dgk_malloc(m->dev, full.size() * sizeof(Tile), full.data(), (void**)out);
 }
 
 int build_tiles(dgk_dgmesh m) {
-  int rc = build_tile_list(m, m->tb, &m->tiles, &m->ntiles);
-  if (rc == DGK_OK) rc = build_tile_list(m, 16, &m->tiles64, &m->ntiles64);
+  int rc = build_tile_list(m, m->tb, &m->tiles, &m->ntiles, nullptr);
+  if (rc == DGK_OK) rc = build_tile_list(m, 16, &m->tiles64, &m->ntiles64, &m->nfull64);
   return rc;
 }
 
@@ -636,6 +1126,36 @@ int launch_stage_mfma(dgk_dgmesh m, const StageParams& p) {
   }
 }
 
+template <int N, int WPB, int MODE, int KIND>
+int launch_stage_pipe_wm(dgk_dgmesh m, const StageParams& p) {
+  int grid = (p.ntiles + WPB - 1) / WPB;
+  const int waves_per_simd = KIND == 0 ? 3 : 2;
+  int bpc = m->blocks_per_cu > 0 ? m->blocks_per_cu : (4 * waves_per_simd) / WPB;
+  int cap = m->dev->cu_count * bpc;
+  if (grid > cap) grid = cap;
+  if (grid < 1) return DGK_OK;
+  if (p.chunked && grid >= 8) grid -= grid % 8;
+  hipLaunchKernelGGL((burgers_stage_pipe_kernel<N, WPB, MODE, KIND>), dim3(grid), dim3(64 * WPB),
+                     0, m->dev->stream, p, m->nfull64);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+template <int N, int WPB, int KIND>
+int launch_stage_pipe_w(dgk_dgmesh m, const StageParams& p) {
+  if (p.mode == 1) return launch_stage_pipe_wm<N, WPB, 2, KIND>(m, p);
+  if (p.a0 != 0.0) return launch_stage_pipe_wm<N, WPB, 0, KIND>(m, p);
+  return launch_stage_pipe_wm<N, WPB, 1, KIND>(m, p);
+}
+
+template <int N, int KIND>
+int launch_stage_pipe(dgk_dgmesh m, const StageParams& p) {
+  switch (m->tbm) {
+    case 32: return launch_stage_pipe_w<N, 2, KIND>(m, p);
+    default: return launch_stage_pipe_w<N, 4, KIND>(m, p);
+  }
+}
+
 bool use_mfma(dgk_dgmesh m, int mode) {
   if (mode == 2 || m->N > 4) return false;
   return m->variant != 1 && m->variant != 3;
@@ -660,11 +1180,33 @@ int launch_stage(dgk_dgmesh m, StageParams p) {
   if (use_mfma(m, p.mode)) {
     p.tiles = m->tiles64;
     p.ntiles = m->ntiles64;
+    if (m->variant == 4) {
+      switch (m->N) {
+        case 1: return launch_stage_mfma<1>(m, p);
+        case 2: return launch_stage_mfma<2>(m, p);
+        case 3: return launch_stage_mfma<3>(m, p);
+        case 4: return launch_stage_mfma<4>(m, p);
+      }
+    }
+    if (m->variant == 5) {
+      switch (m->N) {
+        case 1: return launch_stage_pipe<1, 1>(m, p);
+        case 2: return launch_stage_pipe<2, 1>(m, p);
+        case 3: return launch_stage_pipe<3, 1>(m, p);
+      }
+    }
+    if (m->variant == 6) {
+      switch (m->N) {
+        case 1: return launch_stage_pipe<1, 2>(m, p);
+        case 2: return launch_stage_pipe<2, 2>(m, p);
+        case 3: return launch_stage_pipe<3, 2>(m, p);
+      }
+    }
     switch (m->N) {
-      case 1: return launch_stage_mfma<1>(m, p);
-      case 2: return launch_stage_mfma<2>(m, p);
-      case 3: return launch_stage_mfma<3>(m, p);
-      case 4: return launch_stage_mfma<4>(m, p);
+      case 1: return launch_stage_pipe<1, 0>(m, p);
+      case 2: return launch_stage_pipe<2, 0>(m, p);
+      case 3: return launch_stage_pipe<3, 0>(m, p);
+      case 4: return launch_stage_pipe<4, 0>(m, p);
     }
   }
   switch (m->N) {
@@ -910,8 +1452,11 @@ extern "C" int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked) {
 extern "C" int dgk_burgers_set_variant(dgk_dgmesh m, int variant) {
   dgk::clear_error();
   DGK_REQUIRE(m, "dgk_burgers_set_variant: NULL mesh");
-  DGK_REQUIRE(variant >= 0 && variant <= 3, "dgk_burgers_set_variant: variant %d", variant);
-  DGK_REQUIRE(variant != 2 || m->N <= 4, "dgk_burgers_set_variant: MFMA stage built for N <= 4");
+  DGK_REQUIRE(variant >= 0 && variant <= 6, "dgk_burgers_set_variant: variant %d", variant);
+  DGK_REQUIRE((variant != 2 && variant != 4) || m->N <= 4,
+              "dgk_burgers_set_variant: MFMA stage built for N <= 4");
+  DGK_REQUIRE((variant != 5 && variant != 6) || m->N <= 3,
+              "dgk_burgers_set_variant: variants 5, 6 of the MFMA stage are built for N <= 3");
   m->variant = variant;
   return use_split(m, 0) ? dgk::split_prepare(m) : DGK_OK;
 }

@@ -44,7 +44,9 @@ struct Tile {
 };
 
 constexpr int kGeoRows = 26;   // rx..tz, J, 4 x (nx, ny, nz, Fscale)
-constexpr int kBGeoRows = 11;  // rx+ry+rz, sx+sy+sz, tx+ty+tz, 4 x n.(1,1,1), 4 x Fscale
+// rx+ry+rz, sx+sy+sz, tx+ty+tz, 4 x n.(1,1,1), 4 x Fscale,
+// 4 x Fscale n.(1,1,1)/4, 4 x Fscale |n.(1,1,1)|/2
+constexpr int kBGeoRows = 19;
 
 }  // namespace dgk
 
@@ -61,6 +63,7 @@ struct dgk_dgmesh_s {
   int ntiles = 0;
   dgk::Tile* tiles64 = nullptr;    // tiles of 16 elements (MFMA kernel, one per wave step)
   int ntiles64 = 0;
+  int nfull64 = 0;                 // tiles64[0, nfull64) hold 16 elements each
   int tbm = 64;               // MFMA block: 16 elements per wave, 1/2/4 waves per block
   int tb = 64, blocks_per_cu = 0, chunked = 0;
   int variant = 0;            // 0 auto, 1 VALU, 2 MFMA

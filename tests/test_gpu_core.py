@@ -236,11 +236,14 @@ def test_geometry_and_hmin_vs_oracle(device):
     assert abs(mesh.min_node_distance() - o.min_node_distance()) <= 1e-15
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6])
 @pytest.mark.parametrize("N,n,parts", [(2, 8, [768] * 4), (3, 5, [200, 350, 200]), (1, 6, [1296]),
                                        (4, 3, [100, 62]), (3, 4, [1, 63, 64, 65, 191])])
 def test_burgers_rhs_vs_oracle(device, N, n, parts, variant):
-    """VALU (1) and FP64-MFMA (2) stage kernels against the oracle's L(u)."""
+    """Stage kernels against the oracle's L(u): VALU (1), pipelined FP64 MFMA
+    (2), FP64 MFMA without the load pipeline (4), deep pipeline (5, N <= 3)."""
+    if variant in (5, 6) and N > 3:
+        pytest.skip("the deep-pipeline and register-operator kernels are built for N <= 3")
     m = M.cube_mesh(n, n, n, periodic=True)
     st = dg.make_setup(m, N, Kp=parts)
     u = smooth_u0(st, noise=1e-3)
@@ -252,7 +255,7 @@ def test_burgers_rhs_vs_oracle(device, N, n, parts, variant):
     assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 5, 6])
 def test_config1_burgers_100_steps_vs_oracle(device, variant):
     """Config 1: 8^3 hexes x 6, P=2, 4 slabs, LF + SSP-RK3, CFL 0.3, 100 steps."""
     m = M.cube_mesh(8, 8, 8, periodic=True)
@@ -289,7 +292,10 @@ def test_graph_replay_matches_eager(device):
 
 @pytest.mark.parametrize("variant,tb,bpc,chunk", [(1, 64, 0, 1), (1, 128, 0, 1), (1, 256, 0, 0),
                                                   (1, 64, 3, 1), (2, 64, 1, 1), (2, 64, 0, 0),
-                                                  (2, 16, 5, 1), (2, 32, 0, 0), (2, 16, 0, 0)])
+                                                  (2, 16, 5, 1), (2, 32, 0, 0), (2, 16, 0, 0),
+                                                  (5, 64, 1, 1), (5, 64, 0, 0), (5, 32, 0, 1),
+                                                  (5, 32, 3, 0), (4, 64, 0, 0), (4, 16, 2, 1),
+                                                  (6, 64, 0, 1), (6, 32, 0, 0), (6, 64, 1, 0)])
 def test_tile_shapes_agree(device, variant, tb, bpc, chunk):
     """Tile shape and grid size do not change the result of a kernel variant."""
     m = M.cube_mesh(6, 5, 4, periodic=True)
@@ -313,13 +319,14 @@ def test_mfma_and_valu_stage_agree(device):
     st = dg.make_setup(m, 3, Kp=M.slab_partition(m.K, 5, 0.4, seed=3))
     u0 = smooth_u0(st)
     out = []
-    for v in (1, 2):
+    for v in (1, 2, 4, 5, 6):
         s = dg.BurgersSolver(device, st)
         s.mesh.set_variant(v)
         s.set_state(u0)
         s.step(10)
         out.append(s.state())
-    assert rel_l2(out[1], out[0]) <= 1e-14
+    for o in out[1:]:
+        assert rel_l2(o, out[0]) <= 1e-14
 
 
 def test_face_exchange_c2_size_four_slabs_bit_exact(device):
