@@ -20,6 +20,7 @@ _i32, _i64, _f64, _sz = C.c_int, C.c_int64, C.c_double, C.c_size_t
 _pi64 = C.POINTER(C.c_int64)
 _pi32 = C.POINTER(C.c_int32)
 _pf64 = C.POINTER(C.c_double)
+_pf32 = C.POINTER(C.c_float)
 
 DGK_OK = 0
 DGK_F64, DGK_F32 = 0, 1
@@ -46,6 +47,11 @@ class _ArgVal(C.Union):
 
 class Arg(C.Structure):
     _fields_ = [("kind", C.c_int), ("pad", C.c_int), ("v", _ArgVal)]
+
+
+class AVParams(C.Structure):
+    _fields_ = [("s0", C.c_double), ("kappa", C.c_double), ("ceps", C.c_double),
+                ("tau0", C.c_double)]
 
 
 class DGMeshDesc(C.Structure):
@@ -110,6 +116,23 @@ _SIGS = {
     "dgk_burgers_set_precision": (_i32, [_vp, _i32]),
     "dgk_burgers_set_order": (_i32, [_vp, _i32]),
     "dgk_kernel_set_matrix": (_i32, [_vp, C.c_char_p, _i32, _i32, _pf64]),
+    "dgk_av_create": (_i32, [_vp, _pf64, C.POINTER(AVParams), C.POINTER(_vp)]),
+    "dgk_av_destroy": (_i32, [_vp]),
+    "dgk_av_eps_dev": (_i32, [_vp, C.POINTER(_vp)]),
+    "dgk_av_sensor_pp": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "dgk_av_sensor_mlp": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "dgk_av_rhs": (_i32, [_vp, _vp, _vp]),
+    "dgk_av_rhs_total": (_i32, [_vp, _vp, _vp]),
+    "dgk_av_stage": (_i32, [_vp, _vp, _vp, _vp, _f64, _f64, _vp, _vp]),
+    "dgk_av_cfl_dt": (_i32, [_vp, _vp, _f64, _f64, _f64, _f64]),
+    "dgk_mlp_create": (_i32, [_vp, _i32, _pf32, C.POINTER(_vp)]),
+    "dgk_mlp_destroy": (_i32, [_vp]),
+    "dgk_mlp_num_params": (_i32, [_vp, C.POINTER(_i32)]),
+    "dgk_mlp_get": (_i32, [_vp, _pf32]),
+    "dgk_mlp_set": (_i32, [_vp, _pf32]),
+    "dgk_mlp_forward": (_i32, [_vp, _vp, _i64, _vp]),
+    "dgk_mlp_grad": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "dgk_mlp_train_step": (_i32, [_vp, _vp, _vp, _i64, _i32, C.c_uint64, C.c_float, _vp]),
 }
 
 _lib = None

@@ -1255,7 +1255,7 @@ extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_d
     return DGK_ERR_UNSUPPORTED;
   }
   // Neighbour value offsets and packed codes, SoA.
-  std::vector<int32_t> nbr((size_t)4 * K), codes((size_t)K);
+  std::vector<int32_t> nbr((size_t)4 * K), codes((size_t)K), nbe((size_t)4 * K);
   int p2 = 0;
   for (int64_t g = 0; g < K; ++g) {
     int packed = 0;
@@ -1272,6 +1272,7 @@ extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_d
         p2 = (int)(std::upper_bound(gstart.begin(), gstart.end(), g2) - gstart.begin()) - 1;
       }
       nbr[(size_t)f * K + g] = (int32_t)(m->state_off[p2] + (g2 - gstart[p2]) * Np);
+      nbe[(size_t)f * K + g] = (int32_t)g2;
       packed |= c << (5 * f);
     }
     codes[g] = packed;
@@ -1307,6 +1308,7 @@ extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_d
   rc |= dgk_malloc(dev, (size_t)K * kBGeoRows * 8 + 8, nullptr, (void**)&m->bgeo);
   rc |= dgk_malloc(dev, nbr.size() * 4 + 4, nbr.data(), (void**)&m->nbr);
   rc |= dgk_malloc(dev, codes.size() * 4 + 4, codes.data(), (void**)&m->codes);
+  rc |= dgk_malloc(dev, nbe.size() * 4 + 4, nbe.data(), (void**)&m->nbe);
   rc |= dgk_malloc(dev, 64, nullptr, (void**)&m->scratch);
   if (rc == DGK_OK) rc = build_tiles(m);
   if (rc != DGK_OK) { dgk_dgmesh_destroy(m); return DGK_ERR_HIP; }
@@ -1342,7 +1344,7 @@ extern "C" int dgk_dgmesh_destroy(dgk_dgmesh m) {
   dgk_device dev = m->dev;
   dgk::split_release(m);
   void* ptrs[] = {m->Dr, m->Ds, m->Dt, m->LIFT, m->rst, m->face_nodes, m->vtx, m->geo,
-                  m->bgeo, m->nbr, m->codes, m->tiles, m->tiles64, m->scratch};
+                  m->bgeo, m->nbr, m->codes, m->nbe, m->tiles, m->tiles64, m->scratch};
   for (void* q : ptrs) dgk_free(dev, q);
   delete m;
   return DGK_OK;

@@ -59,6 +59,7 @@ struct dgk_dgmesh_s {
   uint8_t* face_nodes = nullptr;
   double *vtx = nullptr, *geo = nullptr, *bgeo = nullptr;
   int32_t *nbr = nullptr, *codes = nullptr;
+  int32_t* nbe = nullptr;          // [4][K] dense global element index of each face's neighbour
   dgk::Tile* tiles = nullptr;      // tiles of tb elements (VALU kernel)
   int ntiles = 0;
   dgk::Tile* tiles64 = nullptr;    // tiles of 16 elements (MFMA kernel, one per wave step)
@@ -86,6 +87,11 @@ namespace dgk {
 // mode 1: out = L(u).
 int split_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0, double a1,
                 const double* dt, unsigned long long* maxabs, int mode);
+// The two halves of split_stage: the volume term into m->rhs, then flux,
+// LIFT and the update (rhs may be modified in between, e.g. by viscosity).
+int split_volume(dgk_dgmesh m, const double* u);
+int split_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
+                  double a1, const double* dt, unsigned long long* maxabs, int mode);
 // Builds the split stage's plan, FP64 volume buffer and LIFT fragments for
 // (N, prec) if not there yet (allocates: call outside graph capture).
 int split_prepare(dgk_dgmesh m);

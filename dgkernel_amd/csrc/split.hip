@@ -328,12 +328,21 @@ void dgk::split_release(dgk_dgmesh m) {
   m->lift_frag = nullptr;
 }
 
-int dgk::split_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
-                     double a1, const double* dt, unsigned long long* maxabs, int mode) {
+int dgk::split_volume(dgk_dgmesh m, const double* u) {
   int rc = split_prepare(m);
   if (rc != DGK_OK) return rc;
-  rc = dgk::deriv_plan_divergence(m->plan, m->tiles64, m->ntiles64, u, m->geo, m->K, m->rhs);
+  return dgk::deriv_plan_divergence(m->plan, m->tiles64, m->ntiles64, u, m->geo, m->K, m->rhs);
+}
+
+int dgk::split_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
+                     double a1, const double* dt, unsigned long long* maxabs, int mode) {
+  int rc = split_volume(m, u);
   if (rc != DGK_OK) return rc;
+  return split_surface(m, u, u0, out, a0, a1, dt, maxabs, mode);
+}
+
+int dgk::split_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
+                       double a1, const double* dt, unsigned long long* maxabs, int mode) {
   SurfArgs a;
   a.tiles = m->tiles64;
   a.ntiles = m->ntiles64;

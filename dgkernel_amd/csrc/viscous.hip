@@ -1,0 +1,1053 @@
+// Shock capturing for the tetrahedral Burgers stage on gfx950: the
+// Persson-Peraire modal-decay sensor, artificial viscosity div(eps grad u) in
+// BR1 form, and a learned sensor (MLP over element modal coefficients) with
+// its training kernels.
+//
+// None of this exists in the reference (it has no shock capturing); the
+// learned-viscosity workload defines it. Written from:
+//  * Persson & Peraire, AIAA 2006-112: S = sum_{deg m = N} uh_m^2 /
+//    sum_m uh_m^2 over the orthonormal modal coefficients uh = V^-1 u,
+//    s = log10 S, eps = eps0 * ramp(s; s0, kappa) with the sine ramp, and
+//    eps0 = ceps h lambda / N, h = (6 vol)^(1/3), lambda = sqrt(3) max|u|
+//    (the largest |f'(u).n| for f(u) = u^2/2 (1,1,1));
+//  * BR1 (Bassi & Rebay 1997; Hesthaven & Warburton 2008 §7.1-7.2), strong
+//    form with central fluxes and a jump penalty:
+//      q   = grad u + LIFT(Fscale n (u+ - u-)/2),
+//      L_v = div(eps q) - LIFT(Fscale (n.(s- - s+)/2 + tau (u- - u+))),
+//      s = eps q, tau = tau0 max(eps-, eps+).
+// The viscous stage runs on the split path: volume term of the Burgers
+// operator into the mesh's FP64 rhs buffer, then the gradient kernel (q),
+// the viscous kernel (rhs += L_v), then the surface kernel (LF flux, LIFT,
+// SSP-RK3 update).
+//
+// Kernel shapes. The sensor, gradient and viscous kernels take one element
+// per lane, 64 elements (four 16-element tiles) per single-wave block; the
+// tile's nodal values are staged through LDS with unit-stride loads, and the
+// element operators are read with wave-uniform addresses (scalar loads,
+// SGPR operands of v_fma_f64). The learned sensor works on 16-element tiles
+// per wave on matrix cores: modal transform on FP32 MFMA (16x16x4), both
+// hidden layers on BF16 MFMA (16x16x32) with FP32 accumulation. Hidden
+// activations never leave registers between layers: the k order of each
+// layer's product is permuted so that the FP32 result layout of one layer
+// (lane (c, g) holds rows 4g + r of each 16-row tile) is directly the BF16
+// B operand of the next (lane (c, g) holds k = 8g + j of a 32-wide k step).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dgmesh_internal.h"
+
+namespace {
+
+using dgk::FmaskTable;
+using dgk::Tet;
+using dgk::Tile;
+
+typedef const __attribute__((address_space(4))) double* cdouble_ptr;
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kH = 64;  // hidden width of the sensor network
+
+// Modal degree of each column of Vandermonde3D (i, j, k nested, degree
+// i + j + k; element/library/gonudg/vandermonde3d.go:15-41).
+template <int N>
+struct ModeDeg {
+  int d[Tet<N>::Np];
+  constexpr ModeDeg() : d{} {
+    int sk = 0;
+    for (int i = 0; i <= N; ++i)
+      for (int j = 0; j <= N - i; ++j)
+        for (int k = 0; k <= N - i - j; ++k) d[sk++] = i + j + k;
+  }
+};
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wmax(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ double pp_fraction(double S, double s0, double kappa) {
+  if (!(S > 0.0)) return 0.0;
+  const double sv = log10(S);
+  if (sv < s0 - kappa) return 0.0;
+  if (sv > s0 + kappa) return 1.0;
+  return 0.5 * (1.0 + sin(M_PI * (sv - s0) / (2.0 * kappa)));
+}
+
+struct AvArgs {
+  const Tile* tiles;  // 16-element tiles, full tiles first
+  int ntiles;
+  int64_t K;
+  const double* u;
+  const double* geo;          // [26][K]
+  const double* Dr;
+  const double* Ds;
+  const double* Dt;
+  const double* LIFT;
+  const double* Vinv;         // Np x Np row-major (modal transform)
+  const uint8_t* face_nodes;  // [4][24][Nfp]
+  const int32_t* nbr;         // [4][K] neighbour value offsets
+  const int32_t* nbe;         // [4][K] neighbour element indices
+  const int32_t* codes;       // [K]
+  const double* q[3];         // gradient arrays (state layout)
+  double* qo[3];
+  double* eps;                // [K]
+  const double* eps_in;
+  double* rhs;                // state layout, += L_v
+  double* frac;               // [K] or NULL
+  float* frac32;              // [K] or NULL
+  float* feats;               // [K][Np] or NULL
+  double* maxeps;             // atomic max of eps, or NULL
+  double s0, kappa, ceps, tau0;
+};
+
+// The wave's four tiles: lane l belongs to tile 4 grp + l / 16, element
+// l % 16 of it (count 0 past the list).
+__device__ __forceinline__ Tile group_tile(const Tile* tiles, int ntiles, int grp, int lane) {
+  Tile t;
+  const int ti = 4 * grp + (lane >> 4);
+  if (ti < ntiles) {
+    t = tiles[ti];
+  } else {
+    t.vbase = 0;
+    t.gbase = 0;
+    t.count = 0;
+  }
+  return t;
+}
+
+// Stage the nodal values of the wave's four tiles of src into LDS rows of S
+// doubles (row 16 q + e for element e of tile q), unit-stride loads.
+template <int NP, int S>
+__device__ __forceinline__ void stage4(const double* __restrict__ src, const Tile& my, int lane,
+                                       double* sRow) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const long long vb = __shfl((long long)my.vbase, 16 * q, 64);
+    const int cnt = __shfl(my.count, 16 * q, 64);
+    for (int i = lane; i < cnt * NP; i += 64) {
+      const int e = i / NP, n = i - e * NP;
+      sRow[(16 * q + e) * S + n] = src[vb + i];
+    }
+  }
+}
+
+// dst[tile values] (=|+=) LDS rows, unit-stride stores.
+template <int NP, int S, bool ADD>
+__device__ __forceinline__ void unstage4(double* __restrict__ dst, const Tile& my, int lane,
+                                         const double* sRow) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const long long vb = __shfl((long long)my.vbase, 16 * q, 64);
+    const int cnt = __shfl(my.count, 16 * q, 64);
+    for (int i = lane; i < cnt * NP; i += 64) {
+      const int e = i / NP, n = i - e * NP;
+      const double v = sRow[(16 * q + e) * S + n];
+      if (ADD) dst[vb + i] += v;
+      else dst[vb + i] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persson-Peraire sensor, FP64, one element per lane.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(64) pp_sensor_kernel(AvArgs a) {
+  constexpr int NP = Tet<N>::Np, S = NP + 1;
+  constexpr ModeDeg<N> MD{};
+  __shared__ double sU[64 * S];
+  const int lane = threadIdx.x;
+  double emax = 0.0;
+  for (int grp = blockIdx.x; 4 * grp < a.ntiles; grp += gridDim.x) {
+    const Tile my = group_tile(a.tiles, a.ntiles, grp, lane);
+    wave_fence();
+    stage4<NP, S>(a.u, my, lane, sU);
+    wave_fence();
+    if ((lane & 15) < my.count) {
+      const double* su = sU + lane * S;
+      // zero offset that depends on the tile, so the operator loads stay in the loop
+      const int64_t z = (int64_t)(__builtin_amdgcn_readfirstlane(grp) >> 30);
+      const cdouble_ptr Vi = (cdouble_ptr)(a.Vinv + z);
+      double uk[NP];
+      double umax = 0.0;
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        uk[j] = su[j];
+        umax = fmax(umax, fabs(uk[j]));
+      }
+      double uh[NP];
+      double tot = 0.0, top = 0.0;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) s = fma(Vi[i * NP + j], uk[j], s);
+        uh[i] = s;
+        tot = fma(s, s, tot);
+        if (MD.d[i] == N) top = fma(s, s, top);
+      }
+      const double Sv = tot > 0.0 ? top / tot : 0.0;
+      const double fr = pp_fraction(Sv, a.s0, a.kappa);
+      const int64_t g = (int64_t)my.gbase + (lane & 15);
+      const double vol = (4.0 / 3.0) * a.geo[9 * a.K + g];
+      const double h = cbrt(6.0 * vol);
+      const double e0 = a.ceps * h * sqrt(3.0) * umax / N;
+      const double e = e0 * fr;
+      a.eps[g] = e;
+      emax = fmax(emax, e);
+      if (a.frac) a.frac[g] = fr;
+      if (a.frac32) a.frac32[g] = (float)fr;
+      if (a.feats) {
+        const double inv = tot > 0.0 ? 1.0 / sqrt(tot) : 0.0;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) a.feats[g * NP + i] = (float)(uh[i] * inv);
+      }
+    }
+  }
+  if (a.maxeps) {
+    emax = wmax(emax);
+    if (lane == 0 && emax > 0.0)
+      atomicMax((unsigned long long*)a.maxeps, (unsigned long long)__double_as_longlong(emax));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gradient q = grad u + LIFT(Fscale n (u+ - u-)/2), FP64, one element per lane.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(64) av_grad_kernel(AvArgs a) {
+  constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP, S = NP + 1;
+  constexpr FmaskTable<N> FM{};
+  __shared__ double sU[64 * S];
+  __shared__ double sO[3][64 * S];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < 4 * 24 * NFP; i += 64) sFN[i] = a.face_nodes[i];
+  const int64_t K = a.K;
+  for (int grp = blockIdx.x; 4 * grp < a.ntiles; grp += gridDim.x) {
+    const Tile my = group_tile(a.tiles, a.ntiles, grp, lane);
+    wave_fence();
+    stage4<NP, S>(a.u, my, lane, sU);
+    wave_fence();
+    if ((lane & 15) < my.count) {
+      const double* su = sU + lane * S;
+      const int64_t g = (int64_t)my.gbase + (lane & 15);
+      double mt[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + g];
+      double nf[4][3];
+      double jmp[4][NFP];
+      const int packed = a.codes[g];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const double fs = a.geo[(13 + 4 * f) * K + g];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) nf[f][d] = a.geo[(10 + 4 * f + d) * K + g];
+        const int code = (packed >> (5 * f)) & 31;
+        const int32_t nb = a.nbr[f * K + g];
+#pragma unroll
+        for (int fp = 0; fp < NFP; ++fp) {
+          const double uM = su[FM.v[f][fp]];
+          const double uP = a.u[nb + sFN[(f * 24 + code) * NFP + fp]];
+          jmp[f][fp] = 0.5 * fs * (uP - uM);
+        }
+      }
+      const int64_t z = (int64_t)(__builtin_amdgcn_readfirstlane(grp) >> 30);
+      const cdouble_ptr Dr = (cdouble_ptr)(a.Dr + z), Ds = (cdouble_ptr)(a.Ds + z),
+                        Dt = (cdouble_ptr)(a.Dt + z), LF = (cdouble_ptr)(a.LIFT + z);
+#pragma unroll 1
+      for (int i = 0; i < NP; ++i) {
+        double ur = 0.0, us = 0.0, ut = 0.0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          const double uj = su[j];
+          ur = fma(Dr[i * NP + j], uj, ur);
+          us = fma(Ds[i * NP + j], uj, us);
+          ut = fma(Dt[i * NP + j], uj, ut);
+        }
+        double T[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          double s = 0.0;
+#pragma unroll
+          for (int fp = 0; fp < NFP; ++fp) s = fma(LF[i * NF + f * NFP + fp], jmp[f][fp], s);
+          T[f] = s;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double v = mt[d] * ur + mt[3 + d] * us + mt[6 + d] * ut;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) v = fma(nf[f][d], T[f], v);
+          sO[d][lane * S + i] = v;
+        }
+      }
+    }
+    wave_fence();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) unstage4<NP, S, false>(a.qo[d], my, lane, sO[d]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rhs += L_v = div(eps q) - LIFT(Fscale (n.(s- - s+)/2 + tau (u- - u+))),
+// FP64, one element per lane.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(64) av_visc_kernel(AvArgs a) {
+  constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP, S = NP + 1;
+  constexpr FmaskTable<N> FM{};
+  __shared__ double sU[64 * S];
+  __shared__ double sQ[3][64 * S];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < 4 * 24 * NFP; i += 64) sFN[i] = a.face_nodes[i];
+  const int64_t K = a.K;
+  for (int grp = blockIdx.x; 4 * grp < a.ntiles; grp += gridDim.x) {
+    const Tile my = group_tile(a.tiles, a.ntiles, grp, lane);
+    wave_fence();
+    stage4<NP, S>(a.u, my, lane, sU);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) stage4<NP, S>(a.q[d], my, lane, sQ[d]);
+    wave_fence();
+    if ((lane & 15) < my.count) {
+      const int64_t g = (int64_t)my.gbase + (lane & 15);
+      const double ee = a.eps_in[g];
+      double mt[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + g];
+      double* qx = sQ[0] + lane * S;
+      double* qy = sQ[1] + lane * S;
+      double* qz = sQ[2] + lane * S;
+      double* su = sU + lane * S;
+      double fl[4][NFP];
+      const int packed = a.codes[g];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const double fs = a.geo[(13 + 4 * f) * K + g];
+        const double nx = a.geo[(10 + 4 * f) * K + g], ny = a.geo[(11 + 4 * f) * K + g],
+                     nz = a.geo[(12 + 4 * f) * K + g];
+        const int code = (packed >> (5 * f)) & 31;
+        const int32_t nb = a.nbr[f * K + g];
+        const double ep = a.eps_in[a.nbe[f * K + g]];
+        const double tau = a.tau0 * fmax(ee, ep);
+#pragma unroll
+        for (int fp = 0; fp < NFP; ++fp) {
+          const int nm = FM.v[f][fp];
+          const int32_t ip = nb + sFN[(f * 24 + code) * NFP + fp];
+          const double sM = ee * (nx * qx[nm] + ny * qy[nm] + nz * qz[nm]);
+          const double sP = ep * (nx * a.q[0][ip] + ny * a.q[1][ip] + nz * a.q[2][ip]);
+          fl[f][fp] = -fs * (0.5 * (sM - sP) + tau * (su[nm] - a.u[ip]));
+        }
+      }
+      // Contravariant fluxes eps (r_x q_x + r_y q_y + r_z q_z), ..., in place.
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        const double x = qx[j], y = qy[j], zz = qz[j];
+        qx[j] = ee * (mt[0] * x + mt[1] * y + mt[2] * zz);
+        qy[j] = ee * (mt[3] * x + mt[4] * y + mt[5] * zz);
+        qz[j] = ee * (mt[6] * x + mt[7] * y + mt[8] * zz);
+      }
+      const int64_t z = (int64_t)(__builtin_amdgcn_readfirstlane(grp) >> 30);
+      const cdouble_ptr Dr = (cdouble_ptr)(a.Dr + z), Ds = (cdouble_ptr)(a.Ds + z),
+                        Dt = (cdouble_ptr)(a.Dt + z), LF = (cdouble_ptr)(a.LIFT + z);
+#pragma unroll 1
+      for (int i = 0; i < NP; ++i) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          v = fma(Dr[i * NP + j], qx[j], v);
+          v = fma(Ds[i * NP + j], qy[j], v);
+          v = fma(Dt[i * NP + j], qz[j], v);
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+          for (int fp = 0; fp < NFP; ++fp) v = fma(LF[i * NF + f * NFP + fp], fl[f][fp], v);
+        su[i] = v;  // this lane's row of u is no longer read
+      }
+    }
+    wave_fence();
+    unstage4<NP, S, true>(a.rhs, my, lane, sU);
+  }
+}
+
+// dt = min(cfl hmin / max|u|, cfl_v hmin^2 / max eps), clipped to t_end;
+// scal = [max|u|, dt, t, max eps]; max|u| and max eps are cleared for the
+// next step's reductions.
+__global__ void av_dt_kernel(double* scal, double cfl, double cfl_v, double hmin, double t_end) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double m = scal[0], e = scal[3], t = scal[2];
+    double dt = cfl * hmin / m;
+    if (e > 0.0) dt = fmin(dt, cfl_v * hmin * hmin / e);
+    if (t_end > 0.0) dt = fmin(dt, fmax(t_end - t, 0.0));
+    scal[1] = dt;
+    scal[2] = t + dt;
+    scal[0] = 0.0;
+    scal[3] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sensor network: y = sigmoid(w3 . tanh(W2 tanh(W1 x + b1) + b2) + b3),
+// x = modal coefficients / their Euclidean norm, eps = eps0 y.
+// Parameters (FP32 master copy): W1[64][nin], b1[64], W2[64][64], b2[64],
+// w3[64], b3.
+// ---------------------------------------------------------------------------
+struct MlpLayout {
+  int nin, oW1, ob1, oW2, ob2, ow3, ob3, total;
+};
+
+__host__ __device__ inline MlpLayout mlp_layout(int nin) {
+  MlpLayout L;
+  L.nin = nin;
+  L.oW1 = 0;
+  L.ob1 = kH * nin;
+  L.oW2 = L.ob1 + kH;
+  L.ob2 = L.oW2 + kH * kH;
+  L.ow3 = L.ob2 + kH;
+  L.ob3 = L.ow3 + kH;
+  L.total = L.ob3 + 1;
+  return L;
+}
+
+// Feature index of element j of the 8-wide BF16 k slice held by lane group g
+// in k step s: the rows a 16x16 FP32 result places in that lane.
+__host__ __device__ inline int perm_k(int s, int g, int j) {
+  return 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
+}
+
+// Per-lane network state: BF16 fragments of W1 and W2 in the permuted k
+// order, and the biases / output weights of the rows this lane's results
+// hold (rows 16 mt + 4 g + r).
+struct MlpRegs {
+  bf16x8 A1[4][2], A2[4][2];
+  float b1[4][4], b2[4][4], w3[4][4];
+  float b3;
+};
+
+__device__ __forceinline__ void mlp_load(const float* __restrict__ P, const MlpLayout& L, int lane,
+                                         MlpRegs& R) {
+  const int row = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = perm_k(s, g, j);
+        const float w1 = k < L.nin ? P[L.oW1 + (16 * mt + row) * L.nin + k] : 0.f;
+        R.A1[mt][s][j] = (__bf16)w1;
+        R.A2[mt][s][j] = (__bf16)P[L.oW2 + (16 * mt + row) * kH + k];
+      }
+    }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int h = 16 * mt + 4 * g + r;
+      R.b1[mt][r] = P[L.ob1 + h];
+      R.b2[mt][r] = P[L.ob2 + h];
+      R.w3[mt][r] = P[L.ow3 + h];
+    }
+  R.b3 = P[L.ob3];
+}
+
+// Forward pass of 16 inputs (columns); X[s] is lane (c, g)'s BF16 slice of
+// input c in k step s. Returns y for column c (all four lane groups).
+__device__ __forceinline__ float mlp_forward_tile(const MlpRegs& R, const bf16x8 (&X)[2]) {
+  floatx4 z[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    z[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.A1[mt][s], X[s], z[mt], 0, 0, 0);
+  }
+  bf16x8 H[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int mt = 2 * s + (j >> 2), r = j & 3;
+      H[s][j] = (__bf16)tanhf(z[mt][r] + R.b1[mt][r]);
+    }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    z[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.A2[mt][s], H[s], z[mt], 0, 0, 0);
+  }
+  float part = 0.f;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part = fmaf(R.w3[mt][r], tanhf(z[mt][r] + R.b2[mt][r]), part);
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  return 1.f / (1.f + expf(-(part + R.b3)));
+}
+
+// Network on given features x[n][nin] (FP32) -> y[n]; 16 inputs per wave step.
+__global__ void __launch_bounds__(256) mlp_features_kernel(const float* __restrict__ P, int nin,
+                                                           const float* __restrict__ x, int64_t n,
+                                                           float* __restrict__ y) {
+  const MlpLayout L = mlp_layout(nin);
+  const int lane = threadIdx.x & 63;
+  MlpRegs R;
+  mlp_load(P, L, lane, R);
+  const int c = lane & 15, g = lane >> 4;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t * 16 < n; t += nw) {
+    const int64_t e = t * 16 + c;
+    const int64_t ec = e < n ? e : n - 1;
+    bf16x8 X[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = perm_k(s, g, j);
+        X[s][j] = (__bf16)(k < nin ? x[ec * nin + k] : 0.f);
+      }
+    const float yv = mlp_forward_tile(R, X);
+    if (g == 0 && e < n) y[e] = yv;
+  }
+}
+
+// Learned sensor on the state: per 16-element tile, uh = V^-1 u on FP32
+// MFMA, x = uh / |uh|, y = network(x) on BF16 MFMA, eps = eps0 y.
+template <int N>
+__global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* __restrict__ P) {
+  constexpr int NP = Tet<N>::Np, S = NP + 1;
+  constexpr int MTM = (NP + 15) / 16, KSM = (NP + 3) / 4;
+  static_assert(NP <= 48, "modal rows must fit three 16-row tiles");
+  __shared__ double sUall[4][16 * S];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  double* sU = sUall[w];
+  const MlpLayout L = mlp_layout(NP);
+  MlpRegs R;
+  mlp_load(P, L, lane, R);
+  // V^-1 fragments (FP32 A operand of 16x16x4): A[16 mt + row][4 s + g].
+  float vf[MTM][KSM];
+#pragma unroll
+  for (int mt = 0; mt < MTM; ++mt)
+#pragma unroll
+    for (int s = 0; s < KSM; ++s) {
+      const int i = 16 * mt + c, k = 4 * s + g;
+      vf[mt][s] = (i < NP && k < NP) ? (float)a.Vinv[i * NP + k] : 0.f;
+    }
+  double emax = 0.0;
+  const int nw = gridDim.x * 4;
+  for (int t = blockIdx.x * 4 + w; t < a.ntiles; t += nw) {
+    const Tile tl = a.tiles[t];
+    wave_fence();
+    for (int i = lane; i < tl.count * NP; i += 64) {
+      const int e = i / NP, nn = i - e * NP;
+      sU[e * S + nn] = a.u[tl.vbase + i];
+    }
+    wave_fence();
+    const int cc = c < tl.count ? c : 0;
+    floatx4 acc[MTM];
+#pragma unroll
+    for (int mt = 0; mt < MTM; ++mt) acc[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    double umax = 0.0;
+#pragma unroll
+    for (int s = 0; s < KSM; ++s) {
+      const int k = 4 * s + g;
+      const double uv = k < NP ? sU[cc * S + k] : 0.0;
+      umax = fmax(umax, fabs(uv));
+#pragma unroll
+      for (int mt = 0; mt < MTM; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(vf[mt][s], (float)uv, acc[mt], 0, 0, 0);
+    }
+    // lane (c, g) holds uh rows 16 mt + 4 g + r of element c
+    float tot = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < MTM; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tot = fmaf(acc[mt][r], acc[mt][r], tot);
+    tot += __shfl_xor(tot, 16, 64);
+    tot += __shfl_xor(tot, 32, 64);
+    umax = fmax(umax, __shfl_xor(umax, 16, 64));
+    umax = fmax(umax, __shfl_xor(umax, 32, 64));
+    const float inv = tot > 0.f ? 1.f / sqrtf(tot) : 0.f;
+    bf16x8 X[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int mt = 2 * s + (j >> 2), r = j & 3;
+        X[s][j] = (__bf16)(mt < MTM ? acc[mt < MTM ? mt : 0][r] * inv : 0.f);
+      }
+    const float y = mlp_forward_tile(R, X);
+    if (g == 0 && c < tl.count) {
+      const int64_t ge = (int64_t)tl.gbase + c;
+      const double vol = (4.0 / 3.0) * a.geo[9 * a.K + ge];
+      const double h = cbrt(6.0 * vol);
+      const double e = a.ceps * h * sqrt(3.0) * umax / N * (double)y;
+      a.eps[ge] = e;
+      if (a.frac) a.frac[ge] = (double)y;
+      emax = fmax(emax, e);
+    }
+  }
+  if (a.maxeps) {
+    emax = wmax(emax);
+    if (lane == 0 && emax > 0.0)
+      atomicMax((unsigned long long*)a.maxeps, (unsigned long long)__double_as_longlong(emax));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training: FP32, one sample per lane, 64 samples per block. Forward and
+// backward of the mean squared error of y against the target; the block's
+// parameter-gradient partial sums (over its samples) are added to grad with
+// float atomics. Samples: index i of the batch takes data row
+// hash(seed, i) % ndata, or row i when seed == 0 (whole-set gradients).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+typedef const __attribute__((address_space(4))) float* cfloat_ptr;
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(64) mlp_train_kernel(const float* __restrict__ Pg, int nin,
+                                                       const float* __restrict__ x,
+                                                       const float* __restrict__ tgt, int64_t ndata,
+                                                       int batch, uint64_t seed,
+                                                       float* __restrict__ grad,
+                                                       float* __restrict__ loss) {
+  const MlpLayout L = mlp_layout(nin);
+  // Parameters are read with wave-uniform addresses (scalar loads).
+  const cfloat_ptr P = (cfloat_ptr)Pg;
+  extern __shared__ float sm[];
+  float* sX = sm;                 // [64][nin]
+  float* sH1 = sX + 64 * nin;     // [64][64]
+  float* sD1 = sH1 + 64 * kH;
+  float* sD2 = sD1 + 64 * kH;
+  const int t = threadIdx.x;
+  const int64_t bi = (int64_t)blockIdx.x * 64 + t;
+  const bool valid = bi < batch;
+  const int64_t row = !valid ? 0
+                      : (seed ? (int64_t)(mix64(seed + 0x9e3779b97f4a7c15ULL * (uint64_t)(bi + 1)) %
+                                          (uint64_t)ndata)
+                              : bi);
+  for (int j = 0; j < nin; ++j) sX[t * nin + j] = valid ? x[row * nin + j] : 0.f;
+  float h1[kH];
+  for (int i = 0; i < kH; ++i) {
+    float z = P[L.ob1 + i];
+    for (int j = 0; j < nin; ++j) z = fmaf(P[L.oW1 + i * nin + j], sX[t * nin + j], z);
+    h1[i] = tanhf(z);
+    sH1[t * kH + i] = h1[i];
+  }
+  float h2[kH];
+  float acc = P[L.ob3];
+  for (int i = 0; i < kH; ++i) {
+    float z = P[L.ob2 + i];
+    for (int j = 0; j < kH; ++j) z = fmaf(P[L.oW2 + i * kH + j], h1[j], z);
+    h2[i] = tanhf(z);
+    acc = fmaf(P[L.ow3 + i], h2[i], acc);
+  }
+  const float y = 1.f / (1.f + expf(-acc));
+  const float err = valid ? y - tgt[row] : 0.f;
+  const float dy = 2.f * err / (float)batch * y * (1.f - y);
+  const float lsum = wave_sum(err * err / (float)batch);
+  if (t == 0) atomicAdd(loss, lsum);
+  // d/dw3 and d/db3: sums over the block's samples by wave reduction.
+  float gw3 = 0.f;
+  for (int i = 0; i < kH; ++i) {
+    const float v = wave_sum(dy * h2[i]);
+    if (t == i) gw3 = v;
+  }
+  const float gb3 = wave_sum(dy);
+  for (int i = 0; i < kH; ++i) sD2[t * kH + i] = dy * P[L.ow3 + i] * (1.f - h2[i] * h2[i]);
+  for (int j = 0; j < kH; ++j) {
+    float s = 0.f;
+    for (int i = 0; i < kH; ++i) s = fmaf(P[L.oW2 + i * kH + j], sD2[t * kH + i], s);
+    sD1[t * kH + j] = s * (1.f - h1[j] * h1[j]);
+  }
+  __syncthreads();
+  // Partial sums over this block's samples.
+  for (int p = t; p < L.ow3; p += 64) {
+    float s = 0.f;
+    if (p < L.ob1) {
+      const int i = p / nin, j = p - i * nin;
+      for (int q = 0; q < 64; ++q) s = fmaf(sD1[q * kH + i], sX[q * nin + j], s);
+    } else if (p < L.oW2) {
+      const int i = p - L.ob1;
+      for (int q = 0; q < 64; ++q) s += sD1[q * kH + i];
+    } else if (p < L.ob2) {
+      const int i = (p - L.oW2) / kH, j = (p - L.oW2) % kH;
+      for (int q = 0; q < 64; ++q) s = fmaf(sD2[q * kH + i], sH1[q * kH + j], s);
+    } else {
+      const int i = p - L.ob2;
+      for (int q = 0; q < 64; ++q) s += sD2[q * kH + i];
+    }
+    atomicAdd(grad + p, s);
+  }
+  atomicAdd(grad + L.ow3 + t, gw3);
+  if (t == 0) atomicAdd(grad + L.ob3, gb3);
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters; clears grad.
+__global__ void adam_kernel(float* __restrict__ P, float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int n, float lr, float b1, float b2, float eps,
+                            float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  g[i] = 0.f;
+}
+
+size_t train_smem(int nin) { return ((size_t)64 * nin + 3 * 64 * kH) * sizeof(float); }
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct dgk_av_s {
+  dgk_dgmesh m = nullptr;
+  double* Vinv = nullptr;
+  double* eps = nullptr;
+  double* q[3] = {nullptr, nullptr, nullptr};
+  double* zero_eps = nullptr;
+  dgk_av_params p{};
+};
+
+struct dgk_mlp_s {
+  dgk_device dev = nullptr;
+  int nin = 0, total = 0;
+  long step = 0;
+  float* P = nullptr;
+  float* g = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  float* loss = nullptr;
+};
+
+namespace {
+
+AvArgs av_args(dgk_av a) {
+  dgk_dgmesh m = a->m;
+  AvArgs r{};
+  r.tiles = m->tiles64;
+  r.ntiles = m->ntiles64;
+  r.K = m->K;
+  r.geo = m->geo;
+  r.Dr = m->Dr;
+  r.Ds = m->Ds;
+  r.Dt = m->Dt;
+  r.LIFT = m->LIFT;
+  r.Vinv = a->Vinv;
+  r.face_nodes = m->face_nodes;
+  r.nbr = m->nbr;
+  r.nbe = m->nbe;
+  r.codes = m->codes;
+  r.eps = a->eps;
+  r.eps_in = a->eps;
+  for (int d = 0; d < 3; ++d) {
+    r.q[d] = a->q[d];
+    r.qo[d] = a->q[d];
+  }
+  r.s0 = a->p.s0;
+  r.kappa = a->p.kappa;
+  r.ceps = a->p.ceps;
+  r.tau0 = a->p.tau0;
+  return r;
+}
+
+int groups_grid(dgk_dgmesh m, int per_cu) {
+  const int groups = (m->ntiles64 + 3) / 4;
+  return std::max(1, std::min(groups, m->dev->cu_count * per_cu));
+}
+
+#define DGK_AV_LAUNCH(kernel, grid, block, shmem, ...)                               \
+  switch (m->N) {                                                                    \
+    case 1: hipLaunchKernelGGL(kernel<1>, grid, block, shmem, m->dev->stream, __VA_ARGS__); break; \
+    case 2: hipLaunchKernelGGL(kernel<2>, grid, block, shmem, m->dev->stream, __VA_ARGS__); break; \
+    case 3: hipLaunchKernelGGL(kernel<3>, grid, block, shmem, m->dev->stream, __VA_ARGS__); break; \
+    case 4: hipLaunchKernelGGL(kernel<4>, grid, block, shmem, m->dev->stream, __VA_ARGS__); break; \
+    default:                                                                         \
+      dgk::set_error("shock capturing is built for N <= 4 (N=%d)", m->N);            \
+      return DGK_ERR_UNSUPPORTED;                                                    \
+  }
+
+// rhs += L_v(u; eps) (gradient kernel into a->q, then the viscous kernel).
+int av_viscous(dgk_av a, const double* u, double* rhs) {
+  dgk_dgmesh m = a->m;
+  if (m->ntiles64 == 0) return DGK_OK;
+  AvArgs r = av_args(a);
+  r.u = u;
+  r.rhs = rhs;
+  const dim3 grid(groups_grid(m, 8)), block(64);
+  DGK_AV_LAUNCH(av_grad_kernel, grid, block, 0, r);
+  DGK_LAUNCH_CHECK();
+  DGK_AV_LAUNCH(av_visc_kernel, grid, block, 0, r);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+}  // namespace
+
+extern "C" int dgk_av_create(dgk_dgmesh m, const double* Vinv_host, const dgk_av_params* p,
+                             dgk_av* out) {
+  dgk::clear_error();
+  DGK_REQUIRE(m && Vinv_host && p && out, "dgk_av_create: NULL argument");
+  DGK_REQUIRE(m->N >= 1 && m->N <= 4, "dgk_av_create: shock capturing is built for N <= 4");
+  DGK_REQUIRE(p->kappa > 0.0 && p->ceps >= 0.0 && p->tau0 >= 0.0,
+              "dgk_av_create: need kappa > 0, ceps >= 0, tau0 >= 0");
+  *out = nullptr;
+  dgk_av a = new dgk_av_s();
+  a->m = m;
+  a->p = *p;
+  const size_t nstate = (size_t)m->state_off.back() * 8 + 16;
+  int rc = DGK_OK;
+  rc |= dgk_malloc(m->dev, (size_t)m->Np * m->Np * 8, Vinv_host, (void**)&a->Vinv);
+  rc |= dgk_malloc(m->dev, (size_t)m->K * 8 + 8, nullptr, (void**)&a->eps);
+  for (int d = 0; d < 3; ++d) rc |= dgk_malloc(m->dev, nstate, nullptr, (void**)&a->q[d]);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(m->dev, a->eps, 0, (size_t)m->K * 8 + 8);
+  if (rc == DGK_OK) rc = dgk::split_prepare(m);
+  if (rc != DGK_OK) {
+    dgk_av_destroy(a);
+    return rc;
+  }
+  *out = a;
+  return DGK_OK;
+}
+
+extern "C" int dgk_av_destroy(dgk_av a) {
+  if (!a) return DGK_OK;
+  dgk_device dev = a->m->dev;
+  dgk_free(dev, a->Vinv);
+  dgk_free(dev, a->eps);
+  for (int d = 0; d < 3; ++d) dgk_free(dev, a->q[d]);
+  delete a;
+  return DGK_OK;
+}
+
+extern "C" int dgk_av_eps_dev(dgk_av a, double** eps_dev) {
+  DGK_REQUIRE(a && eps_dev, "dgk_av_eps_dev: NULL argument");
+  *eps_dev = a->eps;
+  return DGK_OK;
+}
+
+extern "C" int dgk_av_sensor_pp(dgk_av a, const double* u_dev, double* maxeps_dev,
+                                double* frac_dev, float* frac32_dev, float* feats_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(a && u_dev, "dgk_av_sensor_pp: NULL argument");
+  dgk_dgmesh m = a->m;
+  if (m->ntiles64 == 0) return DGK_OK;
+  AvArgs r = av_args(a);
+  r.u = u_dev;
+  r.maxeps = maxeps_dev;
+  r.frac = frac_dev;
+  r.frac32 = frac32_dev;
+  r.feats = feats_dev;
+  const dim3 grid(groups_grid(m, 8)), block(64);
+  DGK_AV_LAUNCH(pp_sensor_kernel, grid, block, 0, r);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+extern "C" int dgk_av_sensor_mlp(dgk_av a, dgk_mlp net, const double* u_dev, double* maxeps_dev,
+                                 double* frac_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(a && net && u_dev, "dgk_av_sensor_mlp: NULL argument");
+  dgk_dgmesh m = a->m;
+  DGK_REQUIRE(net->nin == m->Np, "dgk_av_sensor_mlp: network input width %d != Np %d", net->nin,
+              m->Np);
+  if (m->ntiles64 == 0) return DGK_OK;
+  AvArgs r = av_args(a);
+  r.u = u_dev;
+  r.maxeps = maxeps_dev;
+  r.frac = frac_dev;
+  const int grid = std::max(1, std::min((m->ntiles64 + 3) / 4, m->dev->cu_count * 4));
+  DGK_AV_LAUNCH(mlp_sensor_kernel, dim3(grid), dim3(256), 0, r, (const float*)net->P);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+extern "C" int dgk_av_rhs(dgk_av a, const double* u_dev, double* out_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(a && u_dev && out_dev, "dgk_av_rhs: NULL argument");
+  DGK_REQUIRE(out_dev != u_dev, "dgk_av_rhs: out must not alias u");
+  const size_t nstate = (size_t)a->m->state_off.back() * 8;
+  int rc = dgk_fill_bytes(a->m->dev, out_dev, 0, nstate);
+  if (rc != DGK_OK) return rc;
+  return av_viscous(a, u_dev, out_dev);
+}
+
+extern "C" int dgk_av_stage(dgk_av a, const double* u_dev, const double* u0_dev, double* out_dev,
+                            double a0, double a1, const double* dt_dev, double* maxabs_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(a && u_dev && out_dev && dt_dev, "dgk_av_stage: NULL argument");
+  DGK_REQUIRE(out_dev != u_dev, "dgk_av_stage: out must not alias u");
+  DGK_REQUIRE(a0 == 0.0 || u0_dev, "dgk_av_stage: u0 is NULL with a0 != 0");
+  dgk_dgmesh m = a->m;
+  int rc = dgk::split_volume(m, u_dev);
+  if (rc != DGK_OK) return rc;
+  rc = av_viscous(a, u_dev, m->rhs);
+  if (rc != DGK_OK) return rc;
+  return dgk::split_surface(m, u_dev, u0_dev ? u0_dev : u_dev, out_dev, a0, a1, dt_dev,
+                            (unsigned long long*)maxabs_dev, 0);
+}
+
+extern "C" int dgk_av_rhs_total(dgk_av a, const double* u_dev, double* out_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(a && u_dev && out_dev, "dgk_av_rhs_total: NULL argument");
+  DGK_REQUIRE(out_dev != u_dev, "dgk_av_rhs_total: out must not alias u");
+  dgk_dgmesh m = a->m;
+  int rc = dgk::split_volume(m, u_dev);
+  if (rc != DGK_OK) return rc;
+  rc = av_viscous(a, u_dev, m->rhs);
+  if (rc != DGK_OK) return rc;
+  return dgk::split_surface(m, u_dev, u_dev, out_dev, 0.0, 1.0, nullptr, nullptr, 1);
+}
+
+extern "C" int dgk_av_cfl_dt(dgk_device dev, double* scal_dev, double cfl, double cfl_v,
+                             double hmin, double t_end) {
+  dgk::clear_error();
+  DGK_REQUIRE(dev && scal_dev, "dgk_av_cfl_dt: NULL argument");
+  hipLaunchKernelGGL(av_dt_kernel, dim3(1), dim3(64), 0, dev->stream, scal_dev, cfl, cfl_v, hmin,
+                     t_end);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+// ------------------------------- network ----------------------------------
+extern "C" int dgk_mlp_create(dgk_device dev, int nin, const float* params_host, dgk_mlp* out) {
+  dgk::clear_error();
+  DGK_REQUIRE(dev && params_host && out, "dgk_mlp_create: NULL argument");
+  // the training kernel's LDS (64 samples x (nin + 3 x 64) floats) stays within 64 KiB
+  DGK_REQUIRE(nin >= 1 && nin <= 56, "dgk_mlp_create: input width %d outside 1..56", nin);
+  *out = nullptr;
+  dgk_mlp n = new dgk_mlp_s();
+  n->dev = dev;
+  n->nin = nin;
+  n->total = mlp_layout(nin).total;
+  const size_t bytes = (size_t)n->total * 4;
+  int rc = DGK_OK;
+  rc |= dgk_malloc(dev, bytes, params_host, (void**)&n->P);
+  rc |= dgk_malloc(dev, bytes, nullptr, (void**)&n->g);
+  rc |= dgk_malloc(dev, bytes, nullptr, (void**)&n->m);
+  rc |= dgk_malloc(dev, bytes, nullptr, (void**)&n->v);
+  rc |= dgk_malloc(dev, 16, nullptr, (void**)&n->loss);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->g, 0, bytes);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->m, 0, bytes);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->v, 0, bytes);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->loss, 0, 16);
+  if (rc != DGK_OK) {
+    dgk_mlp_destroy(n);
+    return rc;
+  }
+  *out = n;
+  return DGK_OK;
+}
+
+extern "C" int dgk_mlp_destroy(dgk_mlp n) {
+  if (!n) return DGK_OK;
+  dgk_free(n->dev, n->P);
+  dgk_free(n->dev, n->g);
+  dgk_free(n->dev, n->m);
+  dgk_free(n->dev, n->v);
+  dgk_free(n->dev, n->loss);
+  delete n;
+  return DGK_OK;
+}
+
+extern "C" int dgk_mlp_num_params(dgk_mlp n, int* count) {
+  DGK_REQUIRE(n && count, "dgk_mlp_num_params: NULL argument");
+  *count = n->total;
+  return DGK_OK;
+}
+
+extern "C" int dgk_mlp_get(dgk_mlp n, float* params_host) {
+  DGK_REQUIRE(n && params_host, "dgk_mlp_get: NULL argument");
+  return dgk_copy_d2h(n->dev, params_host, n->P, (size_t)n->total * 4, 0);
+}
+
+extern "C" int dgk_mlp_set(dgk_mlp n, const float* params_host) {
+  DGK_REQUIRE(n && params_host, "dgk_mlp_set: NULL argument");
+  n->step = 0;
+  const size_t bytes = (size_t)n->total * 4;
+  int rc = dgk_copy_h2d(n->dev, n->P, params_host, bytes, 0);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->m, 0, bytes);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->v, 0, bytes);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->g, 0, bytes);
+  return rc;
+}
+
+extern "C" int dgk_mlp_forward(dgk_mlp n, const float* x_dev, int64_t count, float* y_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(n && x_dev && y_dev, "dgk_mlp_forward: NULL argument");
+  DGK_REQUIRE(count >= 0, "dgk_mlp_forward: negative count");
+  if (count == 0) return DGK_OK;
+  const int64_t tiles = (count + 15) / 16;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, n->dev->cu_count * 4));
+  hipLaunchKernelGGL(mlp_features_kernel, dim3(grid), dim3(256), 0, n->dev->stream, n->P, n->nin,
+                     x_dev, count, y_dev);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+// Gradient of the mean squared error over all `count` rows into grad_dev
+// (FP32 [num_params]); loss_dev (may be NULL) receives the loss.
+extern "C" int dgk_mlp_grad(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t count,
+                            float* grad_dev, float* loss_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(n && x_dev && t_dev && grad_dev, "dgk_mlp_grad: NULL argument");
+  DGK_REQUIRE(count >= 1 && count <= (1 << 30), "dgk_mlp_grad: count %lld", (long long)count);
+  int rc = dgk_fill_bytes(n->dev, grad_dev, 0, (size_t)n->total * 4);
+  if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->loss, 0, 4);
+  if (rc != DGK_OK) return rc;
+  const int blocks = (int)((count + 63) / 64);
+  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(64), train_smem(n->nin), n->dev->stream,
+                     n->P, n->nin, x_dev, t_dev, count, (int)count, (uint64_t)0, grad_dev, n->loss);
+  DGK_LAUNCH_CHECK();
+  if (loss_dev) return dgk_copy_d2d(n->dev, loss_dev, n->loss, 4);
+  return DGK_OK;
+}
+
+// One Adam step on a batch sampled (with replacement) from the ndata rows by
+// a hash of (seed, step); loss of the batch added to *loss_acc_dev if not
+// NULL.
+extern "C" int dgk_mlp_train_step(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t ndata,
+                                  int batch, uint64_t seed, float lr, float* loss_acc_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(n && x_dev && t_dev, "dgk_mlp_train_step: NULL argument");
+  DGK_REQUIRE(ndata >= 1 && batch >= 1, "dgk_mlp_train_step: empty data or batch");
+  n->step += 1;
+  const uint64_t s = (seed | 1ULL) * 0x2545F4914F6CDD1DULL + (uint64_t)n->step;
+  const int blocks = (batch + 63) / 64;
+  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(64), train_smem(n->nin), n->dev->stream,
+                     n->P, n->nin, x_dev, t_dev, ndata, batch, s ? s : 1ULL, n->g,
+                     loss_acc_dev ? loss_acc_dev : n->loss);
+  DGK_LAUNCH_CHECK();
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)n->step), c2 = 1.f - powf(b2, (float)n->step);
+  hipLaunchKernelGGL(adam_kernel, dim3((n->total + 255) / 256), dim3(256), 0, n->dev->stream, n->P,
+                     n->g, n->m, n->v, n->total, lr, b1, b2, 1e-8f, c1, c2);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}

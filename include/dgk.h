@@ -306,6 +306,87 @@ int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked);
  * The fused kernels (N <= 4, FP64) are used otherwise. */
 int dgk_burgers_set_precision(dgk_dgmesh m, int prec);
 
+/* ---------------------------------------------------------------------- */
+/* Shock capturing (no counterpart in the reference, which has no shock    */
+/* capturing; the BASELINE learned-viscosity workload defines it):         */
+/*   u_t + div(u^2/2 (1,1,1)) = div(eps grad u),  eps constant per element. */
+/* Sensor (Persson & Peraire, AIAA 2006-112), on uh = V^-1 u:              */
+/*   S = sum_{deg m = N} uh_m^2 / sum uh_m^2, s = log10 S,                  */
+/*   eps = eps0 ramp(s), ramp = 0 below s0 - kappa, 1 above s0 + kappa,     */
+/*   (1 + sin(pi (s - s0) / (2 kappa)))/2 between;                          */
+/*   eps0 = ceps h sqrt(3) max|u| / N, h = (6 vol)^(1/3).                   */
+/* Viscous operator, BR1 strong form with a jump penalty:                   */
+/*   q = grad u + LIFT(Fscale n (u+ - u-)/2),                               */
+/*   L_v = div(eps q) - LIFT(Fscale (n.(eps q)- - n.(eps q)+)/2             */
+/*                           + Fscale tau (u- - u+)), tau = tau0 max(eps-,eps+). */
+/* The viscous stage uses the split stage path (volume kernel, viscous      */
+/* kernels, surface kernel). Built for N = 1..4. eps lives in the context   */
+/* (dense global element order, FP64).                                      */
+/* ---------------------------------------------------------------------- */
+typedef struct dgk_av_s* dgk_av;
+typedef struct dgk_mlp_s* dgk_mlp;
+typedef struct {
+  double s0;     /* sensor threshold (log10), e.g. -4 log10(N) */
+  double kappa;  /* ramp half width (log10) */
+  double ceps;   /* eps0 = ceps h lambda / N */
+  double tau0;   /* penalty factor */
+} dgk_av_params;
+
+/* Vinv_host: the Np x Np inverse Vandermonde matrix (row-major) of the
+ * orthonormal basis in Vandermonde3D's mode order. Allocates (call outside
+ * graph capture). */
+int dgk_av_create(dgk_dgmesh m, const double* Vinv_host, const dgk_av_params* p, dgk_av* out);
+int dgk_av_destroy(dgk_av a);
+/* Device pointer of eps [Ktot] (FP64). */
+int dgk_av_eps_dev(dgk_av a, double** eps_dev);
+/* Persson-Peraire sensor of u (runner layout) into eps, FP64. maxeps_dev:
+ * atomic max of eps (must hold 0 or a previous max), or NULL. Optional
+ * outputs (NULL to skip): frac = eps/eps0 (FP64 or FP32, [Ktot]) and the
+ * normalised modal coefficients uh/|uh| (FP32, [Ktot][Np]). */
+int dgk_av_sensor_pp(dgk_av a, const double* u_dev, double* maxeps_dev, double* frac_dev,
+                     float* frac32_dev, float* feats_dev);
+/* Learned sensor: eps = eps0 * net(uh/|uh|), modal transform on FP32 matrix
+ * cores, network on BF16 matrix cores with FP32 accumulation. The network's
+ * input width must be Np. frac_dev (FP64, may be NULL) receives net(.). */
+int dgk_av_sensor_mlp(dgk_av a, dgk_mlp net, const double* u_dev, double* maxeps_dev,
+                      double* frac_dev);
+/* L_v(u; eps) alone into out_dev (runner layout). */
+int dgk_av_rhs(dgk_av a, const double* u_dev, double* out_dev);
+/* L(u) + L_v(u; eps) into out_dev (runner layout). */
+int dgk_av_rhs_total(dgk_av a, const double* u_dev, double* out_dev);
+/* One SSP-RK3 stage with viscosity: out = a0 u0 + a1 (u + dt (L + L_v)),
+ * max|out| folded into *maxabs_dev if not NULL. out may alias u0, not u. */
+int dgk_av_stage(dgk_av a, const double* u_dev, const double* u0_dev, double* out_dev, double a0,
+                 double a1, const double* dt_dev, double* maxabs_dev);
+/* scal_dev = [max|u|, dt, t, max eps] (FP64, device):
+ *   dt = min(cfl hmin / max|u|, cfl_v hmin^2 / max eps) (second term when
+ *   max eps > 0), clipped so that t does not pass t_end (t_end > 0);
+ *   t += dt; max|u| and max eps are cleared. */
+int dgk_av_cfl_dt(dgk_device dev, double* scal_dev, double cfl, double cfl_v, double hmin,
+                  double t_end);
+
+/* Sensor network nin -> 64 -> 64 -> 1 (nin <= 56), tanh hidden, sigmoid output.
+ * Parameters (FP32, flat): W1[64][nin], b1[64], W2[64][64], b2[64],
+ * w3[64], b3. The FP32 copy is the training master; inference rounds W1,
+ * W2, the inputs and the first hidden layer to BF16. */
+int dgk_mlp_create(dgk_device dev, int nin, const float* params_host, dgk_mlp* out);
+int dgk_mlp_destroy(dgk_mlp n);
+int dgk_mlp_num_params(dgk_mlp n, int* count);
+int dgk_mlp_get(dgk_mlp n, float* params_host);
+/* Replaces the parameters and resets the optimiser state. */
+int dgk_mlp_set(dgk_mlp n, const float* params_host);
+/* y[i] = net(x[i]) for x [count][nin] (FP32, device) on BF16 matrix cores. */
+int dgk_mlp_forward(dgk_mlp n, const float* x_dev, int64_t count, float* y_dev);
+/* FP32 gradient of mean_i (net(x_i) - t_i)^2 over all rows into grad_dev
+ * [num_params]; the loss into *loss_dev if not NULL. */
+int dgk_mlp_grad(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t count,
+                 float* grad_dev, float* loss_dev);
+/* One Adam step (beta 0.9, 0.999) with learning rate lr on a batch drawn
+ * with replacement from the ndata rows by a hash of (seed, step count); the
+ * batch loss is added to *loss_acc_dev if not NULL. */
+int dgk_mlp_train_step(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t ndata,
+                       int batch, uint64_t seed, float lr, float* loss_acc_dev);
+
 #ifdef __cplusplus
 }
 #endif

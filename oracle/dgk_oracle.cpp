@@ -866,6 +866,130 @@ void burgersRHS(Solver& S, const double* u, double* rhs) {
   applyPerElement(el.LIFT, S.flux.data(), rhs, K, true);
 }
 
+// ---------------------------------------------------------------------------
+// Shock capturing (not in the reference; the BASELINE learned-viscosity
+// configuration). Written from:
+//  * Persson & Peraire, "Sub-cell shock capturing for discontinuous Galerkin
+//    methods", AIAA 2006-112: with orthonormal modal coefficients uh of an
+//    element, S = sum_{deg m = N} uh_m^2 / sum_m uh_m^2, s = log10 S, and
+//      eps = eps0 * 0                                  s < s0 - kappa
+//                   1/2 (1 + sin(pi (s - s0) / (2 kappa)))  |s - s0| <= kappa
+//                   1                                  s > s0 + kappa,
+//    eps0 = ceps * h * lambda / N, h = (6 vol)^(1/3) (the hex edge for the
+//    Kuhn split), lambda = sqrt(3) max|u| (the largest |f'(u).n|);
+//  * the BR1 discretisation of div(eps grad u) (Bassi & Rebay 1997; H&W 2008
+//    §7.1-7.2), strong form, central fluxes plus a jump penalty:
+//      q     = grad u + LIFT(Fscale n (u+ - u-)/2)
+//      L_v   = div(eps q) - LIFT(Fscale (n.(s- - s+)/2 + tau (u- - u+))),
+//      s     = eps q,  tau = tau0 max(eps-, eps+).
+// The modal order of Vandermonde3D (i, j, k nested, degree i + j + k) gives
+// the top-degree modes.
+// ---------------------------------------------------------------------------
+struct SensorParams {
+  double s0, kappa, ceps;
+};
+
+double ppFraction(double S, const SensorParams& sp) {
+  if (!(S > 0.0)) return 0.0;
+  const double sv = std::log10(S);
+  if (sv < sp.s0 - sp.kappa) return 0.0;
+  if (sv > sp.s0 + sp.kappa) return 1.0;
+  return 0.5 * (1.0 + std::sin(M_PI * (sv - sp.s0) / (2.0 * sp.kappa)));
+}
+
+std::vector<int> modeDegrees(int N) {
+  std::vector<int> d;
+  for (int i = 0; i <= N; ++i)
+    for (int j = 0; j <= N - i; ++j)
+      for (int k = 0; k <= N - i - j; ++k) d.push_back(i + j + k);
+  return d;
+}
+
+// eps[K], frac[K] (eps / eps0), eps0[K]; feats[K*Np] (float, may be NULL):
+// the modal coefficients divided by their Euclidean norm (0 for u = 0).
+void ppSensor(const Solver& S, const double* u, const SensorParams& sp, double* eps, double* frac,
+              double* eps0, float* feats) {
+  const Element& el = S.el;
+  const MeshData& m = S.m;
+  const int Np = el.Np, N = el.N;
+  const std::vector<int> deg = modeDegrees(N);
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < (long)m.K; ++k) {
+    const double* uk = u + (size_t)k * Np;
+    double tot = 0.0, top = 0.0, umax = 0.0;
+    std::vector<double> uh(Np);
+    for (int i = 0; i < Np; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < Np; ++j) a += el.Vinv(i, j) * uk[j];
+      uh[i] = a;
+      tot += a * a;
+      if (deg[i] == N) top += a * a;
+      umax = std::max(umax, std::fabs(uk[i]));
+    }
+    const double Sk = tot > 0.0 ? top / tot : 0.0;
+    const double fr = ppFraction(Sk, sp);
+    const double vol = (4.0 / 3.0) * m.J[(size_t)k * Np];
+    const double h = std::cbrt(6.0 * vol);
+    const double e0 = sp.ceps * h * std::sqrt(3.0) * umax / N;
+    eps[k] = e0 * fr;
+    if (frac) frac[k] = fr;
+    if (eps0) eps0[k] = e0;
+    if (feats) {
+      const double inv = tot > 0.0 ? 1.0 / std::sqrt(tot) : 0.0;
+      for (int i = 0; i < Np; ++i) feats[(size_t)k * Np + i] = (float)(uh[i] * inv);
+    }
+  }
+}
+
+// out = L_v(u; eps) (overwritten).
+void avRHS(Solver& S, const double* u, const double* eps, double tau0, double* out) {
+  const Element& el = S.el;
+  MeshData& m = S.m;
+  const int Np = el.Np, Nfp = el.Nfp, NF = 4 * Nfp;
+  const long K = m.K;
+  const size_t n = (size_t)Np * K, nf = (size_t)NF * K;
+  Vec dr(n), ds(n), dt(n), q[3], g[3], sg[3];
+  for (int d = 0; d < 3; ++d) { q[d].assign(n, 0.0); g[d].assign(nf, 0.0); sg[d].assign(n, 0.0); }
+  applyPerElement(el.Dr, u, dr.data(), K, false);
+  applyPerElement(el.Ds, u, ds.data(), K, false);
+  applyPerElement(el.Dt, u, dt.data(), K, false);
+  const Vec* RX[3] = {&m.rx, &m.ry, &m.rz};
+  const Vec* SX[3] = {&m.sx, &m.sy, &m.sz};
+  const Vec* TX[3] = {&m.tx, &m.ty, &m.tz};
+  const Vec* NX[3] = {&m.nx, &m.ny, &m.nz};
+  for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i)
+      q[d][i] = (*RX[d])[i] * dr[i] + (*SX[d])[i] * ds[i] + (*TX[d])[i] * dt[i];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)nf; ++i)
+      g[d][i] = m.Fscale[i] * (*NX[d])[i] * 0.5 * (u[m.vmapP[i]] - u[m.vmapM[i]]);
+    applyPerElement(el.LIFT, g[d].data(), q[d].data(), K, true);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) sg[d][i] = eps[i / Np] * q[d][i];
+  }
+  std::fill(out, out + n, 0.0);
+  for (int d = 0; d < 3; ++d) {
+    applyPerElement(el.Dr, sg[d].data(), dr.data(), K, false);
+    applyPerElement(el.Ds, sg[d].data(), ds.data(), K, false);
+    applyPerElement(el.Dt, sg[d].data(), dt.data(), K, false);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i)
+      out[i] += (*RX[d])[i] * dr[i] + (*SX[d])[i] * ds[i] + (*TX[d])[i] * dt[i];
+  }
+  Vec fl(nf);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)nf; ++i) {
+    const int vM = m.vmapM[i], vP = m.vmapP[i];
+    const double eM = eps[vM / Np], eP = eps[vP / Np];
+    double ns = 0.0;
+    for (int d = 0; d < 3; ++d) ns += (*NX[d])[i] * (sg[d][vM] - sg[d][vP]);
+    const double tau = tau0 * std::max(eM, eP);
+    fl[i] = -m.Fscale[i] * (0.5 * ns + tau * (u[vM] - u[vP]));
+  }
+  applyPerElement(el.LIFT, fl.data(), out, K, true);
+}
+
 // computeMinimumEdgeLength, examples/3d_scalar_burgers_equation.go:697-717
 double minNodeDistance(const Element& el, const MeshData& m) {
   int Np = el.Np;
@@ -1168,6 +1292,67 @@ int orc_ssprk3(void* h, double* u, int nsteps, double cfl, double hmin, double f
       u[q] = (1.0 / 3.0) * u[q] + (2.0 / 3.0) * (u2[q] + dt * L[q]);
   }
   return 0;
+}
+
+// Persson-Peraire sensor per element; frac, eps0, feats may be NULL.
+int orc_pp_sensor(void* h, const double* u, double s0, double kappa, double ceps, double* eps,
+                  double* frac, double* eps0, float* feats) {
+  SensorParams sp{s0, kappa, ceps};
+  ppSensor(*static_cast<Solver*>(h), u, sp, eps, frac, eps0, feats);
+  return 0;
+}
+
+// Viscous operator L_v(u; eps) with penalty tau0 max(eps-, eps+).
+int orc_av_rhs(void* h, const double* u, const double* eps, double tau0, double* out) {
+  avRHS(*static_cast<Solver*>(h), u, eps, tau0, out);
+  return 0;
+}
+
+// SSP-RK3 of u_t + div(u^2/2 (1,1,1)) = div(eps grad u) with eps from the
+// Persson-Peraire sensor of u at the start of each step (frozen over the
+// step; mode 0 runs without viscosity). Per step
+//   dt = min(cfl hmin / max|u|, cfl_v hmin^2 / max eps)  (second term only if
+//   max eps > 0), clipped so that t does not pass t_end (t_end > 0), and
+// steps stop when t reaches t_end or after max_steps. Returns the number of
+// steps taken; dts[max_steps] (may be NULL) receives each dt, *t_out the
+// final time.
+int orc_ssprk3_av(void* h, double* u, int max_steps, double t_end, double cfl, double cfl_v,
+                  double hmin, int mode, double s0, double kappa, double ceps, double tau0,
+                  double* dts, double* t_out) {
+  Solver* S = static_cast<Solver*>(h);
+  const size_t n = (size_t)S->el.Np * S->m.K;
+  Vec u1(n), u2(n), L(n), Lv(n), eps(S->m.K, 0.0);
+  SensorParams sp{s0, kappa, ceps};
+  double t = 0.0;
+  int step = 0;
+  auto rhs = [&](const double* x, double* out) {
+    burgersRHS(*S, x, out);
+    if (mode != 0) {
+      avRHS(*S, x, eps.data(), tau0, Lv.data());
+      for (size_t q = 0; q < n; ++q) out[q] += Lv[q];
+    }
+  };
+  for (; step < max_steps; ++step) {
+    if (t_end > 0.0 && t >= t_end) break;
+    double emax = 0.0;
+    if (mode != 0) {
+      ppSensor(*S, u, sp, eps.data(), nullptr, nullptr, nullptr);
+      for (double e : eps) emax = std::max(emax, e);
+    }
+    double dt = cfl * hmin / maxAbs(u, n);
+    if (emax > 0.0) dt = std::min(dt, cfl_v * hmin * hmin / emax);
+    if (t_end > 0.0) dt = std::min(dt, t_end - t);
+    if (dts) dts[step] = dt;
+    rhs(u, L.data());
+    for (size_t q = 0; q < n; ++q) u1[q] = u[q] + dt * L[q];
+    rhs(u1.data(), L.data());
+    for (size_t q = 0; q < n; ++q) u2[q] = 0.75 * u[q] + 0.25 * (u1[q] + dt * L[q]);
+    rhs(u2.data(), L.data());
+    for (size_t q = 0; q < n; ++q) u[q] = (1.0 / 3.0) * u[q] + (2.0 / 3.0) * (u2[q] + dt * L[q]);
+    t += dt;
+  }
+  if (t_out) *t_out = t;
+  return step;
 }
 
 int orc_num_threads(void) {

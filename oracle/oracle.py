@@ -47,6 +47,13 @@ def load():
         lib.orc_aligned_offsets.restype = C.c_long
         lib.orc_face_connector_count.argtypes = [C.c_long, _I]
         lib.orc_face_connector.argtypes = [C.c_long, C.c_int, C.c_int, C.c_int, _I, _I, _L, _I, _I]
+        lib.orc_pp_sensor.argtypes = [C.c_void_p, _D, C.c_double, C.c_double, C.c_double, _D, _D,
+                                      _D, C.POINTER(C.c_float)]
+        lib.orc_av_rhs.argtypes = [C.c_void_p, _D, _D, C.c_double, _D]
+        lib.orc_ssprk3_av.argtypes = [C.c_void_p, _D, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, _D, _D]
+        lib.orc_ssprk3_av.restype = C.c_int
         lib.orc_num_threads.restype = C.c_int
         lib.orc_set_num_threads.argtypes = [C.c_int]
         _lib = lib
@@ -170,6 +177,35 @@ class Setup:
 
     def min_node_distance(self) -> float:
         return float(load().orc_min_node_distance(self.h))
+
+    def pp_sensor(self, u: np.ndarray, s0: float, kappa: float, ceps: float):
+        """Persson-Peraire sensor: (eps, frac, eps0) per element and the
+        normalised modal coefficients [K, Np] (float32)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        eps, frac, e0 = np.zeros(self.K), np.zeros(self.K), np.zeros(self.K)
+        feats = np.zeros((self.K, self.Np), dtype=np.float32)
+        load().orc_pp_sensor(self.h, _d(u), s0, kappa, ceps, _d(eps), _d(frac), _d(e0),
+                             feats.ctypes.data_as(C.POINTER(C.c_float)))
+        return eps, frac, e0, feats
+
+    def av_rhs(self, u: np.ndarray, eps: np.ndarray, tau0: float) -> np.ndarray:
+        """Viscous operator L_v(u; eps) (BR1 + jump penalty)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        out = np.zeros_like(u)
+        load().orc_av_rhs(self.h, _d(u), _d(eps), tau0, _d(out))
+        return out
+
+    def ssprk3_av(self, u: np.ndarray, t_end: float, cfl: float, cfl_v: float, hmin: float,
+                  s0: float, kappa: float, ceps: float, tau0: float, mode: int = 1,
+                  max_steps: int = 100000):
+        """SSP-RK3 with Persson-Peraire viscosity to t_end; (u, dts, t)."""
+        u = np.array(u, dtype=np.float64, copy=True, order="C")
+        dts = np.zeros(max_steps)
+        t = C.c_double()
+        n = load().orc_ssprk3_av(self.h, _d(u), max_steps, t_end, cfl, cfl_v, hmin, mode, s0,
+                                 kappa, ceps, tau0, _d(dts), C.byref(t))
+        return u, dts[:n], t.value
 
     def ssprk3(self, u: np.ndarray, nsteps: int, cfl: float = 0.3, hmin: float | None = None,
                fixed_dt: float = 0.0):
