@@ -304,6 +304,104 @@ def bench_c4_stage(a, dev) -> dict:
     return out
 
 
+def bench_c5(a, dev) -> dict:
+    """Learned artificial viscosity: 64x64x85 hexes x 6 = 2,088,960 periodic
+    tets (2^21 is not a multiple of 6), P=4, 2 slab partitions,
+    u0 = 0.5 + sin(2 pi x) sin(2 pi y) sin(2 pi z), SSP-RK3 past shock formation.
+
+    The sensor network is trained on the GPU (hand-written kernels) on rows
+    (normalised modal coefficients, Persson-Peraire fraction) from
+    Persson-Peraire runs of 32 seeded fields on a 16^3 x 6 mesh; then the
+    Persson-Peraire and learned-viscosity runs are timed on the full mesh and
+    judged against the exact entropy solution on a node sample."""
+    from dgkernel_amd import av as AV
+    from dgkernel_amd import dg
+    from dgkernel_amd import mesh as M
+    N = 4
+    out = {"metric": "learned vs Persson-Peraire artificial viscosity (P=4 shock-forming Burgers)"}
+    # 1. training data and training
+    t0 = time.perf_counter()
+    # training mesh with the target mesh's hex aspect ratio (nz / n)
+    mt = M.cube_mesh(a.train_n, a.train_n, max(2, round(a.train_n * a.nz / a.c5_n)), periodic=True)
+    stt = dg.make_setup(mt, N, Kp=[mt.K])
+    p = AV.AVParams.for_order(N)
+    X, T, nrows = AV.generate_training_data(dev, stt, p, seeds=range(1000, 1000 + a.train_seeds),
+                                            t_end=a.t_end, every=a.train_every)
+    t_gen = time.perf_counter() - t0
+    log(f"training data: {nrows} rows from {a.train_seeds} fields on {mt.K} tets, {t_gen:.1f} s")
+    net = AV.SensorNet(dev, stt.ref.Np, seed=1)
+    dev.sync()
+    t0 = time.perf_counter()
+    hist = net.train(X, T, nrows, steps=a.train_steps, batch=a.train_batch, lr=3e-3, lr_final=5e-5,
+                     report_every=max(1, a.train_steps // 5))
+    dev.sync()
+    t_train = time.perf_counter() - t0
+    log(f"training: {a.train_steps} Adam steps x {a.train_batch}, {t_train:.1f} s, loss {hist}")
+    out["training"] = {"rows": nrows, "fields": a.train_seeds, "mesh_tets": mt.K,
+                       "generate_s": t_gen, "steps": a.train_steps, "batch": a.train_batch,
+                       "train_s": t_train, "samples_per_s": a.train_steps * a.train_batch / t_train,
+                       "loss_history": hist}
+    del X, T
+    # 2. the full mesh
+    n = a.c5_n
+    m = M.cube_mesh(n, n, a.nz, periodic=True)
+    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, 2))
+    x, y, z = st.node_coords()
+    u0 = 0.5 + np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z)
+    out["config"] = {"workload": f"{n}x{n}x{a.nz}x6={m.K} periodic tets, P={N}, 2 slab "
+                                 f"partitions, SSP-RK3 to t={a.t_end}", "tets": m.K, "order": N,
+                     "Np": st.ref.Np, "t_end": a.t_end, "cfl": 0.3, "cfl_v": p.cfl_v,
+                     "ceps": p.ceps, "kappa": p.kappa, "s0": p.s0}
+    rng = np.random.default_rng(a.seed)
+    sample = rng.choice(x.size, size=min(a.exact_nodes, x.size), replace=False)
+    runs = {}
+    for sensor in ("pp", "mlp"):
+        s = AV.AVSolver(dev, st, p, sensor, net=net if sensor == "mlp" else None)
+        s.set_state(u0)
+        # sensor cost: one evaluation on the initial state, HIP events
+        ms_sensor = _time_ms(dev, s.sense, a.reps)
+        s.set_state(u0)
+        dev.sync()
+        tm = dev.timer()
+        tm.start()
+        last = [time.perf_counter()]
+
+        def progress(k, t, name=sensor):
+            if time.perf_counter() - last[0] > 20.0:
+                last[0] = time.perf_counter()
+                log(f"{name}: {k} steps, t = {t:.4f}")
+
+        steps = (s.run(a.t_end, chunk=64, progress=progress) if a.c5_steps == 0
+                 else (s.step(a.c5_steps, a.t_end) or a.c5_steps))
+        tm.stop()
+        ms = tm.ms()
+        u = s.state()
+        # sensor cost on the final (shocked) state
+        ms_sensor_end = _time_ms(dev, s.sense, a.reps)
+        runs[sensor] = {"u": u.reshape(-1)[sample], "u_full": u, "t": s.time()}
+        out[sensor] = {"run_ms": ms, "steps": steps, "t_final": s.time(),
+                       "sensor_ms_smooth": ms_sensor, "sensor_ms_shocked": ms_sensor_end,
+                       "finite": bool(np.isfinite(u).all())}
+        log(f"{sensor}: {steps} steps to t={s.time():.4f} in {ms / 1e3:.1f} s; sensor "
+            f"{ms_sensor:.4f} ms (smooth) / {ms_sensor_end:.4f} ms (shocked)")
+        del s
+    from oracle import av_oracle as A
+    ue = A.burgers_exact(lambda X_, Y_, Z_: 0.5 + np.sin(2 * np.pi * X_) * np.sin(2 * np.pi * Y_)
+                         * np.sin(2 * np.pi * Z_), x.reshape(-1)[sample], y.reshape(-1)[sample],
+                         z.reshape(-1)[sample], runs["pp"]["t"], umax=1.5)
+    for sensor in ("pp", "mlp"):
+        d = runs[sensor]["u"] - ue
+        out[sensor]["err_vs_exact"] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(ue)),
+                                       "nodes": int(sample.size)}
+    e_pp, e_ml = out["pp"]["err_vs_exact"]["rel_l2"], out["mlp"]["err_vs_exact"]["rel_l2"]
+    out["mlp_vs_pp"] = {"err_ratio": e_ml / e_pp,
+                        "state_rel_l2": _errs(runs["mlp"]["u_full"], runs["pp"]["u_full"])["rel_l2"],
+                        "sensor_speedup_smooth": out["pp"]["sensor_ms_smooth"] / out["mlp"]["sensor_ms_smooth"],
+                        "sensor_speedup_shocked": out["pp"]["sensor_ms_shocked"] / out["mlp"]["sensor_ms_shocked"]}
+    out["device"] = dev.name
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,7 +423,17 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--mode", choices=["stage", "deriv", "c4"], default="stage")
+    ap.add_argument("--mode", choices=["stage", "deriv", "c4", "c5"], default="stage")
+    ap.add_argument("--c5-n", type=int, default=64, help="hexes in x and y (c5)")
+    ap.add_argument("--nz", type=int, default=85, help="hexes in z (c5)")
+    ap.add_argument("--t-end", type=float, default=0.4, help="final time (c5)")
+    ap.add_argument("--c5-steps", type=int, default=0, help="c5: fixed step count (0: run to t-end)")
+    ap.add_argument("--train-n", type=int, default=16, help="c5: hexes per side of the training mesh")
+    ap.add_argument("--train-seeds", type=int, default=32)
+    ap.add_argument("--train-every", type=int, default=100)
+    ap.add_argument("--train-steps", type=int, default=60000)
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--exact-nodes", type=int, default=20000)
     ap.add_argument("--case", choices=["c2", "c4"], default="c4", help="derivative workload")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--precs", default="", help="derivative precisions to run (f32,bf16x3,...)")
@@ -354,6 +462,9 @@ def main(argv=None) -> int:
         return 0
     if a.mode == "c4":
         print(json.dumps(bench_c4_stage(a, _lib.Device())), flush=True)
+        return 0
+    if a.mode == "c5":
+        print(json.dumps(bench_c5(a, _lib.Device())), flush=True)
         return 0
 
     t_setup = time.perf_counter()

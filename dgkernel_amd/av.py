@@ -237,14 +237,19 @@ class AVSolver:
         for _ in range(n):
             self._enqueue_step(t_end)
 
-    def run(self, t_end: float, chunk: int = 32, max_steps: int = 1_000_000) -> int:
-        """Steps until t reaches t_end (the last dt is clipped); returns steps taken
-        (steps after t_end are no-ops with dt = 0 and are not counted)."""
+    def run(self, t_end: float, chunk: int = 32, max_steps: int = 1_000_000,
+            progress=None) -> int:
+        """Steps until t reaches t_end (the last dt is clipped); returns the
+        steps enqueued (whole chunks; steps after t_end are no-ops, dt = 0).
+        progress(steps, t) is called after every chunk if given."""
         steps = 0
         while steps < max_steps:
             self.step(chunk, t_end)
             steps += chunk
-            if self.time() >= t_end * (1 - 1e-15):
+            t = self.time()
+            if progress is not None:
+                progress(steps, t)
+            if t >= t_end * (1 - 1e-15):
                 break
         return steps
 

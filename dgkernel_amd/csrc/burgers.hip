@@ -657,12 +657,12 @@ __device__ __forceinline__ void pipe_tiles(const StageParams& p, const double* s
       for (int s = 0; s < KSL; ++s) uP[s] = p.u[in.nb + idx[s]];
     }
     __builtin_amdgcn_sched_barrier(0);
-    // Geometry of element c (clamped into the tile): folded metrics, and the
-    // flux factors of face gq.
+    // Geometry of element c (clamped into the tile): the folded metrics; the
+    // flux factors of face gq follow from them (face_factors).
     const int64_t g = (int64_t)dcur.gbase + (FULL ? c : min(c, cnt - 1));
-    const double fa = p.bgeo[(11 + gq) * K + g];
-    const double fb = p.bgeo[(15 + gq) * K + g];
     const double ga = p.bgeo[0 * K + g], gb = p.bgeo[1 * K + g], gc = p.bgeo[2 * K + g];
+    double fa, fb;
+    dgk::face_factors(gq, ga, gb, gc, fa, fb);
     // Next tile, in flight while this one computes.
     TileIn<PV> nxt;
     load_tile_in<N>(p, dnxt, lane, nxt);
@@ -765,7 +765,7 @@ struct DeepIn {
   double u[PipeShape<N>::PV];
   double uP[PipeShape<N>::KSL];
   double u0[MODE == 0 ? PipeShape<N>::PV : 1];
-  double ga, gb, gc, fa, fb;
+  double ga, gb, gc;
 };
 
 template <int N, int MODE>
@@ -786,8 +786,6 @@ __device__ __forceinline__ void deep_issue(const StageParams& p, const Tile& d, 
   for (int s = 0; s < KSL; ++s) x.uP[s] = p.u[nb + idx[s]];
   const int64_t K = p.K;
   const int64_t g = (int64_t)d.gbase + min(c, d.count - 1);
-  x.fa = p.bgeo[(11 + gq) * K + g];
-  x.fb = p.bgeo[(15 + gq) * K + g];
   x.ga = p.bgeo[0 * K + g];
   x.gb = p.bgeo[1 * K + g];
   x.gc = p.bgeo[2 * K + g];
@@ -878,12 +876,14 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
     double4_t accL[MTL];
 #pragma unroll
     for (int tt = 0; tt < MTL; ++tt) accL[tt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    double fa, fb;
+    dgk::face_factors(gq, cur.ga, cur.gb, cur.gc, fa, fb);
 #pragma unroll
     for (int s = 0; s < KSL; ++s) {
       const int nm = (own[s / 5] >> (6 * (s % 5))) & 63;
       const double uM = su[nm], up = cur.uP[s];
       const double al = fmax(fabs(uM), fabs(up));
-      const double gm = (uM - up) * fma(cur.fa, uM + up, -cur.fb * al);
+      const double gm = (uM - up) * fma(fa, uM + up, -fb * al);
 #pragma unroll
       for (int tt = 0; tt < MTL; ++tt)
         accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,

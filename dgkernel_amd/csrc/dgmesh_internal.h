@@ -43,6 +43,19 @@ struct Tile {
   int32_t count;  // elements in the tile
 };
 
+// Flux factors of face f, A = Fscale n.(1,1,1)/4 and B = Fscale |n.(1,1,1)|/2,
+// from the folded metrics alone: for an affine element Fscale_f n_f is the
+// unnormalised face normal, -grad t, -grad s, grad(r+s+t), -grad r for
+// faces 0..3 (the Normals3D construction, element/library/gonudg/
+// normals3d.go:82-185), so Fscale_f n_f.(1,1,1) is -gc, -gb, ga+gb+gc, -ga
+// with ga = rx+ry+rz, gb = sx+sy+sz, gc = tx+ty+tz.
+__device__ inline void face_factors(int f, double ga, double gb, double gc, double& A,
+                                             double& B) {
+  const double sn = f == 0 ? -gc : f == 1 ? -gb : f == 2 ? ga + gb + gc : -ga;
+  A = 0.25 * sn;
+  B = 0.5 * fabs(sn);
+}
+
 constexpr int kGeoRows = 26;   // rx..tz, J, 4 x (nx, ny, nz, Fscale)
 // rx+ry+rz, sx+sy+sz, tx+ty+tz, 4 x n.(1,1,1), 4 x Fscale,
 // 4 x Fscale n.(1,1,1)/4, 4 x Fscale |n.(1,1,1)|/2

@@ -25,8 +25,8 @@
 // tile's nodal values are staged through LDS with unit-stride loads, and the
 // element operators are read with wave-uniform addresses (scalar loads,
 // SGPR operands of v_fma_f64). The learned sensor works on 16-element tiles
-// per wave on matrix cores: modal transform on FP32 MFMA (16x16x4), both
-// hidden layers on BF16 MFMA (16x16x32) with FP32 accumulation. Hidden
+// per wave on matrix cores: modal transform on split BF16 MFMA (three
+// products), both hidden layers on BF16 MFMA (16x16x32), FP32 accumulation. Hidden
 // activations never leave registers between layers: the k order of each
 // layer's product is permuted so that the FP32 result layout of one layer
 // (lane (c, g) holds rows 4g + r of each 16-row tile) is directly the BF16
@@ -221,162 +221,328 @@ __global__ void __launch_bounds__(64) pp_sensor_kernel(AvArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Gradient q = grad u + LIFT(Fscale n (u+ - u-)/2), FP64, one element per lane.
+// Matrix-core kernels of the viscous stage (FP64, v_mfma_f64_16x16x4_f64).
+// Each wave takes 16-element tiles (the MFMA N dimension); the operator
+// fragments sit in LDS for the block. FP64 MFMA layout (checked on gfx950):
+// lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[l/16 + 4r][l%16], r = 0..3, so with blocks stacked at multiples of 4
+// rows the lane (c, g) holding row i = g + 4q of one block holds row i of
+// every block, for element c.
+//
+// av_grad_mfma_kernel: q_d = grad_d u + sum_f (Fscale_f n_d,f / 2) T_f with
+//   [Dr; Ds; Dt] u (stacked rows, K = Np) and T_f = LIFT_f (u+ - u-) per face
+//   (face-contiguous k with each face padded to a multiple of 4 points, one
+//   accumulator set per face).
+// av_stage_mfma_kernel: one volume product [Dr | Ds | Dt] [S_r; S_s; S_t]
+//   with S_b = eps (b_x q_x + b_y q_y + b_z q_z) - g_b u^2/2 (b = r, s, t;
+//   g_b = b_x + b_y + b_z: the Burgers flux u^2/2 (1,1,1)), and one LIFT
+//   product over face-per-lane k (k = 4 fp + f) of the Lax-Friedrichs jump
+//   plus the viscous flux jump; RK combination in the epilogue.
 // ---------------------------------------------------------------------------
+typedef double double4v __attribute__((ext_vector_type(4)));
+
 template <int N>
-__global__ void __launch_bounds__(64) av_grad_kernel(AvArgs a) {
-  constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP, S = NP + 1;
+struct AvShape {
+  static constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP;
+  static constexpr int NPR = (NP + 3) / 4 * 4;      // block row/column offset
+  static constexpr int QN = NPR / 4;                // rows per lane
+  static constexpr int MTG = (3 * NPR + 15) / 16;   // row tiles of [Dr; Ds; Dt]
+  static constexpr int KSG = NPR / 4;               // k steps over Np
+  static constexpr int MTL = (NP + 15) / 16;        // row tiles of Np rows
+  static constexpr int SPF = (NFP + 3) / 4;         // k steps per (padded) face
+  static constexpr int KSD = 3 * NPR / 4;           // k steps of [Dr | Ds | Dt]
+  static constexpr int KSL = NFP;                   // k steps, face-per-lane LIFT
+  static constexpr int S = NP + 1;                  // LDS row stride
+  static constexpr int NA_G = (MTG * KSG + MTL * 4 * SPF) * 64;
+  static constexpr int NA_S = (MTL * KSD + MTL * KSL) * 64;
+};
+
+// Own volume node of face point fp of face f, fp = f4 + g (clamped).
+template <int N>
+__device__ __forceinline__ int face_node(int f, int f4, int g) {
   constexpr FmaskTable<N> FM{};
-  __shared__ double sU[64 * S];
-  __shared__ double sO[3][64 * S];
+  constexpr int NFP = Tet<N>::Nfp;
+  const int n0 = FM.v[f][min(f4 + 0, NFP - 1)], n1 = FM.v[f][min(f4 + 1, NFP - 1)];
+  const int n2 = FM.v[f][min(f4 + 2, NFP - 1)], n3 = FM.v[f][min(f4 + 3, NFP - 1)];
+  return g == 0 ? n0 : g == 1 ? n1 : g == 2 ? n2 : n3;
+}
+
+template <int N, int WPB>
+__global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1) av_grad_mfma_kernel(AvArgs a) {
+  using SH = AvShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, NPR = SH::NPR, QN = SH::QN;
+  constexpr int MTG = SH::MTG, KSG = SH::KSG, MTL = SH::MTL, SPF = SH::SPF, S = SH::S;
+  constexpr int NT = 64 * WPB;
+  __shared__ __attribute__((aligned(16))) double sA[SH::NA_G];
+  __shared__ __attribute__((aligned(16))) double sUall[WPB][16 * S];
   __shared__ uint8_t sFN[4 * 24 * NFP];
-  const int lane = threadIdx.x;
-  for (int i = lane; i < 4 * 24 * NFP; i += 64) sFN[i] = a.face_nodes[i];
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) sFN[i] = a.face_nodes[i];
+  for (int q = threadIdx.x; q < SH::NA_G; q += NT) {
+    const int l = q & 63, ts = q >> 6, cl = l & 15, gl = l >> 4;
+    double v = 0.0;
+    if (ts < MTG * KSG) {
+      const int tt = ts / KSG, s = ts % KSG;
+      const int row = 16 * tt + cl, k = 4 * s + gl, blk = row / NPR, i = row % NPR;
+      if (k < NP && blk < 3 && i < NP) v = (blk == 0 ? a.Dr : blk == 1 ? a.Ds : a.Dt)[i * NP + k];
+    } else {
+      const int r2 = ts - MTG * KSG, tt = r2 / (4 * SPF), s = r2 % (4 * SPF);
+      const int row = 16 * tt + cl, f = s / SPF, fp = 4 * (s % SPF) + gl;
+      if (row < NP && fp < NFP) v = a.LIFT[row * NF + f * NFP + fp];
+    }
+    sA[q] = v;
+  }
+  __syncthreads();
+  const double* sAv = sA;
+  const double* sAl = sA + MTG * KSG * 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  double* sU = sUall[w];
   const int64_t K = a.K;
-  for (int grp = blockIdx.x; 4 * grp < a.ntiles; grp += gridDim.x) {
-    const Tile my = group_tile(a.tiles, a.ntiles, grp, lane);
+  const int nw = gridDim.x * WPB;
+  for (int t = blockIdx.x * WPB + w; t < a.ntiles; t += nw) {
+    const Tile tl = a.tiles[t];
+    const int cnt = tl.count;
     wave_fence();
-    stage4<NP, S>(a.u, my, lane, sU);
-    wave_fence();
-    if ((lane & 15) < my.count) {
-      const double* su = sU + lane * S;
-      const int64_t g = (int64_t)my.gbase + (lane & 15);
-      double mt[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + g];
-      double nf[4][3];
-      double jmp[4][NFP];
-      const int packed = a.codes[g];
-#pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        const double fs = a.geo[(13 + 4 * f) * K + g];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) nf[f][d] = a.geo[(10 + 4 * f + d) * K + g];
-        const int code = (packed >> (5 * f)) & 31;
-        const int32_t nb = a.nbr[f * K + g];
-#pragma unroll
-        for (int fp = 0; fp < NFP; ++fp) {
-          const double uM = su[FM.v[f][fp]];
-          const double uP = a.u[nb + sFN[(f * 24 + code) * NFP + fp]];
-          jmp[f][fp] = 0.5 * fs * (uP - uM);
-        }
-      }
-      const int64_t z = (int64_t)(__builtin_amdgcn_readfirstlane(grp) >> 30);
-      const cdouble_ptr Dr = (cdouble_ptr)(a.Dr + z), Ds = (cdouble_ptr)(a.Ds + z),
-                        Dt = (cdouble_ptr)(a.Dt + z), LF = (cdouble_ptr)(a.LIFT + z);
-#pragma unroll 1
-      for (int i = 0; i < NP; ++i) {
-        double ur = 0.0, us = 0.0, ut = 0.0;
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-          const double uj = su[j];
-          ur = fma(Dr[i * NP + j], uj, ur);
-          us = fma(Ds[i * NP + j], uj, us);
-          ut = fma(Dt[i * NP + j], uj, ut);
-        }
-        double T[4];
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          double s = 0.0;
-#pragma unroll
-          for (int fp = 0; fp < NFP; ++fp) s = fma(LF[i * NF + f * NFP + fp], jmp[f][fp], s);
-          T[f] = s;
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          double v = mt[d] * ur + mt[3 + d] * us + mt[6 + d] * ut;
-#pragma unroll
-          for (int f = 0; f < 4; ++f) v = fma(nf[f][d], T[f], v);
-          sO[d][lane * S + i] = v;
-        }
-      }
+    for (int i = lane; i < cnt * NP; i += 64) {
+      const int e = i / NP, n = i - e * NP;
+      sU[e * S + n] = a.u[tl.vbase + i];
     }
     wave_fence();
+    const int ec = min(c, cnt - 1);
+    const int64_t ge = (int64_t)tl.gbase + ec;
+    const double* su = sU + ec * S;
+    const int packed = a.codes[ge];
+    int32_t nb[4];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) unstage4<NP, S, false>(a.qo[d], my, lane, sO[d]);
+    for (int f = 0; f < 4; ++f) nb[f] = a.nbr[f * K + ge];
+    double4v accV[MTG];
+#pragma unroll
+    for (int tt = 0; tt < MTG; ++tt) accV[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KSG; ++s) {
+      const int k = 4 * s + g;
+      const double b = k < NP ? su[k] : 0.0;
+#pragma unroll
+      for (int tt = 0; tt < MTG; ++tt)
+        accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAv[(tt * KSG + s) * 64 + lane], b,
+                                                         accV[tt], 0, 0, 0);
+    }
+    double4v accF[4][MTL];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int code = (packed >> (5 * f)) & 31;
+#pragma unroll
+      for (int tt = 0; tt < MTL; ++tt) accF[f][tt] = (double4v){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int sp = 0; sp < SPF; ++sp) {
+        const int fp = 4 * sp + g;
+        const int fpc = min(fp, NFP - 1);
+        const double uP = a.u[nb[f] + sFN[(f * 24 + code) * NFP + fpc]];
+        const double uM = su[face_node<N>(f, 4 * sp, g)];
+        const double b = fp < NFP ? uP - uM : 0.0;
+#pragma unroll
+        for (int tt = 0; tt < MTL; ++tt)
+          accF[f][tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              sAl[(tt * 4 * SPF + f * SPF + sp) * 64 + lane], b, accF[f][tt], 0, 0, 0);
+      }
+    }
+    double mt[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + ge];
+    double cf[4][3];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const double hfs = 0.5 * a.geo[(13 + 4 * f) * K + ge];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) cf[f][d] = hfs * a.geo[(10 + 4 * f + d) * K + ge];
+    }
+    double qv[3][QN];
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const double ur = accV[(4 * q) / 16][((4 * q) % 16) / 4];
+      const double us = accV[(NPR + 4 * q) / 16][((NPR + 4 * q) % 16) / 4];
+      const double ut = accV[(2 * NPR + 4 * q) / 16][((2 * NPR + 4 * q) % 16) / 4];
+      double T[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f)
+        T[f] = (4 * q) / 16 < MTL ? accF[f][((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double v = mt[d] * ur + mt[3 + d] * us + mt[6 + d] * ut;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) v = fma(cf[f][d], T[f], v);
+        qv[d][q] = v;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      wave_fence();
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const int i = g + 4 * q;
+        if (i < NP && c < cnt) sU[c * S + i] = qv[d][q];
+      }
+      wave_fence();
+      for (int i = lane; i < cnt * NP; i += 64) {
+        const int e = i / NP, n = i - e * NP;
+        a.qo[d][tl.vbase + i] = sU[e * S + n];
+      }
+    }
   }
 }
 
-// ---------------------------------------------------------------------------
-// rhs += L_v = div(eps q) - LIFT(Fscale (n.(s- - s+)/2 + tau (u- - u+))),
-// FP64, one element per lane.
-// ---------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(64) av_visc_kernel(AvArgs a) {
-  constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP, S = NP + 1;
-  constexpr FmaskTable<N> FM{};
-  __shared__ double sU[64 * S];
-  __shared__ double sQ[3][64 * S];
+// MODE 0: out = a0 u0 + a1 (u + dt R); 1: out = a1 (u + dt R); 2: out = R,
+// with R = L_v(u) + (BURG ? L(u) : 0).
+struct StageAv {
+  const double* u0;
+  double* out;
+  double a0, a1;
+  const double* dt;
+  double* maxabs;
+  const double* bgeo;  // [19][K]
+};
+
+template <int N, int WPB, int MODE, bool BURG>
+__global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
+    av_stage_mfma_kernel(AvArgs a, StageAv o) {
+  using SH = AvShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, NPR = SH::NPR, QN = SH::QN;
+  constexpr int MTL = SH::MTL, KSD = SH::KSD, KSL = SH::KSL, S = SH::S;
+  constexpr int NT = 64 * WPB;
+  __shared__ __attribute__((aligned(16))) double sA[SH::NA_S];
+  __shared__ __attribute__((aligned(16))) double sUall[WPB][16 * S];
   __shared__ uint8_t sFN[4 * 24 * NFP];
-  const int lane = threadIdx.x;
-  for (int i = lane; i < 4 * 24 * NFP; i += 64) sFN[i] = a.face_nodes[i];
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) sFN[i] = a.face_nodes[i];
+  for (int q = threadIdx.x; q < SH::NA_S; q += NT) {
+    const int l = q & 63, ts = q >> 6, cl = l & 15, gl = l >> 4;
+    double v = 0.0;
+    if (ts < MTL * KSD) {
+      const int tt = ts / KSD, s = ts % KSD;
+      const int row = 16 * tt + cl, k = 4 * s + gl, blk = k / NPR, j = k % NPR;
+      if (row < NP && j < NP) v = (blk == 0 ? a.Dr : blk == 1 ? a.Ds : a.Dt)[row * NP + j];
+    } else {
+      const int r2 = ts - MTL * KSD, tt = r2 / KSL, s = r2 % KSL;
+      const int row = 16 * tt + cl;
+      if (row < NP) v = a.LIFT[row * NF + gl * NFP + s];
+    }
+    sA[q] = v;
+  }
+  __syncthreads();
+  const double* sAd = sA;
+  const double* sAl = sA + MTL * KSD * 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  double* sU = sUall[w];
   const int64_t K = a.K;
-  for (int grp = blockIdx.x; 4 * grp < a.ntiles; grp += gridDim.x) {
-    const Tile my = group_tile(a.tiles, a.ntiles, grp, lane);
+  const double dt = MODE == 2 ? 0.0 : *o.dt;
+  double lmax = 0.0;
+  const int nw = gridDim.x * WPB;
+  for (int t = blockIdx.x * WPB + w; t < a.ntiles; t += nw) {
+    const Tile tl = a.tiles[t];
+    const int cnt = tl.count;
     wave_fence();
-    stage4<NP, S>(a.u, my, lane, sU);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) stage4<NP, S>(a.q[d], my, lane, sQ[d]);
-    wave_fence();
-    if ((lane & 15) < my.count) {
-      const int64_t g = (int64_t)my.gbase + (lane & 15);
-      const double ee = a.eps_in[g];
-      double mt[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + g];
-      double* qx = sQ[0] + lane * S;
-      double* qy = sQ[1] + lane * S;
-      double* qz = sQ[2] + lane * S;
-      double* su = sU + lane * S;
-      double fl[4][NFP];
-      const int packed = a.codes[g];
-#pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        const double fs = a.geo[(13 + 4 * f) * K + g];
-        const double nx = a.geo[(10 + 4 * f) * K + g], ny = a.geo[(11 + 4 * f) * K + g],
-                     nz = a.geo[(12 + 4 * f) * K + g];
-        const int code = (packed >> (5 * f)) & 31;
-        const int32_t nb = a.nbr[f * K + g];
-        const double ep = a.eps_in[a.nbe[f * K + g]];
-        const double tau = a.tau0 * fmax(ee, ep);
-#pragma unroll
-        for (int fp = 0; fp < NFP; ++fp) {
-          const int nm = FM.v[f][fp];
-          const int32_t ip = nb + sFN[(f * 24 + code) * NFP + fp];
-          const double sM = ee * (nx * qx[nm] + ny * qy[nm] + nz * qz[nm]);
-          const double sP = ep * (nx * a.q[0][ip] + ny * a.q[1][ip] + nz * a.q[2][ip]);
-          fl[f][fp] = -fs * (0.5 * (sM - sP) + tau * (su[nm] - a.u[ip]));
-        }
-      }
-      // Contravariant fluxes eps (r_x q_x + r_y q_y + r_z q_z), ..., in place.
-#pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        const double x = qx[j], y = qy[j], zz = qz[j];
-        qx[j] = ee * (mt[0] * x + mt[1] * y + mt[2] * zz);
-        qy[j] = ee * (mt[3] * x + mt[4] * y + mt[5] * zz);
-        qz[j] = ee * (mt[6] * x + mt[7] * y + mt[8] * zz);
-      }
-      const int64_t z = (int64_t)(__builtin_amdgcn_readfirstlane(grp) >> 30);
-      const cdouble_ptr Dr = (cdouble_ptr)(a.Dr + z), Ds = (cdouble_ptr)(a.Ds + z),
-                        Dt = (cdouble_ptr)(a.Dt + z), LF = (cdouble_ptr)(a.LIFT + z);
-#pragma unroll 1
-      for (int i = 0; i < NP; ++i) {
-        double v = 0.0;
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-          v = fma(Dr[i * NP + j], qx[j], v);
-          v = fma(Ds[i * NP + j], qy[j], v);
-          v = fma(Dt[i * NP + j], qz[j], v);
-        }
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-#pragma unroll
-          for (int fp = 0; fp < NFP; ++fp) v = fma(LF[i * NF + f * NFP + fp], fl[f][fp], v);
-        su[i] = v;  // this lane's row of u is no longer read
-      }
+    for (int i = lane; i < cnt * NP; i += 64) {
+      const int e = i / NP, n = i - e * NP;
+      sU[e * S + n] = a.u[tl.vbase + i];
     }
     wave_fence();
-    unstage4<NP, S, true>(a.rhs, my, lane, sU);
+    const int ec = min(c, cnt - 1);
+    const int64_t ge = (int64_t)tl.gbase + ec;
+    const int64_t own = tl.vbase + (int64_t)ec * NP;  // this element's node 0 in state arrays
+    const double* su = sU + ec * S;
+    const double ee = a.eps_in[ge];
+    // Surface: face g of element c, point s per k step.
+    double4v accL[MTL];
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt) accL[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
+    {
+      const double fs = a.geo[(13 + 4 * g) * K + ge];
+      const double nx = a.geo[(10 + 4 * g) * K + ge], ny = a.geo[(11 + 4 * g) * K + ge],
+                   nz = a.geo[(12 + 4 * g) * K + ge];
+      const int code = (a.codes[ge] >> (5 * g)) & 31;
+      const int32_t nb = a.nbr[g * K + ge];
+      const double ep = a.eps_in[a.nbe[g * K + ge]];
+      const double tau = a.tau0 * fmax(ee, ep);
+      double fa = 0.0, fb = 0.0;
+      if (BURG)
+        dgk::face_factors(g, o.bgeo[0 * K + ge], o.bgeo[1 * K + ge], o.bgeo[2 * K + ge], fa, fb);
+      const uint8_t* fnt = sFN + (g * 24 + code) * NFP;
+#pragma unroll
+      for (int s = 0; s < KSL; ++s) {
+        const int nm = face_node<N>(g, 4 * (s / 4), s % 4);  // own node of point s of face g
+        const int32_t ip = nb + fnt[s];
+        const double uM = su[nm], uP = a.u[ip];
+        const double sM = ee * (nx * a.q[0][own + nm] + ny * a.q[1][own + nm] + nz * a.q[2][own + nm]);
+        const double sP = ep * (nx * a.q[0][ip] + ny * a.q[1][ip] + nz * a.q[2][ip]);
+        double b = -fs * (0.5 * (sM - sP) + tau * (uM - uP));
+        if (BURG) b = fma(uM - uP, fma(fa, uM + uP, -fb * fmax(fabs(uM), fabs(uP))), b);
+#pragma unroll
+        for (int tt = 0; tt < MTL; ++tt)
+          accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], b,
+                                                           accL[tt], 0, 0, 0);
+      }
+    }
+    // Volume: [Dr | Ds | Dt] [S_r; S_s; S_t].
+    double4v accD[MTL];
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt) accD[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
+    {
+      double mt[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) mt[q] = ee * a.geo[q * K + ge];
+      double gb3[3] = {0.0, 0.0, 0.0};
+      if (BURG) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gb3[d] = 0.5 * o.bgeo[d * K + ge];
+      }
+#pragma unroll
+      for (int s = 0; s < KSD; ++s) {
+        const int blk = (4 * s) / NPR;
+        const int j = 4 * s - blk * NPR + g;
+        double b = 0.0;
+        if (j < NP) {
+          const double qx = a.q[0][own + j], qy = a.q[1][own + j], qz = a.q[2][own + j];
+          b = mt[3 * blk] * qx + mt[3 * blk + 1] * qy + mt[3 * blk + 2] * qz;
+          if (BURG) {
+            const double uj = su[j];
+            b = fma(-gb3[blk] * uj, uj, b);
+          }
+        }
+#pragma unroll
+        for (int tt = 0; tt < MTL; ++tt)
+          accD[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAd[(tt * KSD + s) * 64 + lane], b,
+                                                           accD[tt], 0, 0, 0);
+      }
+    }
+    double vout[QN];
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int i = g + 4 * q;
+      const double r = (4 * q) / 16 < MTL ? accD[((4 * q) / 16) % MTL][((4 * q) % 16) / 4] +
+                                                accL[((4 * q) / 16) % MTL][((4 * q) % 16) / 4]
+                                          : 0.0;
+      vout[q] = MODE == 2 ? r : fma(dt, r, su[i < NP ? i : NP - 1]);
+    }
+    wave_fence();
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int i = g + 4 * q;
+      if (i < NP && c < cnt) sU[c * S + i] = vout[q];
+    }
+    wave_fence();
+    for (int i = lane; i < cnt * NP; i += 64) {
+      const int e = i / NP, n = i - e * NP;
+      double v = sU[e * S + n];
+      if (MODE == 0) v = fma(o.a0, o.u0[tl.vbase + i], o.a1 * v);
+      if (MODE == 1) v = o.a1 * v;
+      o.out[tl.vbase + i] = v;
+      lmax = fmax(lmax, fabs(v));
+    }
+  }
+  if (MODE != 2 && o.maxabs) {
+    lmax = wmax(lmax);
+    if (lane == 0 && lmax > 0.0)
+      atomicMax((unsigned long long*)o.maxabs, (unsigned long long)__double_as_longlong(lmax));
   }
 }
 
@@ -461,6 +627,12 @@ __device__ __forceinline__ void mlp_load(const float* __restrict__ P, const MlpL
   R.b3 = P[L.ob3];
 }
 
+// tanh(x) = 1 - 2 / (1 + e^(2x)) on the hardware exponential and
+// reciprocal: ~1e-7 relative, far below the BF16 rounding of this path.
+__device__ __forceinline__ float tanh_fast(float x) {
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * x));
+}
+
 // Forward pass of 16 inputs (columns); X[s] is lane (c, g)'s BF16 slice of
 // input c in k step s. Returns y for column c (all four lane groups).
 __device__ __forceinline__ float mlp_forward_tile(const MlpRegs& R, const bf16x8 (&X)[2]) {
@@ -477,7 +649,7 @@ __device__ __forceinline__ float mlp_forward_tile(const MlpRegs& R, const bf16x8
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int mt = 2 * s + (j >> 2), r = j & 3;
-      H[s][j] = (__bf16)tanhf(z[mt][r] + R.b1[mt][r]);
+      H[s][j] = (__bf16)tanh_fast(z[mt][r] + R.b1[mt][r]);
     }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
@@ -489,10 +661,10 @@ __device__ __forceinline__ float mlp_forward_tile(const MlpRegs& R, const bf16x8
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) part = fmaf(R.w3[mt][r], tanhf(z[mt][r] + R.b2[mt][r]), part);
+    for (int r = 0; r < 4; ++r) part = fmaf(R.w3[mt][r], tanh_fast(z[mt][r] + R.b2[mt][r]), part);
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
-  return 1.f / (1.f + expf(-(part + R.b3)));
+  return __builtin_amdgcn_rcpf(1.f + __expf(-(part + R.b3)));
 }
 
 // Network on given features x[n][nin] (FP32) -> y[n]; 16 inputs per wave step.
@@ -521,12 +693,19 @@ __global__ void __launch_bounds__(256) mlp_features_kernel(const float* __restri
   }
 }
 
-// Learned sensor on the state: per 16-element tile, uh = V^-1 u on FP32
-// MFMA, x = uh / |uh|, y = network(x) on BF16 MFMA, eps = eps0 y.
+// Two-part BF16 split of an FP32 value: x ~ hi + lo, ~16 significant bits.
+__device__ __forceinline__ __bf16 bf16_hi(float x) { return (__bf16)x; }
+__device__ __forceinline__ __bf16 bf16_lo(float x) { return (__bf16)(x - (float)(__bf16)x); }
+
+// Learned sensor on the state: per 16-element tile, uh = V^-1 u on BF16
+// matrix cores in the split form hi*hi + hi*lo + lo*hi (Henry, Tang &
+// Heinecke, ARITH 2019; ~16 significant bits, ample for an energy fraction
+// resolved to a few digits), x = uh / |uh|, y = network(x) on BF16 MFMA,
+// eps = eps0 y.
 template <int N>
 __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* __restrict__ P) {
   constexpr int NP = Tet<N>::Np, S = NP + 1;
-  constexpr int MTM = (NP + 15) / 16, KSM = (NP + 3) / 4;
+  constexpr int MTM = (NP + 15) / 16, KSM = (NP + 31) / 32;
   static_assert(NP <= 48, "modal rows must fit three 16-row tiles");
   __shared__ double sUall[4][16 * S];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -535,15 +714,20 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
   const MlpLayout L = mlp_layout(NP);
   MlpRegs R;
   mlp_load(P, L, lane, R);
-  // V^-1 fragments (FP32 A operand of 16x16x4): A[16 mt + row][4 s + g].
-  float vf[MTM][KSM];
+  // V^-1 fragments (BF16 A operand of 16x16x32, hi and lo parts):
+  // A[16 mt + row][32 s + 8 g + j].
+  bf16x8 vh[MTM][KSM], vl[MTM][KSM];
 #pragma unroll
   for (int mt = 0; mt < MTM; ++mt)
 #pragma unroll
-    for (int s = 0; s < KSM; ++s) {
-      const int i = 16 * mt + c, k = 4 * s + g;
-      vf[mt][s] = (i < NP && k < NP) ? (float)a.Vinv[i * NP + k] : 0.f;
-    }
+    for (int s = 0; s < KSM; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int i = 16 * mt + c, k = 32 * s + 8 * g + j;
+        const float v = (i < NP && k < NP) ? (float)a.Vinv[i * NP + k] : 0.f;
+        vh[mt][s][j] = bf16_hi(v);
+        vl[mt][s][j] = bf16_lo(v);
+      }
   double emax = 0.0;
   const int nw = gridDim.x * 4;
   for (int t = blockIdx.x * 4 + w; t < a.ntiles; t += nw) {
@@ -561,12 +745,21 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
     double umax = 0.0;
 #pragma unroll
     for (int s = 0; s < KSM; ++s) {
-      const int k = 4 * s + g;
-      const double uv = k < NP ? sU[cc * S + k] : 0.0;
-      umax = fmax(umax, fabs(uv));
+      bf16x8 bh, bl;
 #pragma unroll
-      for (int mt = 0; mt < MTM; ++mt)
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(vf[mt][s], (float)uv, acc[mt], 0, 0, 0);
+      for (int j = 0; j < 8; ++j) {
+        const int k = 32 * s + 8 * g + j;
+        const double uv = k < NP ? sU[cc * S + k] : 0.0;
+        umax = fmax(umax, fabs(uv));
+        bh[j] = bf16_hi((float)uv);
+        bl[j] = bf16_lo((float)uv);
+      }
+#pragma unroll
+      for (int mt = 0; mt < MTM; ++mt) {
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[mt][s], bh, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt][s], bl, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt][s], bh, acc[mt], 0, 0, 0);
+      }
     }
     // lane (c, g) holds uh rows 16 mt + 4 g + r of element c
     float tot = 0.f;
@@ -793,19 +986,77 @@ int groups_grid(dgk_dgmesh m, int per_cu) {
       return DGK_ERR_UNSUPPORTED;                                                    \
   }
 
-// rhs += L_v(u; eps) (gradient kernel into a->q, then the viscous kernel).
-int av_viscous(dgk_av a, const double* u, double* rhs) {
+// Waves per block: at N = 4 the operator fragments take 57-65 KiB of LDS, so
+// one block of 8 waves (two per SIMD) shares them; lower orders use 4.
+template <int N>
+constexpr int av_wpb() { return N >= 4 ? 8 : 4; }
+
+int av_blocks(dgk_dgmesh m) {
+  const int wpb = m->N >= 4 ? 8 : 4;
+  return std::max(1, std::min((m->ntiles64 + wpb - 1) / wpb, m->dev->cu_count * (m->N >= 4 ? 1 : 2)));
+}
+
+// q = grad u + LIFT(...) into a->q.
+int av_grad(dgk_av a, const double* u) {
   dgk_dgmesh m = a->m;
   if (m->ntiles64 == 0) return DGK_OK;
   AvArgs r = av_args(a);
   r.u = u;
-  r.rhs = rhs;
-  const dim3 grid(groups_grid(m, 8)), block(64);
-  DGK_AV_LAUNCH(av_grad_kernel, grid, block, 0, r);
-  DGK_LAUNCH_CHECK();
-  DGK_AV_LAUNCH(av_visc_kernel, grid, block, 0, r);
+  const dim3 grid(av_blocks(m));
+  switch (m->N) {
+    case 1: hipLaunchKernelGGL((av_grad_mfma_kernel<1, av_wpb<1>()>), grid, dim3(64 * av_wpb<1>()), 0, m->dev->stream, r); break;
+    case 2: hipLaunchKernelGGL((av_grad_mfma_kernel<2, av_wpb<2>()>), grid, dim3(64 * av_wpb<2>()), 0, m->dev->stream, r); break;
+    case 3: hipLaunchKernelGGL((av_grad_mfma_kernel<3, av_wpb<3>()>), grid, dim3(64 * av_wpb<3>()), 0, m->dev->stream, r); break;
+    case 4: hipLaunchKernelGGL((av_grad_mfma_kernel<4, av_wpb<4>()>), grid, dim3(64 * av_wpb<4>()), 0, m->dev->stream, r); break;
+    default:
+      dgk::set_error("shock capturing is built for N <= 4 (N=%d)", m->N);
+      return DGK_ERR_UNSUPPORTED;
+  }
   DGK_LAUNCH_CHECK();
   return DGK_OK;
+}
+
+template <int N, int MODE, bool BURG>
+int av_stage_launch(dgk_av a, const AvArgs& r, const StageAv& o) {
+  dgk_dgmesh m = a->m;
+  hipLaunchKernelGGL((av_stage_mfma_kernel<N, av_wpb<N>(), MODE, BURG>), dim3(av_blocks(m)),
+                     dim3(64 * av_wpb<N>()), 0, m->dev->stream, r, o);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+template <int N>
+int av_stage_n(dgk_av a, const AvArgs& r, const StageAv& o, int mode, bool burg) {
+  if (mode == 2) return burg ? av_stage_launch<N, 2, true>(a, r, o) : av_stage_launch<N, 2, false>(a, r, o);
+  if (mode == 0) return av_stage_launch<N, 0, true>(a, r, o);
+  return av_stage_launch<N, 1, true>(a, r, o);
+}
+
+// grad, then the fused kernel: mode 0/1 stage (a0 != 0 / a0 == 0), mode 2 rhs.
+int av_run(dgk_av a, const double* u, const double* u0, double* out, double a0, double a1,
+           const double* dt, double* maxabs, int mode, bool burg) {
+  dgk_dgmesh m = a->m;
+  if (m->ntiles64 == 0) return DGK_OK;
+  int rc = av_grad(a, u);
+  if (rc != DGK_OK) return rc;
+  AvArgs r = av_args(a);
+  r.u = u;
+  StageAv o;
+  o.u0 = u0;
+  o.out = out;
+  o.a0 = a0;
+  o.a1 = a1;
+  o.dt = dt;
+  o.maxabs = maxabs;
+  o.bgeo = m->bgeo;
+  switch (m->N) {
+    case 1: return av_stage_n<1>(a, r, o, mode, burg);
+    case 2: return av_stage_n<2>(a, r, o, mode, burg);
+    case 3: return av_stage_n<3>(a, r, o, mode, burg);
+    case 4: return av_stage_n<4>(a, r, o, mode, burg);
+  }
+  dgk::set_error("shock capturing is built for N <= 4 (N=%d)", m->N);
+  return DGK_ERR_UNSUPPORTED;
 }
 
 }  // namespace
@@ -827,7 +1078,6 @@ extern "C" int dgk_av_create(dgk_dgmesh m, const double* Vinv_host, const dgk_av
   rc |= dgk_malloc(m->dev, (size_t)m->K * 8 + 8, nullptr, (void**)&a->eps);
   for (int d = 0; d < 3; ++d) rc |= dgk_malloc(m->dev, nstate, nullptr, (void**)&a->q[d]);
   if (rc == DGK_OK) rc = dgk_fill_bytes(m->dev, a->eps, 0, (size_t)m->K * 8 + 8);
-  if (rc == DGK_OK) rc = dgk::split_prepare(m);
   if (rc != DGK_OK) {
     dgk_av_destroy(a);
     return rc;
@@ -892,10 +1142,7 @@ extern "C" int dgk_av_rhs(dgk_av a, const double* u_dev, double* out_dev) {
   dgk::clear_error();
   DGK_REQUIRE(a && u_dev && out_dev, "dgk_av_rhs: NULL argument");
   DGK_REQUIRE(out_dev != u_dev, "dgk_av_rhs: out must not alias u");
-  const size_t nstate = (size_t)a->m->state_off.back() * 8;
-  int rc = dgk_fill_bytes(a->m->dev, out_dev, 0, nstate);
-  if (rc != DGK_OK) return rc;
-  return av_viscous(a, u_dev, out_dev);
+  return av_run(a, u_dev, u_dev, out_dev, 0.0, 1.0, nullptr, nullptr, 2, false);
 }
 
 extern "C" int dgk_av_stage(dgk_av a, const double* u_dev, const double* u0_dev, double* out_dev,
@@ -904,25 +1151,15 @@ extern "C" int dgk_av_stage(dgk_av a, const double* u_dev, const double* u0_dev,
   DGK_REQUIRE(a && u_dev && out_dev && dt_dev, "dgk_av_stage: NULL argument");
   DGK_REQUIRE(out_dev != u_dev, "dgk_av_stage: out must not alias u");
   DGK_REQUIRE(a0 == 0.0 || u0_dev, "dgk_av_stage: u0 is NULL with a0 != 0");
-  dgk_dgmesh m = a->m;
-  int rc = dgk::split_volume(m, u_dev);
-  if (rc != DGK_OK) return rc;
-  rc = av_viscous(a, u_dev, m->rhs);
-  if (rc != DGK_OK) return rc;
-  return dgk::split_surface(m, u_dev, u0_dev ? u0_dev : u_dev, out_dev, a0, a1, dt_dev,
-                            (unsigned long long*)maxabs_dev, 0);
+  return av_run(a, u_dev, u0_dev ? u0_dev : u_dev, out_dev, a0, a1, dt_dev, maxabs_dev,
+                a0 != 0.0 ? 0 : 1, true);
 }
 
 extern "C" int dgk_av_rhs_total(dgk_av a, const double* u_dev, double* out_dev) {
   dgk::clear_error();
   DGK_REQUIRE(a && u_dev && out_dev, "dgk_av_rhs_total: NULL argument");
   DGK_REQUIRE(out_dev != u_dev, "dgk_av_rhs_total: out must not alias u");
-  dgk_dgmesh m = a->m;
-  int rc = dgk::split_volume(m, u_dev);
-  if (rc != DGK_OK) return rc;
-  rc = av_viscous(a, u_dev, m->rhs);
-  if (rc != DGK_OK) return rc;
-  return dgk::split_surface(m, u_dev, u_dev, out_dev, 0.0, 1.0, nullptr, nullptr, 1);
+  return av_run(a, u_dev, u_dev, out_dev, 0.0, 1.0, nullptr, nullptr, 2, true);
 }
 
 extern "C" int dgk_av_cfl_dt(dgk_device dev, double* scal_dev, double cfl, double cfl_v,

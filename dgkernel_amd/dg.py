@@ -187,9 +187,14 @@ class BurgersSolver:
     """
 
     def __init__(self, dev: _lib.Device, st: DGSetup, cfl: float = 0.3,
-                 hmin: float | None = None):
+                 hmin: float | None = None, tuned: bool = True):
         self.dev, self.st, self.cfl = dev, st, cfl
         self.mesh = DGMesh(dev, st)
+        # launch shape from the autotuner's table (dgkernel_amd/tuned_stage.json), if any
+        self.tuned = None
+        if tuned:
+            from . import autotune
+            self.tuned = autotune.apply(self.mesh)
         n = st.state_bytes // 8
         self.u = dev.zeros(n)
         self.u1 = dev.zeros(n)

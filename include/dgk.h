@@ -345,8 +345,9 @@ int dgk_av_eps_dev(dgk_av a, double** eps_dev);
  * normalised modal coefficients uh/|uh| (FP32, [Ktot][Np]). */
 int dgk_av_sensor_pp(dgk_av a, const double* u_dev, double* maxeps_dev, double* frac_dev,
                      float* frac32_dev, float* feats_dev);
-/* Learned sensor: eps = eps0 * net(uh/|uh|), modal transform on FP32 matrix
- * cores, network on BF16 matrix cores with FP32 accumulation. The network's
+/* Learned sensor: eps = eps0 * net(uh/|uh|), modal transform on BF16 matrix
+ * cores in split form (hi*hi + hi*lo + lo*hi, ~16 bits), network on BF16
+ * matrix cores with FP32 accumulation. The network's
  * input width must be Np. frac_dev (FP64, may be NULL) receives net(.). */
 int dgk_av_sensor_mlp(dgk_av a, dgk_mlp net, const double* u_dev, double* maxeps_dev,
                       double* frac_dev);
