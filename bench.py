@@ -48,6 +48,21 @@ def stage_flops_per_elem(Np: int, Nfp: int) -> float:
     return 6.0 * Np * Np + 8.0 * Np * Nfp + 12.0 * Np + 72.0 * Nfp
 
 
+def stage_kernel_name(N: int, solver) -> str:
+    """The stage kernel the solver's configuration launches (variant numbers of
+    dgk_burgers_set_variant; the tuned entry when the autotuner table is used)."""
+    e = solver.tuned or {}
+    v = e.get("variant", 0)
+    if N > 4 or v == 3:
+        return f"split stage (deriv_affine_kernel<{N}, 0, true> + surface_update_kernel<{N}, 0>)"
+    if v == 1:
+        return f"burgers_stage_kernel<{N}, {e.get('tile') or 64}>"
+    if v == 4:
+        return f"burgers_stage_mfma_kernel<{N}, {max(1, (e.get('tile') or 64) // 16)}>"
+    kind = {5: 1, 6: 2}.get(v, 0)
+    return f"burgers_stage_pipe_kernel<{N}, {max(1, (e.get('tile') or 64) // 16)}, MODE, {kind}>"
+
+
 def read_traffic(name: str):
     """HBM bytes per launch recorded from rocprofv3 --pmc runs (profiles/<name>)."""
     p = ROOT / "profiles" / name
@@ -478,6 +493,11 @@ def main(argv=None) -> int:
         solver.mesh.set_order(bool(a.xcd))
     if a.tile or a.bpc:
         solver.mesh.set_tile(a.tile or 64, a.bpc)
+    if a.variant >= 0 or a.tile or a.bpc or a.xcd >= 0:
+        e = dict(solver.tuned or {})
+        e.update({k: v for k, v in (("variant", a.variant), ("tile", a.tile), ("bpc", a.bpc),
+                                     ("xcd", a.xcd)) if v not in (-1, 0) or k == "variant" and v == 0})
+        solver.tuned = e
     solver.set_state(u0)
     del u0
     dev.sync()
@@ -564,8 +584,7 @@ def main(argv=None) -> int:
                    "kpart_max": int(st.Kp.max()), "stages_timed": 3 * a.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"burgers_stage_mfma_kernel<{a.order},4>" if a.order <= 4
-                                else f"burgers_stage_kernel<{a.order},64>"),
+                     "kernel": stage_kernel_name(a.order, solver),
                      "bytes_per_elem": bpe,
                      "stage_ms_mean": float(stage_ms.mean()),
                      "fp64_tflops": flops, "fp64_frac": flops / FP64_PEAK_TFLOPS},

@@ -706,39 +706,61 @@ template <int N>
 __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* __restrict__ P) {
   constexpr int NP = Tet<N>::Np, S = NP + 1;
   constexpr int MTM = (NP + 15) / 16, KSM = (NP + 31) / 32;
+  constexpr int PV = (16 * NP + 63) / 64;  // tile values per lane
   static_assert(NP <= 48, "modal rows must fit three 16-row tiles");
   __shared__ double sUall[4][16 * S];
+  // V^-1 fragments (BF16 A operand of 16x16x32, hi and lo parts), shared by
+  // the block: entry [part][mt][s][lane] holds A[16 mt + row][32 s + 8 g + j].
+  __shared__ bf16x8 sV[2][MTM][KSM][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;
+  for (int q = threadIdx.x; q < MTM * KSM * 64; q += 256) {
+    const int l = q & 63, ms = q >> 6, mt = ms / KSM, s = ms % KSM;
+    bf16x8 h, lo;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int i = 16 * mt + (l & 15), k = 32 * s + 8 * (l >> 4) + j;
+      const float v = (i < NP && k < NP) ? (float)a.Vinv[i * NP + k] : 0.f;
+      h[j] = bf16_hi(v);
+      lo[j] = bf16_lo(v);
+    }
+    sV[0][mt][s][l] = h;
+    sV[1][mt][s][l] = lo;
+  }
   double* sU = sUall[w];
   const MlpLayout L = mlp_layout(NP);
   MlpRegs R;
   mlp_load(P, L, lane, R);
-  // V^-1 fragments (BF16 A operand of 16x16x32, hi and lo parts):
-  // A[16 mt + row][32 s + 8 g + j].
-  bf16x8 vh[MTM][KSM], vl[MTM][KSM];
-#pragma unroll
-  for (int mt = 0; mt < MTM; ++mt)
-#pragma unroll
-    for (int s = 0; s < KSM; ++s)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int i = 16 * mt + c, k = 32 * s + 8 * g + j;
-        const float v = (i < NP && k < NP) ? (float)a.Vinv[i * NP + k] : 0.f;
-        vh[mt][s][j] = bf16_hi(v);
-        vl[mt][s][j] = bf16_lo(v);
-      }
+  __syncthreads();
   double emax = 0.0;
   const int nw = gridDim.x * 4;
-  for (int t = blockIdx.x * 4 + w; t < a.ntiles; t += nw) {
-    const Tile tl = a.tiles[t];
+  int t = blockIdx.x * 4 + w;
+  // the wave's first tile, then each next tile's values in flight during the current one
+  Tile tl;
+  double pre[PV];
+  if (t < a.ntiles) {
+    tl = a.tiles[t];
+#pragma unroll
+    for (int j = 0; j < PV; ++j) pre[j] = a.u[tl.vbase + min(lane + 64 * j, tl.count * NP - 1)];
+  }
+  for (; t < a.ntiles; t += nw) {
+    const Tile cur = tl;
     wave_fence();
-    for (int i = lane; i < tl.count * NP; i += 64) {
-      const int e = i / NP, nn = i - e * NP;
-      sU[e * S + nn] = a.u[tl.vbase + i];
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      if (i < cur.count * NP) {
+        const int e = i / NP, nn = i - e * NP;
+        sU[e * S + nn] = pre[j];
+      }
+    }
+    if (t + nw < a.ntiles) {
+      tl = a.tiles[t + nw];
+#pragma unroll
+      for (int j = 0; j < PV; ++j) pre[j] = a.u[tl.vbase + min(lane + 64 * j, tl.count * NP - 1)];
     }
     wave_fence();
-    const int cc = c < tl.count ? c : 0;
+    const int cc = c < cur.count ? c : 0;
     floatx4 acc[MTM];
 #pragma unroll
     for (int mt = 0; mt < MTM; ++mt) acc[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
@@ -756,9 +778,10 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
       }
 #pragma unroll
       for (int mt = 0; mt < MTM; ++mt) {
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[mt][s], bh, acc[mt], 0, 0, 0);
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt][s], bl, acc[mt], 0, 0, 0);
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[mt][s], bh, acc[mt], 0, 0, 0);
+        const bf16x8 vh = sV[0][mt][s][lane], vl = sV[1][mt][s][lane];
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl, bh, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh, bl, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh, bh, acc[mt], 0, 0, 0);
       }
     }
     // lane (c, g) holds uh rows 16 mt + 4 g + r of element c
@@ -781,8 +804,8 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
         X[s][j] = (__bf16)(mt < MTM ? acc[mt < MTM ? mt : 0][r] * inv : 0.f);
       }
     const float y = mlp_forward_tile(R, X);
-    if (g == 0 && c < tl.count) {
-      const int64_t ge = (int64_t)tl.gbase + c;
+    if (g == 0 && c < cur.count) {
+      const int64_t ge = (int64_t)cur.gbase + c;
       const double vol = (4.0 / 3.0) * a.geo[9 * a.K + ge];
       const double h = cbrt(6.0 * vol);
       const double e = a.ceps * h * sqrt(3.0) * umax / N * (double)y;
