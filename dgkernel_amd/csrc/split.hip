@@ -13,10 +13,13 @@
 //
 // Surface kernel layout: each wave takes 16-element tiles on its own (the
 // MFMA N dimension); the LIFT fragments sit in LDS for the block. Lane
-// (c, g) computes the B operand for element c directly: the flux values at
-// face points k = 4 s + g (FP64/FP32) or 32 s + 8 g + e (bf16), so the flux
-// never leaves registers. State stays FP64; the product runs in the plan's
-// precision with FP32 accumulation below FP64.
+// (c, g) owns face g of element c: it gathers that face's own and
+// neighbour traces, all at once, and computes the LF jumps of the face's
+// points directly as its B operand. The LIFT columns are taken face by face:
+// MFMA step s multiplies point s of every face (FP64/FP32, k = 4) or points
+// 8s..8s+7 (bf16, k = 32), so no value moves between lanes. State stays
+// FP64; the product runs in the plan's precision with FP32 accumulation
+// below FP64.
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -38,8 +41,10 @@ template <int N, int PREC>
 struct SCfg {
   static constexpr int NP = dgk::Tet<N>::Np, NFP = dgk::Tet<N>::Nfp, NF = 4 * NFP;
   static constexpr int MTL = (NP + 15) / 16;  // 16-row tiles of LIFT
-  static constexpr int KSTEP = is_bf(PREC) ? 32 : 4;
-  static constexpr int KSL = (NF + KSTEP - 1) / KSTEP;
+  // MFMA steps over the face points: one point per face (k = 4) or eight
+  // points per face (bf16, k = 32, padded past Nfp with zero columns).
+  static constexpr int KSL = is_bf(PREC) ? (NFP + 7) / 8 : NFP;
+  static constexpr int NPT = is_bf(PREC) ? 8 * KSL : NFP;  // padded points per face
   static constexpr int PARTS = parts_of(PREC);
   static constexpr size_t NFRAG = (size_t)MTL * KSL * 64;  // fragments per part
   static constexpr size_t FRAG_BYTES =
@@ -52,7 +57,7 @@ struct SurfArgs {
   int ntiles;
   const void* frag;           // LIFT fragments, [part][MTL][KSL][64]
   const uint8_t* face_nodes;  // [4][24][Nfp]
-  const double* bgeo;         // [11][K]
+  const double* bgeo;         // [19][K]; rows 0..2 the folded metrics
   int64_t K;
   const int32_t* nbr;         // [4][K] value offset of each neighbour's node 0
   const int32_t* codes;       // [K] four 5-bit orientation codes
@@ -69,125 +74,136 @@ struct SurfArgs {
 template <int PREC>
 using AccT = typename std::conditional<PREC == kF64, doublex4, floatx4>::type;
 
+// Row of LIFT g held in accumulator slot r of lane group g: FP64 D lanes hold
+// rows g + 4r of a 16-row tile, FP32 D lanes rows 4g + r.
+template <int PREC>
+__device__ inline int acc_row(int g, int r) {
+  return PREC == kF64 ? g + 4 * r : 4 * g + r;
+}
+
 template <int N, int PREC>
 __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel(SurfArgs a) {
   using C = SCfg<N, PREC>;
   constexpr int NP = C::NP, NFP = C::NFP, NF = C::NF, MTL = C::MTL, KSL = C::KSL, W = C::W;
-  constexpr int PARTS = C::PARTS;
+  constexpr int NPT = C::NPT, PARTS = C::PARTS;
   constexpr dgk::FmaskTable<N> FM{};
   __shared__ __attribute__((aligned(16))) unsigned char sA[C::FRAG_BYTES];
   __shared__ uint8_t sFN[4 * 24 * NFP];
-  __shared__ int16_t sFM[NF];
+  __shared__ uint8_t sFM[NF];
   {
     const uint4* src = (const uint4*)a.frag;
     uint4* dst = (uint4*)sA;
     for (int i = threadIdx.x; i < (int)(C::FRAG_BYTES / 16); i += 64 * W) dst[i] = src[i];
     for (int i = threadIdx.x; i < 4 * 24 * NFP; i += 64 * W) sFN[i] = a.face_nodes[i];
-    for (int i = threadIdx.x; i < NF; i += 64 * W) sFM[i] = (int16_t)FM.v[i / NFP][i % NFP];
+    for (int i = threadIdx.x; i < NF; i += 64 * W) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
   }
   __syncthreads();
   const double dt = a.mode == 0 ? *a.dt : 0.0;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int c = lane & 15, g = lane >> 4;
+  const int c = lane & 15, g = lane >> 4;  // element of the tile, face
   const int64_t K = a.K;
   double lmax = 0.0;
   for (int t = blockIdx.x * W + w; t < a.ntiles; t += gridDim.x * W) {
     const dgk::Tile tile = a.tiles[t];
     const bool valid = c < tile.count;
-    const int64_t ge = tile.gbase + (valid ? c : 0);
-    const int64_t vb = tile.vbase + (int64_t)(valid ? c : 0) * NP;
-    const int packed = a.codes[ge];
-    int32_t nb[4];
-    double sn[4], fs[4];
+    const int ce = valid ? c : tile.count - 1;  // clamped: every load stays in the tile
+    const int64_t ge = tile.gbase + ce;
+    const int64_t vb = tile.vbase + (int64_t)ce * NP;
+    const int code = (a.codes[ge] >> (5 * g)) & 31;
+    const int32_t nb = a.nbr[(int64_t)g * K + ge];
+    double fa, fb;
+    dgk::face_factors(g, a.bgeo[ge], a.bgeo[K + ge], a.bgeo[2 * K + ge], fa, fb);
+    // Own and neighbour traces of face g, all gathers in flight together.
+    const uint8_t* fm = sFM + g * NFP;
+    const uint8_t* fn = sFN + (g * 24 + code) * NFP;
+    double uM[NFP], uP[NFP];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      nb[f] = a.nbr[f * K + ge];
-      sn[f] = a.bgeo[(3 + f) * K + ge];
-      fs[f] = a.bgeo[(7 + f) * K + ge];
+    for (int q = 0; q < NFP; ++q) {
+      uM[q] = a.u[vb + fm[q]];
+      uP[q] = a.u[nb + fn[q]];
     }
-    // LF flux jump at face point k of element c (FP64).
-    auto flux = [&](int k) -> double {
-      if (!valid || k >= NF) return 0.0;
-      const int f = k / NFP, fp = k - f * NFP;
-      const int code = (packed >> (5 * f)) & 31;
-      const int32_t nbf = f == 0 ? nb[0] : f == 1 ? nb[1] : f == 2 ? nb[2] : nb[3];
-      const double snf = f == 0 ? sn[0] : f == 1 ? sn[1] : f == 2 ? sn[2] : sn[3];
-      const double fsf = f == 0 ? fs[0] : f == 1 ? fs[1] : f == 2 ? fs[2] : fs[3];
-      const double uM = a.u[vb + sFM[k]];
-      const double uP = a.u[nbf + sFN[(f * 24 + code) * NFP + fp]];
-      const double fM = 0.5 * uM * uM, fP = 0.5 * uP * uP;
-      const double al = fmax(fabs(uM), fabs(uP));
-      return fsf * (0.5 * (fM - fP) * snf + 0.5 * al * fabs(snf) * (uP - uM));
-    };
-    AccT<PREC> acc[MTL];
+    // Fscale (n.f(u-) - n.f*), f(u) = u^2/2 (1,1,1), factored.
+    double gj[NPT];
 #pragma unroll
-    for (int tt = 0; tt < MTL; ++tt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[tt][r] = 0;
-    // Not unrolled: the flux gathers of every step would otherwise be
-    // hoisted together and exhaust the registers.
-#pragma unroll 1
-    for (int s = 0; s < KSL; ++s) {
-      if constexpr (PREC == kF64 || PREC == kF32) {
-        const double b = flux(4 * s + g);
-#pragma unroll
-        for (int tt = 0; tt < MTL; ++tt) {
-          const size_t q = ((size_t)tt * KSL + s) * 64 + lane;
-          if constexpr (PREC == kF64)
-            acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(((const double*)sA)[q], b, acc[tt], 0, 0, 0);
-          else
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(((const float*)sA)[q], (float)b, acc[tt], 0,
-                                                           0, 0);
-        }
+    for (int q = 0; q < NPT; ++q) {
+      if (q < NFP) {
+        const double al = fmax(fabs(uM[q]), fabs(uP[q]));
+        gj[q] = (uM[q] - uP[q]) * fma(fa, uM[q] + uP[q], -fb * al);
       } else {
-        bf16x8 bp[PARTS];
+        gj[q] = 0.0;
+      }
+    }
+    // B operand of each step in the product's precision.
+    bf16x8 bp[is_bf(PREC) ? KSL : 1][PARTS];
+    if constexpr (is_bf(PREC)) {
+#pragma unroll
+      for (int s = 0; s < KSL; ++s)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const double x = flux(32 * s + 8 * g + e);
+          const double x = gj[8 * s + e];
           const __bf16 h = (__bf16)(float)x;
           const double r1 = x - (double)(float)h;
           const __bf16 m = (__bf16)(float)r1;
-          bp[0][e] = h;
-          bp[1][e] = m;
-          if constexpr (PARTS == 3) bp[2][e] = (__bf16)(float)(r1 - (double)(float)m);
+          bp[s][0][e] = h;
+          bp[s][1][e] = m;
+          if constexpr (PARTS == 3) bp[s][2][e] = (__bf16)(float)(r1 - (double)(float)m);
         }
+    }
 #pragma unroll
-        for (int tt = 0; tt < MTL; ++tt) {
-          const size_t q = ((size_t)tt * KSL + s) * 64 + lane;
+    for (int tt = 0; tt < MTL; ++tt) {
+      // Epilogue operands of this row tile, loaded ahead of its product.
+      double rv[4], uv[4], u0v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = min(16 * tt + acc_row<PREC>(g, r), NP - 1);
+        rv[r] = a.rhs[vb + i];
+        if (a.mode == 0) {
+          uv[r] = a.u[vb + i];
+          u0v[r] = a.a0 != 0.0 ? __builtin_nontemporal_load(a.u0 + vb + i) : 0.0;
+        }
+      }
+      AccT<PREC> acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = 0;
+#pragma unroll
+      for (int s = 0; s < KSL; ++s) {
+        const size_t q = ((size_t)tt * KSL + s) * 64 + lane;
+        if constexpr (PREC == kF64) {
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(((const double*)sA)[q], gj[s], acc, 0, 0, 0);
+        } else if constexpr (PREC == kF32) {
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((const float*)sA)[q], (float)gj[s], acc, 0, 0,
+                                                     0);
+        } else {
           const bf16x8* F = (const bf16x8*)sA;
           if constexpr (PREC == kBF3) {
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[0], acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[1], acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[0], acc[tt], 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[s][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][0], acc, 0, 0, 0);
           } else {
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[2 * C::NFRAG + q], bp[0], acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[2], acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[1], acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[0], acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[1], acc[tt], 0, 0, 0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[0], acc[tt], 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[2 * C::NFRAG + q], bp[s][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[s][1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[s][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][0], acc, 0, 0, 0);
           }
         }
       }
-    }
-    // Epilogue. FP64 D lanes hold rows g + 4r of a tile, FP32 D lanes rows 4g + r.
-    if (valid) {
-#pragma unroll
-      for (int tt = 0; tt < MTL; ++tt) {
+      if (valid) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = 16 * tt + (PREC == kF64 ? g + 4 * r : 4 * g + r);
+          const int i = 16 * tt + acc_row<PREC>(g, r);
           if (i < NP) {
-            const double l = a.rhs[vb + i] + (double)acc[tt][r];
+            const double l = rv[r] + (double)acc[r];
             double o;
             if (a.mode == 0) {
-              o = a.a1 * fma(dt, l, a.u[vb + i]);
-              if (a.a0 != 0.0) o = fma(a.a0, a.u0[vb + i], o);
+              o = a.a1 * fma(dt, l, uv[r]);
+              if (a.a0 != 0.0) o = fma(a.a0, u0v[r], o);
               lmax = fmax(lmax, fabs(o));
             } else {
               o = l;
             }
-            a.out[vb + i] = o;
+            __builtin_nontemporal_store(o, a.out + vb + i);
           }
         }
       }
@@ -218,20 +234,24 @@ float bf16_to_float(uint16_t b) {
   return f;
 }
 
-// LIFT fragments in the MFMA A-operand lane order of the precision.
+// LIFT fragments in the MFMA A-operand lane order of the precision, columns
+// taken face by face: lane (row, g) of step s holds LIFT[row][g Nfp + s]
+// (k = 4) or LIFT[row][g Nfp + 8 s + e], e = 0..7 (bf16, zero past Nfp).
 std::vector<unsigned char> lift_fragments(const std::vector<double>& LIFT, int N, int prec) {
-  const int NP = dgk::np_of(N), NF = 4 * dgk::nfp_of(N);
-  const int MTL = (NP + 15) / 16, KSTEP = is_bf(prec) ? 32 : 4;
-  const int KSL = (NF + KSTEP - 1) / KSTEP, PARTS = parts_of(prec);
+  const int NP = dgk::np_of(N), NFP = dgk::nfp_of(N), NF = 4 * NFP;
+  const int MTL = (NP + 15) / 16;
+  const int KSL = is_bf(prec) ? (NFP + 7) / 8 : NFP, PARTS = parts_of(prec);
   const size_t nfrag = (size_t)MTL * KSL * 64;
-  auto val = [&](int i, int k) { return (i < NP && k < NF) ? LIFT[(size_t)i * NF + k] : 0.0; };
+  auto val = [&](int i, int g, int q) {
+    return (i < NP && q < NFP) ? LIFT[(size_t)i * NF + g * NFP + q] : 0.0;
+  };
   std::vector<unsigned char> buf;
   if (!is_bf(prec)) {
     buf.resize(nfrag * (prec == kF64 ? 8 : 4));
     for (int tt = 0; tt < MTL; ++tt)
       for (int s = 0; s < KSL; ++s)
         for (int l = 0; l < 64; ++l) {
-          const double v = val(16 * tt + (l & 15), 4 * s + (l >> 4));
+          const double v = val(16 * tt + (l & 15), l >> 4, s);
           const size_t q = ((size_t)tt * KSL + s) * 64 + l;
           if (prec == kF64) ((double*)buf.data())[q] = v;
           else ((float*)buf.data())[q] = (float)v;
@@ -243,7 +263,7 @@ std::vector<unsigned char> lift_fragments(const std::vector<double>& LIFT, int N
       for (int s = 0; s < KSL; ++s)
         for (int l = 0; l < 64; ++l)
           for (int e = 0; e < 8; ++e) {
-            const double v = val(16 * tt + (l & 15), 32 * s + 8 * (l >> 4) + e);
+            const double v = val(16 * tt + (l & 15), l >> 4, 8 * s + e);
             const size_t q = (((size_t)tt * KSL + s) * 64 + l) * 8 + e;
             const uint16_t h = bf16_bits((float)v);
             const double r1 = v - (double)bf16_to_float(h);
