@@ -276,7 +276,10 @@ def bench_c4_stage(a, dev) -> dict:
     from dgkernel_amd import mesh as M
     n, N = a.n, a.order
     m = M.cube_mesh(n, n, n, periodic=True)
-    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, max(2, a.parts), 0.0, seed=a.seed))
+    Kp = M.slab_partition(m.K, max(2, a.parts), 0.0, seed=a.seed)
+    if a.morton:
+        m = M.morton_within_partitions(m, Kp)
+    st = dg.make_setup(m, N, Kp=Kp)
     x, y, z = st.node_coords()
     u0 = (0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
           + M.uniform(a.seed, 7, x.size, -1e-6, 1e-6).reshape(x.shape))
@@ -285,7 +288,8 @@ def bench_c4_stage(a, dev) -> dict:
     ref = None
     out = {"metric": "DOF-stage-updates/s per contraction precision (P=6 high-order case)",
            "config": {"workload": f"Burgers SSP-RK3, {n}^3x6={K} periodic tets, P={N}, "
-                                  f"{st.npart} slab partitions, {a.steps} steps",
+                                  f"{st.npart} slab partitions, {a.steps} steps"
+                                  + (", Morton order within partitions" if a.morton else ""),
                       "tets": K, "order": N, "Np": Np, "steps": a.steps},
            "precisions": {}}
     solver = None

@@ -12,10 +12,11 @@
 //       - alpha |n.(1,1,1)| (u+ - u-)/2,  alpha = max(|u-|, |u+|).
 //
 // Surface kernel layout: each wave takes 16-element tiles on its own (the
-// MFMA N dimension); the LIFT fragments sit in LDS for the block. Lane
-// (c, g) owns face g of element c: it gathers that face's own and
-// neighbour traces, all at once, and computes the LF jumps of the face's
-// points directly as its B operand. The LIFT columns are taken face by face:
+// MFMA N dimension); the LIFT fragments sit in LDS for the block, and each
+// wave has an LDS buffer for its tile. Lane (c, g) owns face g of element c:
+// it reads that face's own trace from the staged tile, gathers the
+// neighbour trace (all gathers in flight at once), and computes the LF jumps
+// of the face's points directly as its B operand. The LIFT columns are taken face by face:
 // MFMA step s multiplies point s of every face (FP64/FP32, k = 4) or points
 // 8s..8s+7 (bf16, k = 32), so no value moves between lanes. State stays
 // FP64; the product runs in the plan's precision with FP32 accumulation
@@ -49,7 +50,18 @@ struct SCfg {
   static constexpr size_t NFRAG = (size_t)MTL * KSL * 64;  // fragments per part
   static constexpr size_t FRAG_BYTES =
       PREC == kF64 ? NFRAG * 8 : (PREC == kF32 ? NFRAG * 4 : NFRAG * 16 * PARTS);
-  static constexpr int W = 8;  // waves per block
+  static constexpr int NPU = NP | 1;                          // LDS row stride of a tile
+  static constexpr size_t TILE_BYTES = (size_t)16 * NPU * 8;  // one wave's tile buffer
+  // STAGE: each wave stages its whole tile in LDS (own traces from LDS, RK
+  // update over the whole tile with unit stride) when the fragments and 8
+  // tile buffers fit the 160 KiB of LDS, with 8 waves per block; otherwise
+  // own traces are read from memory, the update runs per 16-row tile through
+  // a 16 x 17 buffer, and a block has 12 waves (three per SIMD) to cover the
+  // gathers' latency.
+  static constexpr bool STAGE = FRAG_BYTES + 8 * TILE_BYTES + 4096 <= 160 * 1024;
+  static constexpr int W = STAGE ? 8 : 12;                    // waves per block
+  static constexpr int BUF = STAGE ? 16 * NPU : 16 * 17;      // doubles per wave
+  static constexpr int NV = (16 * NP + 63) / 64;  // tile values per lane
 };
 
 struct SurfArgs {
@@ -81,13 +93,28 @@ __device__ inline int acc_row(int g, int r) {
   return PREC == kF64 ? g + 4 * r : 4 * g + r;
 }
 
+// LDS writes of one lane made visible to the other lanes of the same wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per tile, each wave: (1) stages the tile's u in its LDS buffer with
+// unit-stride loads (STAGE); (2) lane (c, g) reads the own trace of face g
+// (from LDS, or from memory), gathers the neighbour trace, and forms the LF
+// jumps as its B operand; (3) LIFT on matrix cores, results parked in the
+// LDS buffer; (4) the RK update with loads of rhs, u, u0 and stores that are
+// unit-stride over the tile (STAGE) or over 128-byte row segments.
 template <int N, int PREC>
 __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel(SurfArgs a) {
   using C = SCfg<N, PREC>;
   constexpr int NP = C::NP, NFP = C::NFP, NF = C::NF, MTL = C::MTL, KSL = C::KSL, W = C::W;
-  constexpr int NPT = C::NPT, PARTS = C::PARTS;
+  constexpr int NPT = C::NPT, PARTS = C::PARTS, NPU = C::NPU, NV = C::NV;
+  constexpr bool STAGE = C::STAGE;
   constexpr dgk::FmaskTable<N> FM{};
   __shared__ __attribute__((aligned(16))) unsigned char sA[C::FRAG_BYTES];
+  __shared__ double sTall[W][C::BUF];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[NF];
   {
@@ -101,46 +128,70 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
   const double dt = a.mode == 0 ? *a.dt : 0.0;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;  // element of the tile, face
+  double* sT = sTall[w];
   const int64_t K = a.K;
   double lmax = 0.0;
-  for (int t = blockIdx.x * W + w; t < a.ntiles; t += gridDim.x * W) {
+  // Tile order: with a grid that is a multiple of 8, block b runs on XCD
+  // b % 8 (round-robin dispatch), and each XCD walks one contiguous eighth of
+  // the tile list, so neighbour traces mostly hit that XCD's L2; otherwise a
+  // plain grid stride.
+  int t0 = blockIdx.x * W + w, t1 = a.ntiles, ts = gridDim.x * W;
+  if (gridDim.x % 8 == 0) {
+    const int x = blockIdx.x % 8;
+    t0 = (int)((int64_t)a.ntiles * x / 8) + (blockIdx.x / 8) * W + w;
+    t1 = (int)((int64_t)a.ntiles * (x + 1) / 8);
+    ts = (gridDim.x / 8) * W;
+  }
+  for (int t = t0; t < t1; t += ts) {
     const dgk::Tile tile = a.tiles[t];
-    const bool valid = c < tile.count;
-    const int ce = valid ? c : tile.count - 1;  // clamped: every load stays in the tile
+    const int cnt = tile.count, lim = cnt * NP;
+    const int ce = c < cnt ? c : cnt - 1;  // clamped: every load stays in the tile
     const int64_t ge = tile.gbase + ce;
-    const int64_t vb = tile.vbase + (int64_t)ce * NP;
+    const double* ut = a.u + tile.vbase;
+    // (1) the tile's u into LDS
+    if constexpr (STAGE) {
+      wave_sync();  // previous tile's readers of sT are done
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int v = lane + 64 * j;
+        if (v < lim) {
+          const int e = v / NP;
+          sT[e * NPU + (v - e * NP)] = ut[v];
+        }
+      }
+    }
     const int code = (a.codes[ge] >> (5 * g)) & 31;
     const int32_t nb = a.nbr[(int64_t)g * K + ge];
     double fa, fb;
     dgk::face_factors(g, a.bgeo[ge], a.bgeo[K + ge], a.bgeo[2 * K + ge], fa, fb);
-    // Own and neighbour traces of face g, all gathers in flight together.
-    const uint8_t* fm = sFM + g * NFP;
+    // (2) traces of face g, all loads in flight together
     const uint8_t* fn = sFN + (g * 24 + code) * NFP;
-    double uM[NFP], uP[NFP];
+    const uint8_t* fm = sFM + g * NFP;
+    double uP[NFP], uMg[STAGE ? 1 : NFP];
 #pragma unroll
     for (int q = 0; q < NFP; ++q) {
-      uM[q] = a.u[vb + fm[q]];
       uP[q] = a.u[nb + fn[q]];
+      if constexpr (!STAGE) uMg[q] = ut[ce * NP + fm[q]];
     }
-    // Fscale (n.f(u-) - n.f*), f(u) = u^2/2 (1,1,1), factored.
-    double gj[NPT];
-#pragma unroll
-    for (int q = 0; q < NPT; ++q) {
-      if (q < NFP) {
-        const double al = fmax(fabs(uM[q]), fabs(uP[q]));
-        gj[q] = (uM[q] - uP[q]) * fma(fa, uM[q] + uP[q], -fb * al);
-      } else {
-        gj[q] = 0.0;
-      }
-    }
-    // B operand of each step in the product's precision.
+    if constexpr (STAGE) wave_sync();  // sT holds the tile
+    const double* su = sT + ce * NPU;
+    // Fscale (n.f(u-) - n.f*), f(u) = u^2/2 (1,1,1), factored; zero past Nfp.
+    auto jump = [&](int q) -> double {
+      if (q >= NFP) return 0.0;
+      const double uM = STAGE ? su[fm[q]] : uMg[q];
+      const double al = fmax(fabs(uM), fabs(uP[q]));
+      return (uM - uP[q]) * fma(fa, uM + uP[q], -fb * al);
+    };
+    // B operand of each step in the product's precision: the jumps (k = 4)
+    // or their bf16 parts (k = 32), split as soon as each is formed.
+    double gj[is_bf(PREC) ? 1 : NPT];
     bf16x8 bp[is_bf(PREC) ? KSL : 1][PARTS];
     if constexpr (is_bf(PREC)) {
 #pragma unroll
       for (int s = 0; s < KSL; ++s)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const double x = gj[8 * s + e];
+          const double x = jump(8 * s + e);
           const __bf16 h = (__bf16)(float)x;
           const double r1 = x - (double)(float)h;
           const __bf16 m = (__bf16)(float)r1;
@@ -148,20 +199,30 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
           bp[s][1][e] = m;
           if constexpr (PARTS == 3) bp[s][2][e] = (__bf16)(float)(r1 - (double)(float)m);
         }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NPT; ++q) gj[q] = jump(q);
     }
+    if constexpr (STAGE) wave_sync();  // all reads of sT as u are done
+    const double* rt = a.rhs + tile.vbase;
+    const double* u0t = a.a0 != 0.0 ? a.u0 + tile.vbase : nullptr;
+    double* ot = a.out + tile.vbase;
+    // out = a0 u0 + a1 (u + dt (rhs + LIFT g)) (mode 0) or rhs + LIFT g (mode 1)
+    auto update = [&](int64_t idx, double lg) {
+      const double l = rt[idx] + lg;
+      double o;
+      if (a.mode == 0) {
+        o = a.a1 * fma(dt, l, ut[idx]);
+        if (a.a0 != 0.0) o = fma(a.a0, __builtin_nontemporal_load(u0t + idx), o);
+        lmax = fmax(lmax, fabs(o));
+      } else {
+        o = l;
+      }
+      __builtin_nontemporal_store(o, ot + idx);
+    };
+    // (3) LIFT g, row tile by row tile
 #pragma unroll
     for (int tt = 0; tt < MTL; ++tt) {
-      // Epilogue operands of this row tile, loaded ahead of its product.
-      double rv[4], uv[4], u0v[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = min(16 * tt + acc_row<PREC>(g, r), NP - 1);
-        rv[r] = a.rhs[vb + i];
-        if (a.mode == 0) {
-          uv[r] = a.u[vb + i];
-          u0v[r] = a.a0 != 0.0 ? __builtin_nontemporal_load(a.u0 + vb + i) : 0.0;
-        }
-      }
       AccT<PREC> acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = 0;
@@ -189,22 +250,35 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
           }
         }
       }
-      if (valid) {
+      if constexpr (STAGE) {
+        // park in sT[c][row] for the whole-tile update below
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = 16 * tt + acc_row<PREC>(g, r);
-          if (i < NP) {
-            const double l = rv[r] + (double)acc[r];
-            double o;
-            if (a.mode == 0) {
-              o = a.a1 * fma(dt, l, uv[r]);
-              if (a.a0 != 0.0) o = fma(a.a0, u0v[r], o);
-              lmax = fmax(lmax, fabs(o));
-            } else {
-              o = l;
-            }
-            __builtin_nontemporal_store(o, a.out + vb + i);
-          }
+          if (i < NP) sT[c * NPU + i] = (double)acc[r];
+        }
+      } else {
+        // update rows 16 tt .. 16 tt + 15 of the tile's elements now
+        wave_sync();  // previous readers of the buffer are done
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sT[c * 17 + acc_row<PREC>(g, r)] = (double)acc[r];
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
+          if (e < cnt && i < NP) update((int64_t)e * NP + i, sT[e * 17 + (lane & 15)]);
+        }
+      }
+    }
+    if constexpr (STAGE) {
+      wave_sync();  // sT holds LIFT g
+      // (4) RK update, unit stride over the tile
+#pragma unroll 7
+      for (int j = 0; j < NV; ++j) {
+        const int v = lane + 64 * j;
+        if (v < lim) {
+          const int e = v / NP;
+          update(v, sT[e * NPU + (v - e * NP)]);
         }
       }
     }
