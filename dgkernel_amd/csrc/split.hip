@@ -220,9 +220,39 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
       }
       __builtin_nontemporal_store(o, ot + idx);
     };
-    // (3) LIFT g, row tile by row tile
-#pragma unroll
+    // out from preloaded rhs, u, u0 values
+    auto finish = [&](int64_t idx, double lg, double rv, double uv, double u0v) {
+      const double l = rv + lg;
+      double o;
+      if (a.mode == 0) {
+        o = a.a1 * fma(dt, l, uv);
+        if (a.a0 != 0.0) o = fma(a.a0, u0v, o);
+        lmax = fmax(lmax, fabs(o));
+      } else {
+        o = l;
+      }
+      __builtin_nontemporal_store(o, ot + idx);
+    };
+    // (3) LIFT g, row tile by row tile (not unrolled without STAGE: the
+    // preloads of every row tile would otherwise be hoisted together)
+#pragma unroll(STAGE ? MTL : 1)
     for (int tt = 0; tt < MTL; ++tt) {
+      // Without STAGE: the rows this lane updates after the product (element
+      // 4 j + lane / 16, row 16 tt + lane % 16), loaded ahead of the product.
+      double pr[4], pu[4], p0[4];
+      int64_t pidx[4];
+      bool pok[4];
+      if constexpr (!STAGE) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
+          pok[j] = e < cnt && i < NP;
+          pidx[j] = pok[j] ? (int64_t)e * NP + i : 0;
+          pr[j] = rt[pidx[j]];
+          pu[j] = a.mode == 0 ? ut[pidx[j]] : 0.0;
+          p0[j] = a.a0 != 0.0 ? __builtin_nontemporal_load(u0t + pidx[j]) : 0.0;
+        }
+      }
       AccT<PREC> acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = 0;
@@ -264,10 +294,8 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
         for (int r = 0; r < 4; ++r) sT[c * 17 + acc_row<PREC>(g, r)] = (double)acc[r];
         wave_sync();
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
-          if (e < cnt && i < NP) update((int64_t)e * NP + i, sT[e * 17 + (lane & 15)]);
-        }
+        for (int j = 0; j < 4; ++j)
+          if (pok[j]) finish(pidx[j], sT[(4 * j + (lane >> 4)) * 17 + (lane & 15)], pr[j], pu[j], p0[j]);
       }
     }
     if constexpr (STAGE) {
