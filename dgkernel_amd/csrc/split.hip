@@ -12,15 +12,14 @@
 //       - alpha |n.(1,1,1)| (u+ - u-)/2,  alpha = max(|u-|, |u+|).
 //
 // Surface kernel layout: each wave takes 16-element tiles on its own (the
-// MFMA N dimension); the LIFT fragments sit in LDS for the block, and each
-// wave has an LDS buffer for its tile. Lane (c, g) owns face g of element c:
-// it reads that face's own trace from the staged tile, gathers the
-// neighbour trace (all gathers in flight at once), and computes the LF jumps
-// of the face's points directly as its B operand. The LIFT columns are taken face by face:
-// MFMA step s multiplies point s of every face (FP64/FP32, k = 4) or points
-// 8s..8s+7 (bf16, k = 32), so no value moves between lanes. State stays
-// FP64; the product runs in the plan's precision with FP32 accumulation
-// below FP64.
+// MFMA N dimension); the LIFT fragments sit in LDS for the block. Lane
+// (c, g) owns face g of element c: it gathers that face's own and
+// neighbour traces (all loads in flight at once) and computes the LF jumps
+// of the face's points directly as its B operand. The LIFT columns are
+// taken face by face: MFMA step s multiplies point s of every face
+// (FP64/FP32, k = 4) or points 8s..8s+7 (bf16, k = 32), so no value moves
+// between lanes. State stays FP64; the product runs in the plan's precision
+// with FP32 accumulation below FP64.
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -50,18 +49,9 @@ struct SCfg {
   static constexpr size_t NFRAG = (size_t)MTL * KSL * 64;  // fragments per part
   static constexpr size_t FRAG_BYTES =
       PREC == kF64 ? NFRAG * 8 : (PREC == kF32 ? NFRAG * 4 : NFRAG * 16 * PARTS);
-  static constexpr int NPU = NP | 1;                          // LDS row stride of a tile
-  static constexpr size_t TILE_BYTES = (size_t)16 * NPU * 8;  // one wave's tile buffer
-  // STAGE: each wave stages its whole tile in LDS (own traces from LDS, RK
-  // update over the whole tile with unit stride) when the fragments and 8
-  // tile buffers fit the 160 KiB of LDS, with 8 waves per block; otherwise
-  // own traces are read from memory, the update runs per 16-row tile through
-  // a 16 x 17 buffer, and a block has 12 waves (three per SIMD) to cover the
-  // gathers' latency.
-  static constexpr bool STAGE = FRAG_BYTES + 8 * TILE_BYTES + 4096 <= 160 * 1024;
-  static constexpr int W = STAGE ? 8 : 12;                    // waves per block
-  static constexpr int BUF = STAGE ? 16 * NPU : 16 * 17;      // doubles per wave
-  static constexpr int NV = (16 * NP + 63) / 64;  // tile values per lane
+  // Waves per block: three per SIMD (four for BF16x3, which needs fewer
+  // registers) to cover the latency of the trace gathers.
+  static constexpr int W = PREC == kBF3 ? 16 : 12;
 };
 
 struct SurfArgs {
@@ -100,21 +90,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per tile, each wave: (1) stages the tile's u in its LDS buffer with
-// unit-stride loads (STAGE); (2) lane (c, g) reads the own trace of face g
-// (from LDS, or from memory), gathers the neighbour trace, and forms the LF
-// jumps as its B operand; (3) LIFT on matrix cores, results parked in the
-// LDS buffer; (4) the RK update with loads of rhs, u, u0 and stores that are
-// unit-stride over the tile (STAGE) or over 128-byte row segments.
+// Per tile, each wave: lane (c, g) loads the own and neighbour traces of
+// face g (all loads in flight together) and forms the LF jumps as its B
+// operand; then, per 16-row tile of LIFT: the rhs, u, u0 values this lane
+// will update are loaded, the product runs on matrix cores, and the result
+// goes through the wave's 16 x 17 LDS buffer so that each lane updates and
+// stores rows of 16 consecutive values.
 template <int N, int PREC>
 __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel(SurfArgs a) {
   using C = SCfg<N, PREC>;
   constexpr int NP = C::NP, NFP = C::NFP, NF = C::NF, MTL = C::MTL, KSL = C::KSL, W = C::W;
-  constexpr int NPT = C::NPT, PARTS = C::PARTS, NPU = C::NPU, NV = C::NV;
-  constexpr bool STAGE = C::STAGE;
+  constexpr int NPT = C::NPT, PARTS = C::PARTS;
   constexpr dgk::FmaskTable<N> FM{};
   __shared__ __attribute__((aligned(16))) unsigned char sA[C::FRAG_BYTES];
-  __shared__ double sTall[W][C::BUF];
+  __shared__ double sTall[W][16 * 17];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[NF];
   {
@@ -148,39 +137,24 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
     const int ce = c < cnt ? c : cnt - 1;  // clamped: every load stays in the tile
     const int64_t ge = tile.gbase + ce;
     const double* ut = a.u + tile.vbase;
-    // (1) the tile's u into LDS
-    if constexpr (STAGE) {
-      wave_sync();  // previous tile's readers of sT are done
-#pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        const int v = lane + 64 * j;
-        if (v < lim) {
-          const int e = v / NP;
-          sT[e * NPU + (v - e * NP)] = ut[v];
-        }
-      }
-    }
     const int code = (a.codes[ge] >> (5 * g)) & 31;
     const int32_t nb = a.nbr[(int64_t)g * K + ge];
     double fa, fb;
     dgk::face_factors(g, a.bgeo[ge], a.bgeo[K + ge], a.bgeo[2 * K + ge], fa, fb);
-    // (2) traces of face g, all loads in flight together
+    // Traces of face g, all loads in flight together.
     const uint8_t* fn = sFN + (g * 24 + code) * NFP;
     const uint8_t* fm = sFM + g * NFP;
-    double uP[NFP], uMg[STAGE ? 1 : NFP];
+    double uP[NFP], uM[NFP];
 #pragma unroll
     for (int q = 0; q < NFP; ++q) {
       uP[q] = a.u[nb + fn[q]];
-      if constexpr (!STAGE) uMg[q] = ut[ce * NP + fm[q]];
+      uM[q] = ut[ce * NP + fm[q]];
     }
-    if constexpr (STAGE) wave_sync();  // sT holds the tile
-    const double* su = sT + ce * NPU;
     // Fscale (n.f(u-) - n.f*), f(u) = u^2/2 (1,1,1), factored; zero past Nfp.
     auto jump = [&](int q) -> double {
       if (q >= NFP) return 0.0;
-      const double uM = STAGE ? su[fm[q]] : uMg[q];
-      const double al = fmax(fabs(uM), fabs(uP[q]));
-      return (uM - uP[q]) * fma(fa, uM + uP[q], -fb * al);
+      const double al = fmax(fabs(uM[q]), fabs(uP[q]));
+      return (uM[q] - uP[q]) * fma(fa, uM[q] + uP[q], -fb * al);
     };
     // B operand of each step in the product's precision: the jumps (k = 4)
     // or their bf16 parts (k = 32), split as soon as each is formed.
@@ -203,23 +177,9 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
 #pragma unroll
       for (int q = 0; q < NPT; ++q) gj[q] = jump(q);
     }
-    if constexpr (STAGE) wave_sync();  // all reads of sT as u are done
     const double* rt = a.rhs + tile.vbase;
     const double* u0t = a.a0 != 0.0 ? a.u0 + tile.vbase : nullptr;
     double* ot = a.out + tile.vbase;
-    // out = a0 u0 + a1 (u + dt (rhs + LIFT g)) (mode 0) or rhs + LIFT g (mode 1)
-    auto update = [&](int64_t idx, double lg) {
-      const double l = rt[idx] + lg;
-      double o;
-      if (a.mode == 0) {
-        o = a.a1 * fma(dt, l, ut[idx]);
-        if (a.a0 != 0.0) o = fma(a.a0, __builtin_nontemporal_load(u0t + idx), o);
-        lmax = fmax(lmax, fabs(o));
-      } else {
-        o = l;
-      }
-      __builtin_nontemporal_store(o, ot + idx);
-    };
     // out from preloaded rhs, u, u0 values
     auto finish = [&](int64_t idx, double lg, double rv, double uv, double u0v) {
       const double l = rv + lg;
@@ -233,25 +193,23 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
       }
       __builtin_nontemporal_store(o, ot + idx);
     };
-    // (3) LIFT g, row tile by row tile (not unrolled without STAGE: the
-    // preloads of every row tile would otherwise be hoisted together)
-#pragma unroll(STAGE ? MTL : 1)
+    // LIFT g, row tile by row tile (not unrolled: the preloads of every row
+    // tile would otherwise be hoisted together).
+#pragma unroll 1
     for (int tt = 0; tt < MTL; ++tt) {
-      // Without STAGE: the rows this lane updates after the product (element
-      // 4 j + lane / 16, row 16 tt + lane % 16), loaded ahead of the product.
+      // The rows this lane updates after the product (element 4 j + lane / 16,
+      // row 16 tt + lane % 16), loaded ahead of the product.
       double pr[4], pu[4], p0[4];
       int64_t pidx[4];
       bool pok[4];
-      if constexpr (!STAGE) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
-          pok[j] = e < cnt && i < NP;
-          pidx[j] = pok[j] ? (int64_t)e * NP + i : 0;
-          pr[j] = rt[pidx[j]];
-          pu[j] = a.mode == 0 ? ut[pidx[j]] : 0.0;
-          p0[j] = a.a0 != 0.0 ? __builtin_nontemporal_load(u0t + pidx[j]) : 0.0;
-        }
+      for (int j = 0; j < 4; ++j) {
+        const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
+        pok[j] = e < cnt && i < NP;
+        pidx[j] = pok[j] ? (int64_t)e * NP + i : 0;
+        pr[j] = rt[pidx[j]];
+        pu[j] = a.mode == 0 ? ut[pidx[j]] : 0.0;
+        p0[j] = a.a0 != 0.0 ? __builtin_nontemporal_load(u0t + pidx[j]) : 0.0;
       }
       AccT<PREC> acc;
 #pragma unroll
@@ -280,35 +238,14 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
           }
         }
       }
-      if constexpr (STAGE) {
-        // park in sT[c][row] for the whole-tile update below
+      // Through the wave's 16 x 17 buffer to rows of 16 consecutive values.
+      wave_sync();  // previous readers of the buffer are done
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 16 * tt + acc_row<PREC>(g, r);
-          if (i < NP) sT[c * NPU + i] = (double)acc[r];
-        }
-      } else {
-        // update rows 16 tt .. 16 tt + 15 of the tile's elements now
-        wave_sync();  // previous readers of the buffer are done
+      for (int r = 0; r < 4; ++r) sT[c * 17 + acc_row<PREC>(g, r)] = (double)acc[r];
+      wave_sync();
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sT[c * 17 + acc_row<PREC>(g, r)] = (double)acc[r];
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (pok[j]) finish(pidx[j], sT[(4 * j + (lane >> 4)) * 17 + (lane & 15)], pr[j], pu[j], p0[j]);
-      }
-    }
-    if constexpr (STAGE) {
-      wave_sync();  // sT holds LIFT g
-      // (4) RK update, unit stride over the tile
-#pragma unroll 7
-      for (int j = 0; j < NV; ++j) {
-        const int v = lane + 64 * j;
-        if (v < lim) {
-          const int e = v / NP;
-          update(v, sT[e * NPU + (v - e * NP)]);
-        }
-      }
+      for (int j = 0; j < 4; ++j)
+        if (pok[j]) finish(pidx[j], sT[(4 * j + (lane >> 4)) * 17 + (lane & 15)], pr[j], pu[j], p0[j]);
     }
   }
   if (a.maxabs) {
