@@ -70,8 +70,9 @@ struct ACfg {
   // SIMD) so that two blocks share a CU and one block's matrix phase
   // overlaps the other's memory phase. FP64 needs ~170 fragment and
   // accumulator registers per matrix wave and runs one block per CU.
-  static constexpr int G = MT < 4 ? MT : (FRAG_REGS / 4 > 100 && PREC != kF64 ? 8 : 4);
-  static constexpr int WPE = PREC == kF64 ? 2 : 4;
+  static constexpr int G =
+      MT < 4 ? MT : ((PREC == kF64 && MT >= 8) || (FRAG_REGS / 4 > 100 && PREC != kF64) ? 8 : 4);
+  static constexpr int WPE = PREC == kF64 && G < 8 ? 2 : 4;
   static constexpr int TPW = (MT + G - 1) / G;
   static constexpr int NT = 128 * G;                     // threads per block
   static constexpr int NTM = 64 * G;                     // memory threads
