@@ -283,13 +283,19 @@ deriv_affine_kernel(AffineArgs a) {
       }
       if constexpr (PREC == kF32) {
         sWf[bu][e * NPF + i] = (float)x;
-      } else if constexpr (is_bf(PREC)) {
+      } else if constexpr (PREC == kBF3) {
+        // two parts carry ~16 bits: the FP32 residual is exact enough
+        const float xf = (float)x;
+        const __bf16 h = (__bf16)xf;
+        sWb[bu][0][e * KPB + i] = h;
+        sWb[bu][1][e * KPB + i] = (__bf16)(xf - (float)h);
+      } else if constexpr (PREC == kBF6) {
         const __bf16 h = (__bf16)(float)x;
         const double r1 = x - (double)(float)h;
         const __bf16 m = (__bf16)(float)r1;
         sWb[bu][0][e * KPB + i] = h;
         sWb[bu][1][e * KPB + i] = m;
-        if constexpr (PREC == kBF6) sWb[bu][2][e * KPB + i] = (__bf16)(float)(r1 - (double)(float)m);
+        sWb[bu][2][e * KPB + i] = (__bf16)(float)(r1 - (double)(float)m);
       }
     }
   };

@@ -166,12 +166,20 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const double x = jump(8 * s + e);
-          const __bf16 h = (__bf16)(float)x;
-          const double r1 = x - (double)(float)h;
-          const __bf16 m = (__bf16)(float)r1;
-          bp[s][0][e] = h;
-          bp[s][1][e] = m;
-          if constexpr (PARTS == 3) bp[s][2][e] = (__bf16)(float)(r1 - (double)(float)m);
+          if constexpr (PARTS == 2) {
+            // two parts carry ~16 bits: the FP32 residual is exact enough
+            const float xf = (float)x;
+            const __bf16 h = (__bf16)xf;
+            bp[s][0][e] = h;
+            bp[s][1][e] = (__bf16)(xf - (float)h);
+          } else {
+            const __bf16 h = (__bf16)(float)x;
+            const double r1 = x - (double)(float)h;
+            const __bf16 m = (__bf16)(float)r1;
+            bp[s][0][e] = h;
+            bp[s][1][e] = m;
+            bp[s][2][e] = (__bf16)(float)(r1 - (double)(float)m);
+          }
         }
     } else {
 #pragma unroll
