@@ -173,6 +173,10 @@ deriv_affine_kernel(AffineArgs a) {
   // r, s, t slopes, padded to 12 doubles so the epilogue reads it with six
   // 16-byte loads.
   __shared__ __attribute__((aligned(16))) double sM[4][16 * 12];
+  // BF16x3: the same per-element values in FP32 for an FP32 epilogue (its
+  // rounding is far below the split's ~1e-5).
+  constexpr bool EPI32 = PREC == kBF3 && !DIV;
+  __shared__ __attribute__((aligned(16))) float sMf[EPI32 ? 4 : 1][EPI32 ? 16 * 12 : 1];
   __shared__ double sLam[MIXED ? 3 * NP : 1];
 
   const int tid = threadIdx.x;
@@ -251,11 +255,16 @@ deriv_affine_kernel(AffineArgs a) {
       }
     }
     if constexpr (NTM >= 9 * 16) {
-      if (mt < 9 * 16) sM[bm][(mt & 15) * 12 + (mt >> 4)] = PM;
+      if (mt < 9 * 16) {
+        sM[bm][(mt & 15) * 12 + (mt >> 4)] = PM;
+        if constexpr (EPI32) sMf[bm][(mt & 15) * 12 + (mt >> 4)] = (float)PM;
+      }
     } else {
       for (int v = mt; v < 9 * 16; v += NTM) {
         const int q = v >> 4, e = v & 15;
-        sM[bm][e * 12 + q] = e < tr.count ? a.geo[q * a.gs + tr.gbase + e] : 0.0;
+        const double mv = e < tr.count ? a.geo[q * a.gs + tr.gbase + e] : 0.0;
+        sM[bm][e * 12 + q] = mv;
+        if constexpr (EPI32) sMf[bm][e * 12 + q] = (float)mv;
       }
     }
     if (n + PD < nb) prefetch(n + PD, P, PM);
@@ -279,6 +288,11 @@ deriv_affine_kernel(AffineArgs a) {
           sM[bl][e * 12 + 9] = 0.5 * a1;
           sM[bl][e * 12 + 10] = 0.5 * a2;
           sM[bl][e * 12 + 11] = 0.5 * a3;
+          if constexpr (EPI32) {
+            sMf[bl][e * 12 + 9] = (float)(0.5 * a1);
+            sMf[bl][e * 12 + 10] = (float)(0.5 * a2);
+            sMf[bl][e * 12 + 11] = (float)(0.5 * a3);
+          }
         }
       }
       if constexpr (PREC == kF32) {
@@ -375,7 +389,21 @@ deriv_affine_kernel(AffineArgs a) {
 #pragma unroll
     for (int q = 0; q < NIT; ++q) {
       const int v = mt + q * NTM;
-      if (v < lim) {
+      if (EPI32 && v < lim) {
+        const int e = v / NP, i = v - e * NP;
+        const R* re = sR[br] + e * ROWSP;
+        const float4* M4 = (const float4*)(sMf[bm] + e * 12);
+        // m0 = (rx, ry, rz, sx), m1 = (sy, sz, tx, ty), m2 = (tz, slopes r, s, t)
+        const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2];
+        const float ur = (float)re[i] + m2.y, us = (float)re[NP + i] + m2.z,
+                    ut = (float)re[2 * NP + i] + m2.w;
+        __builtin_nontemporal_store(fmaf(m1.z, ut, fmaf(m0.w, us, m0.x * ur)),
+                                    (float*)a.ox + tr.vbase + v);
+        __builtin_nontemporal_store(fmaf(m1.w, ut, fmaf(m1.x, us, m0.y * ur)),
+                                    (float*)a.oy + tr.vbase + v);
+        __builtin_nontemporal_store(fmaf(m2.x, ut, fmaf(m1.y, us, m0.z * ur)),
+                                    (float*)a.oz + tr.vbase + v);
+      } else if (v < lim) {
         const int e = v / NP, i = v - e * NP;
         const R* re = sR[br] + e * ROWSP;
         const doublex2* M2 = (const doublex2*)(sM[bm] + e * 12);
