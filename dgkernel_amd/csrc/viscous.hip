@@ -15,22 +15,24 @@
 //      q   = grad u + LIFT(Fscale n (u+ - u-)/2),
 //      L_v = div(eps q) - LIFT(Fscale (n.(s- - s+)/2 + tau (u- - u+))),
 //      s = eps q, tau = tau0 max(eps-, eps+).
-// The viscous stage runs on the split path: volume term of the Burgers
-// operator into the mesh's FP64 rhs buffer, then the gradient kernel (q),
-// the viscous kernel (rhs += L_v), then the surface kernel (LF flux, LIFT,
-// SSP-RK3 update).
+// A viscous stage (Burgers + L_v + SSP-RK3 update) is two launches: a
+// volume kernel (BR1 gradient on matrix cores, kept in registers; the
+// divergence of eps q minus that of u^2/2 (1,1,1); the face traces eps n.q)
+// and a surface kernel (LF and viscous flux jumps, one face per lane; LIFT on
+// matrix cores; the RK update).
 //
-// Kernel shapes. The sensor, gradient and viscous kernels take one element
-// per lane, 64 elements (four 16-element tiles) per single-wave block; the
-// tile's nodal values are staged through LDS with unit-stride loads, and the
-// element operators are read with wave-uniform addresses (scalar loads,
-// SGPR operands of v_fma_f64). The learned sensor works on 16-element tiles
-// per wave on matrix cores: modal transform on split BF16 MFMA (three
-// products), both hidden layers on BF16 MFMA (16x16x32), FP32 accumulation. Hidden
-// activations never leave registers between layers: the k order of each
-// layer's product is permuted so that the FP32 result layout of one layer
-// (lane (c, g) holds rows 4g + r of each 16-row tile) is directly the BF16
-// B operand of the next (lane (c, g) holds k = 8g + j of a 32-wide k step).
+// Kernel shapes. The Persson-Peraire sensor takes one element per lane, 64
+// elements (four 16-element tiles) per single-wave block; the tile's nodal
+// values are staged through LDS with unit-stride loads, and V^-1 is read
+// with wave-uniform addresses (scalar loads, SGPR operands of v_fma_f64).
+// The stage kernels and the learned sensor work on 16-element tiles per wave
+// on matrix cores. The learned sensor: modal transform on split BF16 MFMA
+// (three products), both hidden layers on BF16 MFMA (16x16x32), FP32
+// accumulation. Hidden activations never leave registers between layers:
+// the k order of each layer's product is permuted so that the FP32 result
+// layout of one layer (lane (c, g) holds rows 4g + r of each 16-row tile) is
+// directly the BF16 B operand of the next (lane (c, g) holds k = 8g + j of a
+// 32-wide k step).
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -97,8 +99,8 @@ struct AvArgs {
   const int32_t* nbr;         // [4][K] neighbour value offsets
   const int32_t* nbe;         // [4][K] neighbour element indices
   const int32_t* codes;       // [K]
-  const double* q[3];         // gradient arrays (state layout)
-  double* qo[3];
+  double* vol;                // state layout: volume terms of the stage RHS
+  double* sig;                // [K][4][Nfp]: eps n.q at each own face point
   double* eps;                // [K]
   const double* eps_in;
   double* rhs;                // state layout, += L_v
@@ -229,15 +231,11 @@ __global__ void __launch_bounds__(64) pp_sensor_kernel(AvArgs a) {
 // rows the lane (c, g) holding row i = g + 4q of one block holds row i of
 // every block, for element c.
 //
-// av_grad_mfma_kernel: q_d = grad_d u + sum_f (Fscale_f n_d,f / 2) T_f with
-//   [Dr; Ds; Dt] u (stacked rows, K = Np) and T_f = LIFT_f (u+ - u-) per face
-//   (face-contiguous k with each face padded to a multiple of 4 points, one
-//   accumulator set per face).
-// av_stage_mfma_kernel: one volume product [Dr | Ds | Dt] [S_r; S_s; S_t]
-//   with S_b = eps (b_x q_x + b_y q_y + b_z q_z) - g_b u^2/2 (b = r, s, t;
-//   g_b = b_x + b_y + b_z: the Burgers flux u^2/2 (1,1,1)), and one LIFT
-//   product over face-per-lane k (k = 4 fp + f) of the Lax-Friedrichs jump
-//   plus the viscous flux jump; RK combination in the epilogue.
+// The BR1 gradient is q_d = grad_d u + sum_f (Fscale_f n_d,f / 2) T_f with
+// [Dr; Ds; Dt] u (stacked rows, K = Np) and T_f = LIFT_f (u+ - u-) per face
+// (face-contiguous k with each face padded to a multiple of 4 points, one
+// accumulator set per face); the divergence is one product [Dr | Ds | Dt]
+// [S_r; S_s; S_t]; the surface term one LIFT product over face-per-lane k.
 // ---------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
 
@@ -254,7 +252,6 @@ struct AvShape {
   static constexpr int KSL = NFP;                   // k steps, face-per-lane LIFT
   static constexpr int S = NP + 1;                  // LDS row stride
   static constexpr int NA_G = (MTG * KSG + MTL * 4 * SPF) * 64;
-  static constexpr int NA_S = (MTL * KSD + MTL * KSL) * 64;
 };
 
 // Own volume node of face point fp of face f, fp = f4 + g (clamped).
@@ -267,33 +264,72 @@ __device__ __forceinline__ int face_node(int f, int f4, int g) {
   return g == 0 ? n0 : g == 1 ? n1 : g == 2 ? n2 : n3;
 }
 
-template <int N, int WPB>
-__global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1) av_grad_mfma_kernel(AvArgs a) {
+// MODE 0: out = a0 u0 + a1 (u + dt R); 1: out = a1 (u + dt R); 2: out = R,
+// with R = L_v(u) + (BURG ? L(u) : 0).
+struct StageAv {
+  const double* u0;
+  double* out;
+  double a0, a1;
+  const double* dt;
+  double* maxabs;
+  const double* bgeo;  // [19][K]
+};
+
+// ---------------------------------------------------------------------------
+// Viscous stage in two launches (the gradient never goes to memory):
+// av_vol_kernel: per 16-element tile, the BR1 gradient q on matrix cores
+//   (kept in registers), then
+//     vol = [Dr | Ds | Dt] [S_r; S_s; S_t],  S_b = eps (b.q) - g_b u^2/2 (BURG)
+//   (the FP64 result layout of q -- lane (c, g) holds nodes g + 4 i -- is
+//   exactly the B layout of this product, so q feeds it from registers),
+//   and the face traces sig[f][fp] = eps n_f . q at own face points (q goes
+//   through LDS one component at a time).
+// av_surf_kernel: lane (c, f) gathers face f's own and neighbour traces of u
+//   and sig (all loads in flight together), forms
+//     -Fscale ((sig- + sig+)/2 + tau (u- - u+))  [+ LF jump (BURG)]
+//   (sig+ is the neighbour's own eps n'.q with n' = -n, so the central flux
+//   n.(eps q)- - n.(eps q)+ is sig- + sig+; a boundary face, its own
+//   neighbour, has zero jump), LIFT on matrix cores per 16-row tile, and the
+//   RK update through a 16 x 17 LDS buffer with row-contiguous accesses.
+// ---------------------------------------------------------------------------
+template <int N, int WPB, bool BURG>
+__global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
+    av_vol_kernel(AvArgs a, const double* __restrict__ bgeo) {
   using SH = AvShape<N>;
   constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, NPR = SH::NPR, QN = SH::QN;
   constexpr int MTG = SH::MTG, KSG = SH::KSG, MTL = SH::MTL, SPF = SH::SPF, S = SH::S;
+  constexpr int KSD = SH::KSD;
   constexpr int NT = 64 * WPB;
-  __shared__ __attribute__((aligned(16))) double sA[SH::NA_G];
+  constexpr int NAD = MTL * KSD * 64;  // [Dr | Ds | Dt] fragments
+  constexpr FmaskTable<N> FM{};
+  __shared__ __attribute__((aligned(16))) double sA[SH::NA_G + NAD];
   __shared__ __attribute__((aligned(16))) double sUall[WPB][16 * S];
   __shared__ uint8_t sFN[4 * 24 * NFP];
+  __shared__ uint8_t sFM[NF];
   for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) sFN[i] = a.face_nodes[i];
-  for (int q = threadIdx.x; q < SH::NA_G; q += NT) {
+  for (int i = threadIdx.x; i < NF; i += NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
+  for (int q = threadIdx.x; q < SH::NA_G + NAD; q += NT) {
     const int l = q & 63, ts = q >> 6, cl = l & 15, gl = l >> 4;
     double v = 0.0;
-    if (ts < MTG * KSG) {
+    if (ts < MTG * KSG) {  // [Dr; Ds; Dt] for the gradient
       const int tt = ts / KSG, s = ts % KSG;
       const int row = 16 * tt + cl, k = 4 * s + gl, blk = row / NPR, i = row % NPR;
       if (k < NP && blk < 3 && i < NP) v = (blk == 0 ? a.Dr : blk == 1 ? a.Ds : a.Dt)[i * NP + k];
-    } else {
+    } else if (ts < MTG * KSG + MTL * 4 * SPF) {  // LIFT per face, padded faces
       const int r2 = ts - MTG * KSG, tt = r2 / (4 * SPF), s = r2 % (4 * SPF);
       const int row = 16 * tt + cl, f = s / SPF, fp = 4 * (s % SPF) + gl;
       if (row < NP && fp < NFP) v = a.LIFT[row * NF + f * NFP + fp];
+    } else {  // [Dr | Ds | Dt] for the divergence
+      const int r3 = ts - MTG * KSG - MTL * 4 * SPF, tt = r3 / KSD, s = r3 % KSD;
+      const int row = 16 * tt + cl, k = 4 * s + gl, blk = k / NPR, j = k % NPR;
+      if (row < NP && j < NP) v = (blk == 0 ? a.Dr : blk == 1 ? a.Ds : a.Dt)[row * NP + j];
     }
     sA[q] = v;
   }
   __syncthreads();
   const double* sAv = sA;
   const double* sAl = sA + MTG * KSG * 64;
+  const double* sAd = sA + SH::NA_G;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;
   double* sU = sUall[w];
@@ -315,6 +351,7 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1) av_grad_mfma_kerne
     int32_t nb[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) nb[f] = a.nbr[f * K + ge];
+    // Gradient: [Dr; Ds; Dt] u and LIFT_f (u+ - u-) per face.
     double4v accV[MTG];
 #pragma unroll
     for (int tt = 0; tt < MTG; ++tt) accV[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
@@ -349,164 +386,53 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1) av_grad_mfma_kerne
     double mt[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + ge];
-    double cf[4][3];
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      const double hfs = 0.5 * a.geo[(13 + 4 * f) * K + ge];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) cf[f][d] = hfs * a.geo[(10 + 4 * f + d) * K + ge];
-    }
     double qv[3][QN];
+    {
+      double cf[4][3];
 #pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      const double ur = accV[(4 * q) / 16][((4 * q) % 16) / 4];
-      const double us = accV[(NPR + 4 * q) / 16][((NPR + 4 * q) % 16) / 4];
-      const double ut = accV[(2 * NPR + 4 * q) / 16][((2 * NPR + 4 * q) % 16) / 4];
-      double T[4];
+      for (int f = 0; f < 4; ++f) {
+        const double hfs = 0.5 * a.geo[(13 + 4 * f) * K + ge];
 #pragma unroll
-      for (int f = 0; f < 4; ++f)
-        T[f] = (4 * q) / 16 < MTL ? accF[f][((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double v = mt[d] * ur + mt[3 + d] * us + mt[6 + d] * ut;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) v = fma(cf[f][d], T[f], v);
-        qv[d][q] = v;
+        for (int d = 0; d < 3; ++d) cf[f][d] = hfs * a.geo[(10 + 4 * f + d) * K + ge];
       }
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      wave_fence();
 #pragma unroll
       for (int q = 0; q < QN; ++q) {
-        const int i = g + 4 * q;
-        if (i < NP && c < cnt) sU[c * S + i] = qv[d][q];
+        const double ur = accV[(4 * q) / 16][((4 * q) % 16) / 4];
+        const double us = accV[(NPR + 4 * q) / 16][((NPR + 4 * q) % 16) / 4];
+        const double ut = accV[(2 * NPR + 4 * q) / 16][((2 * NPR + 4 * q) % 16) / 4];
+        double T[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+          T[f] = (4 * q) / 16 < MTL ? accF[f][((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double v = mt[d] * ur + mt[3 + d] * us + mt[6 + d] * ut;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) v = fma(cf[f][d], T[f], v);
+          qv[d][q] = v;
+        }
       }
-      wave_fence();
-      for (int i = lane; i < cnt * NP; i += 64) {
-        const int e = i / NP, n = i - e * NP;
-        a.qo[d][tl.vbase + i] = sU[e * S + n];
-      }
     }
-  }
-}
-
-// MODE 0: out = a0 u0 + a1 (u + dt R); 1: out = a1 (u + dt R); 2: out = R,
-// with R = L_v(u) + (BURG ? L(u) : 0).
-struct StageAv {
-  const double* u0;
-  double* out;
-  double a0, a1;
-  const double* dt;
-  double* maxabs;
-  const double* bgeo;  // [19][K]
-};
-
-template <int N, int WPB, int MODE, bool BURG>
-__global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
-    av_stage_mfma_kernel(AvArgs a, StageAv o) {
-  using SH = AvShape<N>;
-  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, NPR = SH::NPR, QN = SH::QN;
-  constexpr int MTL = SH::MTL, KSD = SH::KSD, KSL = SH::KSL, S = SH::S;
-  constexpr int NT = 64 * WPB;
-  __shared__ __attribute__((aligned(16))) double sA[SH::NA_S];
-  __shared__ __attribute__((aligned(16))) double sUall[WPB][16 * S];
-  __shared__ uint8_t sFN[4 * 24 * NFP];
-  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) sFN[i] = a.face_nodes[i];
-  for (int q = threadIdx.x; q < SH::NA_S; q += NT) {
-    const int l = q & 63, ts = q >> 6, cl = l & 15, gl = l >> 4;
-    double v = 0.0;
-    if (ts < MTL * KSD) {
-      const int tt = ts / KSD, s = ts % KSD;
-      const int row = 16 * tt + cl, k = 4 * s + gl, blk = k / NPR, j = k % NPR;
-      if (row < NP && j < NP) v = (blk == 0 ? a.Dr : blk == 1 ? a.Ds : a.Dt)[row * NP + j];
-    } else {
-      const int r2 = ts - MTL * KSD, tt = r2 / KSL, s = r2 % KSL;
-      const int row = 16 * tt + cl;
-      if (row < NP) v = a.LIFT[row * NF + gl * NFP + s];
-    }
-    sA[q] = v;
-  }
-  __syncthreads();
-  const double* sAd = sA;
-  const double* sAl = sA + MTL * KSD * 64;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int c = lane & 15, g = lane >> 4;
-  double* sU = sUall[w];
-  const int64_t K = a.K;
-  const double dt = MODE == 2 ? 0.0 : *o.dt;
-  double lmax = 0.0;
-  const int nw = gridDim.x * WPB;
-  for (int t = blockIdx.x * WPB + w; t < a.ntiles; t += nw) {
-    const Tile tl = a.tiles[t];
-    const int cnt = tl.count;
-    wave_fence();
-    for (int i = lane; i < cnt * NP; i += 64) {
-      const int e = i / NP, n = i - e * NP;
-      sU[e * S + n] = a.u[tl.vbase + i];
-    }
-    wave_fence();
-    const int ec = min(c, cnt - 1);
-    const int64_t ge = (int64_t)tl.gbase + ec;
-    const int64_t own = tl.vbase + (int64_t)ec * NP;  // this element's node 0 in state arrays
-    const double* su = sU + ec * S;
+    // Volume: [Dr | Ds | Dt] [S_r; S_s; S_t], B from registers.
     const double ee = a.eps_in[ge];
-    // Surface: face g of element c, point s per k step.
-    double4v accL[MTL];
-#pragma unroll
-    for (int tt = 0; tt < MTL; ++tt) accL[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
-    {
-      const double fs = a.geo[(13 + 4 * g) * K + ge];
-      const double nx = a.geo[(10 + 4 * g) * K + ge], ny = a.geo[(11 + 4 * g) * K + ge],
-                   nz = a.geo[(12 + 4 * g) * K + ge];
-      const int code = (a.codes[ge] >> (5 * g)) & 31;
-      const int32_t nb = a.nbr[g * K + ge];
-      const double ep = a.eps_in[a.nbe[g * K + ge]];
-      const double tau = a.tau0 * fmax(ee, ep);
-      double fa = 0.0, fb = 0.0;
-      if (BURG)
-        dgk::face_factors(g, o.bgeo[0 * K + ge], o.bgeo[1 * K + ge], o.bgeo[2 * K + ge], fa, fb);
-      const uint8_t* fnt = sFN + (g * 24 + code) * NFP;
-#pragma unroll
-      for (int s = 0; s < KSL; ++s) {
-        const int nm = face_node<N>(g, 4 * (s / 4), s % 4);  // own node of point s of face g
-        const int32_t ip = nb + fnt[s];
-        const double uM = su[nm], uP = a.u[ip];
-        const double sM = ee * (nx * a.q[0][own + nm] + ny * a.q[1][own + nm] + nz * a.q[2][own + nm]);
-        const double sP = ep * (nx * a.q[0][ip] + ny * a.q[1][ip] + nz * a.q[2][ip]);
-        double b = -fs * (0.5 * (sM - sP) + tau * (uM - uP));
-        if (BURG) b = fma(uM - uP, fma(fa, uM + uP, -fb * fmax(fabs(uM), fabs(uP))), b);
-#pragma unroll
-        for (int tt = 0; tt < MTL; ++tt)
-          accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], b,
-                                                           accL[tt], 0, 0, 0);
-      }
-    }
-    // Volume: [Dr | Ds | Dt] [S_r; S_s; S_t].
     double4v accD[MTL];
 #pragma unroll
     for (int tt = 0; tt < MTL; ++tt) accD[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
     {
-      double mt[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) mt[q] = ee * a.geo[q * K + ge];
-      double gb3[3] = {0.0, 0.0, 0.0};
+      double gh[3] = {0.0, 0.0, 0.0};
       if (BURG) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) gb3[d] = 0.5 * o.bgeo[d * K + ge];
+        for (int d = 0; d < 3; ++d) gh[d] = 0.5 * bgeo[d * K + ge];
       }
 #pragma unroll
       for (int s = 0; s < KSD; ++s) {
         const int blk = (4 * s) / NPR;
-        const int j = 4 * s - blk * NPR + g;
+        const int sj = s - blk * (NPR / 4), j = 4 * sj + g;
         double b = 0.0;
         if (j < NP) {
-          const double qx = a.q[0][own + j], qy = a.q[1][own + j], qz = a.q[2][own + j];
-          b = mt[3 * blk] * qx + mt[3 * blk + 1] * qy + mt[3 * blk + 2] * qz;
-          if (BURG) {
-            const double uj = su[j];
-            b = fma(-gb3[blk] * uj, uj, b);
-          }
+          b = ee * (mt[3 * blk] * qv[0][sj] + mt[3 * blk + 1] * qv[1][sj] +
+                    mt[3 * blk + 2] * qv[2][sj]);
+          if (BURG) b = fma(-gh[blk] * su[j], su[j], b);
         }
 #pragma unroll
         for (int tt = 0; tt < MTL; ++tt)
@@ -514,29 +440,161 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
                                                            accD[tt], 0, 0, 0);
       }
     }
-    double vout[QN];
+    // Face traces sig = eps n_g . q at the points of face g of element c.
+    double sg[NFP];
 #pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      const int i = g + 4 * q;
-      const double r = (4 * q) / 16 < MTL ? accD[((4 * q) / 16) % MTL][((4 * q) % 16) / 4] +
-                                                accL[((4 * q) / 16) % MTL][((4 * q) % 16) / 4]
-                                          : 0.0;
-      vout[q] = MODE == 2 ? r : fma(dt, r, su[i < NP ? i : NP - 1]);
+    for (int fp = 0; fp < NFP; ++fp) sg[fp] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      wave_fence();  // previous readers of sU are done
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const int i = g + 4 * q;
+        if (i < NP) sU[c * S + i] = qv[d][q];
+      }
+      wave_fence();
+      const double nd = a.geo[(10 + 4 * g + d) * K + ge];
+#pragma unroll
+      for (int fp = 0; fp < NFP; ++fp) sg[fp] = fma(nd, su[sFM[g * NFP + fp]], sg[fp]);
     }
+    if (c < cnt) {
+      double* so = a.sig + (ge * 4 + g) * NFP;
+#pragma unroll
+      for (int fp = 0; fp < NFP; ++fp) so[fp] = ee * sg[fp];
+    }
+    // vol through sU, unit-stride stores.
     wave_fence();
 #pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      const int i = g + 4 * q;
-      if (i < NP && c < cnt) sU[c * S + i] = vout[q];
-    }
+    for (int tt = 0; tt < MTL; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * tt + g + 4 * r;
+        if (i < NP) sU[c * S + i] = accD[tt][r];
+      }
     wave_fence();
     for (int i = lane; i < cnt * NP; i += 64) {
       const int e = i / NP, n = i - e * NP;
-      double v = sU[e * S + n];
-      if (MODE == 0) v = fma(o.a0, o.u0[tl.vbase + i], o.a1 * v);
-      if (MODE == 1) v = o.a1 * v;
-      o.out[tl.vbase + i] = v;
-      lmax = fmax(lmax, fabs(v));
+      a.vol[tl.vbase + i] = sU[e * S + n];
+    }
+  }
+}
+
+template <int N, int WPB, int MODE, bool BURG>
+__global__ void __launch_bounds__(64 * WPB) av_surf_kernel(AvArgs a, StageAv o) {
+  using SH = AvShape<N>;
+  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, MTL = SH::MTL, KSL = SH::KSL;
+  constexpr int NT = 64 * WPB;
+  constexpr FmaskTable<N> FM{};
+  __shared__ __attribute__((aligned(16))) double sA[MTL * KSL * 64];
+  __shared__ double sBuf[WPB][16 * 17];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  __shared__ uint8_t sFP[4 * 24 * NFP];  // neighbour's face point of each (face, code, point)
+  __shared__ uint8_t sFM[NF];
+  for (int i = threadIdx.x; i < NF; i += NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) {
+    const int nn = a.face_nodes[i];
+    sFN[i] = (uint8_t)nn;
+    const int f2 = ((i / NFP) % 24) / 6;
+    int fp2 = 0;
+    for (int q = 0; q < NFP; ++q)
+      if (FM.v[f2][q] == nn) fp2 = q;
+    sFP[i] = (uint8_t)fp2;
+  }
+  for (int q = threadIdx.x; q < MTL * KSL * 64; q += NT) {
+    const int l = q & 63, ts = q >> 6, cl = l & 15, gl = l >> 4;
+    const int tt = ts / KSL, s = ts % KSL, row = 16 * tt + cl;
+    sA[q] = row < NP ? a.LIFT[row * NF + gl * NFP + s] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;  // element of the tile, face
+  double* sB = sBuf[w];
+  const int64_t K = a.K;
+  const double dt = MODE == 2 ? 0.0 : *o.dt;
+  double lmax = 0.0;
+  // XCD-contiguous tile ranges (see the split surface kernel).
+  int t0 = blockIdx.x * WPB + w, t1 = a.ntiles, ts = gridDim.x * WPB;
+  if (gridDim.x % 8 == 0) {
+    const int x = blockIdx.x % 8;
+    t0 = (int)((int64_t)a.ntiles * x / 8) + (blockIdx.x / 8) * WPB + w;
+    t1 = (int)((int64_t)a.ntiles * (x + 1) / 8);
+    ts = (gridDim.x / 8) * WPB;
+  }
+  for (int t = t0; t < t1; t += ts) {
+    const Tile tl = a.tiles[t];
+    const int cnt = tl.count;
+    const int ce = c < cnt ? c : cnt - 1;
+    const int64_t ge = (int64_t)tl.gbase + ce;
+    const double* ut = a.u + tl.vbase;
+    const int code = (a.codes[ge] >> (5 * g)) & 31, f2 = code / 6;
+    const int32_t nb = a.nbr[(int64_t)g * K + ge];
+    const int32_t ne = a.nbe[(int64_t)g * K + ge];
+    const bool bnd = ne == ge && f2 == g;  // boundary face: its own neighbour
+    const double ee = a.eps_in[ge], ep = a.eps_in[ne];
+    const double tau = a.tau0 * fmax(ee, ep);
+    const double fs = a.geo[(13 + 4 * g) * K + ge];
+    double fa = 0.0, fb = 0.0;
+    if (BURG) dgk::face_factors(g, o.bgeo[ge], o.bgeo[K + ge], o.bgeo[2 * K + ge], fa, fb);
+    const uint8_t* fn = sFN + (g * 24 + code) * NFP;
+    const uint8_t* fq = sFP + (g * 24 + code) * NFP;
+    const uint8_t* fm = sFM + g * NFP;
+    const double* sgM = a.sig + (ge * 4 + g) * NFP;
+    const double* sgP = a.sig + ((int64_t)ne * 4 + f2) * NFP;
+    double uM[NFP], uP[NFP], sM[NFP], sP[NFP];
+#pragma unroll
+    for (int q = 0; q < NFP; ++q) {
+      uM[q] = ut[ce * NP + fm[q]];
+      uP[q] = a.u[nb + fn[q]];
+      sM[q] = sgM[q];
+      sP[q] = sgP[fq[q]];
+    }
+    double bj[NFP];
+#pragma unroll
+    for (int q = 0; q < NFP; ++q) {
+      const double sn = bnd ? -sM[q] : sP[q];
+      double b = -fs * (0.5 * (sM[q] + sn) + tau * (uM[q] - uP[q]));
+      if (BURG)
+        b = fma(uM[q] - uP[q], fma(fa, uM[q] + uP[q], -fb * fmax(fabs(uM[q]), fabs(uP[q]))), b);
+      bj[q] = b;
+    }
+    // LIFT per 16-row tile; the rows this lane updates are loaded first.
+#pragma unroll 1
+    for (int tt = 0; tt < MTL; ++tt) {
+      double pv[4], pu[4], p0[4];
+      int64_t pidx[4];
+      bool pok[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
+        pok[j] = e < cnt && i < NP;
+        pidx[j] = tl.vbase + (pok[j] ? (int64_t)e * NP + i : 0);
+        pv[j] = a.vol[pidx[j]];
+        pu[j] = MODE == 2 ? 0.0 : a.u[pidx[j]];
+        p0[j] = MODE == 0 ? __builtin_nontemporal_load(o.u0 + pidx[j]) : 0.0;
+      }
+      double4v acc = (double4v){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KSL; ++s)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[(tt * KSL + s) * 64 + lane], bj[s], acc, 0,
+                                                   0, 0);
+      wave_fence();  // previous readers of the buffer are done
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sB[c * 17 + g + 4 * r] = acc[r];
+      wave_fence();
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (!pok[j]) continue;
+        const double r = pv[j] + sB[(4 * j + (lane >> 4)) * 17 + (lane & 15)];
+        double v;
+        if (MODE == 2) {
+          v = r;
+        } else {
+          v = o.a1 * fma(dt, r, pu[j]);
+          if (MODE == 0) v = fma(o.a0, p0[j], v);
+          lmax = fmax(lmax, fabs(v));
+        }
+        o.out[pidx[j]] = v;
+      }
     }
   }
   if (MODE != 2 && o.maxabs) {
@@ -946,7 +1004,8 @@ struct dgk_av_s {
   dgk_dgmesh m = nullptr;
   double* Vinv = nullptr;
   double* eps = nullptr;
-  double* q[3] = {nullptr, nullptr, nullptr};
+  double* vol = nullptr;  // volume terms of the viscous stage RHS (state layout)
+  double* sig = nullptr;  // [K][4][Nfp] viscous face traces
   double* zero_eps = nullptr;
   dgk_av_params p{};
 };
@@ -982,10 +1041,8 @@ AvArgs av_args(dgk_av a) {
   r.codes = m->codes;
   r.eps = a->eps;
   r.eps_in = a->eps;
-  for (int d = 0; d < 3; ++d) {
-    r.q[d] = a->q[d];
-    r.qo[d] = a->q[d];
-  }
+  r.vol = a->vol;
+  r.sig = a->sig;
   r.s0 = a->p.s0;
   r.kappa = a->p.kappa;
   r.ceps = a->p.ceps;
@@ -1009,59 +1066,59 @@ int groups_grid(dgk_dgmesh m, int per_cu) {
       return DGK_ERR_UNSUPPORTED;                                                    \
   }
 
-// Waves per block: at N = 4 the operator fragments take 57-65 KiB of LDS, so
-// one block of 8 waves (two per SIMD) shares them; lower orders use 4.
+// Waves per block of the volume kernel: at N = 4 its operator fragments take
+// 98 KiB of LDS, so one block of 8 waves (two per SIMD) shares them; lower
+// orders use 4 waves and two blocks per CU.
 template <int N>
 constexpr int av_wpb() { return N >= 4 ? 8 : 4; }
+constexpr int kSurfW = 12;  // waves per block of the surface kernel (three per SIMD)
 
 int av_blocks(dgk_dgmesh m) {
   const int wpb = m->N >= 4 ? 8 : 4;
   return std::max(1, std::min((m->ntiles64 + wpb - 1) / wpb, m->dev->cu_count * (m->N >= 4 ? 1 : 2)));
 }
 
-// q = grad u + LIFT(...) into a->q.
-int av_grad(dgk_av a, const double* u) {
-  dgk_dgmesh m = a->m;
-  if (m->ntiles64 == 0) return DGK_OK;
-  AvArgs r = av_args(a);
-  r.u = u;
-  const dim3 grid(av_blocks(m));
-  switch (m->N) {
-    case 1: hipLaunchKernelGGL((av_grad_mfma_kernel<1, av_wpb<1>()>), grid, dim3(64 * av_wpb<1>()), 0, m->dev->stream, r); break;
-    case 2: hipLaunchKernelGGL((av_grad_mfma_kernel<2, av_wpb<2>()>), grid, dim3(64 * av_wpb<2>()), 0, m->dev->stream, r); break;
-    case 3: hipLaunchKernelGGL((av_grad_mfma_kernel<3, av_wpb<3>()>), grid, dim3(64 * av_wpb<3>()), 0, m->dev->stream, r); break;
-    case 4: hipLaunchKernelGGL((av_grad_mfma_kernel<4, av_wpb<4>()>), grid, dim3(64 * av_wpb<4>()), 0, m->dev->stream, r); break;
-    default:
-      dgk::set_error("shock capturing is built for N <= 4 (N=%d)", m->N);
-      return DGK_ERR_UNSUPPORTED;
-  }
+template <int N, bool BURG>
+int av_vol_launch(dgk_dgmesh m, const AvArgs& r) {
+  hipLaunchKernelGGL((av_vol_kernel<N, av_wpb<N>(), BURG>), dim3(av_blocks(m)),
+                     dim3(64 * av_wpb<N>()), 0, m->dev->stream, r, (const double*)m->bgeo);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
+// Persistent grid of the surface kernel: resident blocks per CU times the
+// CU count (a multiple of 8, for the XCD-contiguous tile order).
 template <int N, int MODE, bool BURG>
-int av_stage_launch(dgk_av a, const AvArgs& r, const StageAv& o) {
-  dgk_dgmesh m = a->m;
-  hipLaunchKernelGGL((av_stage_mfma_kernel<N, av_wpb<N>(), MODE, BURG>), dim3(av_blocks(m)),
-                     dim3(64 * av_wpb<N>()), 0, m->dev->stream, r, o);
+int av_surf_launch(dgk_dgmesh m, const AvArgs& r, const StageAv& o) {
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    DGK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, av_surf_kernel<N, kSurfW, MODE, BURG>, 64 * kSurfW, 0));
+    if (per_cu < 1) per_cu = 1;
+  }
+  const int grid = std::max(1, std::min((m->ntiles64 + kSurfW - 1) / kSurfW, m->dev->cu_count * per_cu));
+  hipLaunchKernelGGL((av_surf_kernel<N, kSurfW, MODE, BURG>), dim3(grid), dim3(64 * kSurfW), 0,
+                     m->dev->stream, r, o);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
 template <int N>
-int av_stage_n(dgk_av a, const AvArgs& r, const StageAv& o, int mode, bool burg) {
-  if (mode == 2) return burg ? av_stage_launch<N, 2, true>(a, r, o) : av_stage_launch<N, 2, false>(a, r, o);
-  if (mode == 0) return av_stage_launch<N, 0, true>(a, r, o);
-  return av_stage_launch<N, 1, true>(a, r, o);
+int av_run_n(dgk_dgmesh m, const AvArgs& r, const StageAv& o, int mode, bool burg) {
+  int rc = burg ? av_vol_launch<N, true>(m, r) : av_vol_launch<N, false>(m, r);
+  if (rc != DGK_OK) return rc;
+  if (mode == 2)
+    return burg ? av_surf_launch<N, 2, true>(m, r, o) : av_surf_launch<N, 2, false>(m, r, o);
+  if (mode == 0) return av_surf_launch<N, 0, true>(m, r, o);
+  return av_surf_launch<N, 1, true>(m, r, o);
 }
 
-// grad, then the fused kernel: mode 0/1 stage (a0 != 0 / a0 == 0), mode 2 rhs.
+// Volume kernel, then the surface kernel: mode 0/1 stage (a0 != 0 / a0 == 0),
+// mode 2 rhs.
 int av_run(dgk_av a, const double* u, const double* u0, double* out, double a0, double a1,
            const double* dt, double* maxabs, int mode, bool burg) {
   dgk_dgmesh m = a->m;
   if (m->ntiles64 == 0) return DGK_OK;
-  int rc = av_grad(a, u);
-  if (rc != DGK_OK) return rc;
   AvArgs r = av_args(a);
   r.u = u;
   StageAv o;
@@ -1073,10 +1130,10 @@ int av_run(dgk_av a, const double* u, const double* u0, double* out, double a0, 
   o.maxabs = maxabs;
   o.bgeo = m->bgeo;
   switch (m->N) {
-    case 1: return av_stage_n<1>(a, r, o, mode, burg);
-    case 2: return av_stage_n<2>(a, r, o, mode, burg);
-    case 3: return av_stage_n<3>(a, r, o, mode, burg);
-    case 4: return av_stage_n<4>(a, r, o, mode, burg);
+    case 1: return av_run_n<1>(m, r, o, mode, burg);
+    case 2: return av_run_n<2>(m, r, o, mode, burg);
+    case 3: return av_run_n<3>(m, r, o, mode, burg);
+    case 4: return av_run_n<4>(m, r, o, mode, burg);
   }
   dgk::set_error("shock capturing is built for N <= 4 (N=%d)", m->N);
   return DGK_ERR_UNSUPPORTED;
@@ -1099,7 +1156,8 @@ extern "C" int dgk_av_create(dgk_dgmesh m, const double* Vinv_host, const dgk_av
   int rc = DGK_OK;
   rc |= dgk_malloc(m->dev, (size_t)m->Np * m->Np * 8, Vinv_host, (void**)&a->Vinv);
   rc |= dgk_malloc(m->dev, (size_t)m->K * 8 + 8, nullptr, (void**)&a->eps);
-  for (int d = 0; d < 3; ++d) rc |= dgk_malloc(m->dev, nstate, nullptr, (void**)&a->q[d]);
+  rc |= dgk_malloc(m->dev, nstate, nullptr, (void**)&a->vol);
+  rc |= dgk_malloc(m->dev, (size_t)m->K * 4 * m->Nfp * 8 + 8, nullptr, (void**)&a->sig);
   if (rc == DGK_OK) rc = dgk_fill_bytes(m->dev, a->eps, 0, (size_t)m->K * 8 + 8);
   if (rc != DGK_OK) {
     dgk_av_destroy(a);
@@ -1114,7 +1172,8 @@ extern "C" int dgk_av_destroy(dgk_av a) {
   dgk_device dev = a->m->dev;
   dgk_free(dev, a->Vinv);
   dgk_free(dev, a->eps);
-  for (int d = 0; d < 3; ++d) dgk_free(dev, a->q[d]);
+  dgk_free(dev, a->vol);
+  dgk_free(dev, a->sig);
   delete a;
   return DGK_OK;
 }
