@@ -71,6 +71,23 @@ def test_viscous_rhs_vs_oracle(device, N, n, parts):
     assert np.max(np.abs(tot - want_t)) <= tol * np.max(np.abs(want_t))
 
 
+@pytest.mark.parametrize("N,parts", [(2, [162]), (4, [60, 102])])
+def test_viscous_rhs_with_boundary_faces_vs_oracle(device, N, parts):
+    """Non-periodic cube: boundary faces are their own neighbours (zero jump
+    in u and in the viscous flux), as in the oracle's VmapP."""
+    m = M.cube_mesh(3, 3, 3, periodic=False, jitter=0.05, seed=30 + N)
+    st = dg.make_setup(m, N, Kp=parts)
+    o = O.Setup.from_mesh(N, m)
+    u = smooth_u0(st, amp=0.8, noise=1e-2)
+    eps = M.uniform(N, 4, m.K, 0.0, 0.02)
+    want = o.av_rhs(u, eps, 1.0)
+    s = AV.AVSolver(device, st, AV.AVParams.for_order(N), "pp")
+    s.set_eps(eps)
+    got = s.rhs_visc(u)
+    tol = 1e-11 + 50 * _spread(o)
+    assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want))
+
+
 @pytest.mark.parametrize("N", [3, 4])
 def test_viscous_ssprk3_vs_oracle(device, N):
     m = M.cube_mesh(4, 4, 4, periodic=True)
