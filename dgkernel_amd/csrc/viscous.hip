@@ -1090,12 +1090,15 @@ int av_vol_launch(dgk_dgmesh m, const AvArgs& r) {
 // CU count (a multiple of 8, for the XCD-contiguous tile order).
 template <int N, int MODE, bool BURG>
 int av_surf_launch(dgk_dgmesh m, const AvArgs& r, const StageAv& o) {
-  static int per_cu = 0;
-  if (per_cu == 0) {
-    DGK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, av_surf_kernel<N, kSurfW, MODE, BURG>, 64 * kSurfW, 0));
-    if (per_cu < 1) per_cu = 1;
-  }
+  // Resident blocks per CU of this instantiation (thread-safe one-time
+  // initialisation; the same for every gfx950 device).
+  static const int per_cu = [] {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, av_surf_kernel<N, kSurfW, MODE, BURG>,
+                                                     64 * kSurfW, 0) != hipSuccess)
+      v = 1;
+    return v < 1 ? 1 : v;
+  }();
   const int grid = std::max(1, std::min((m->ntiles64 + kSurfW - 1) / kSurfW, m->dev->cu_count * per_cu));
   hipLaunchKernelGGL((av_surf_kernel<N, kSurfW, MODE, BURG>), dim3(grid), dim3(64 * kSurfW), 0,
                      m->dev->stream, r, o);
