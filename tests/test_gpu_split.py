@@ -79,8 +79,8 @@ BOUND = {"f32": 5e-5, "bf16x3": 5e-3, "bf16x6": 5e-5}
 
 
 @pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
-def test_p6_mixed_precision_vs_fp64(device, prec):
-    N = 6
+@pytest.mark.parametrize("N", [3, 5, 6])
+def test_mixed_precision_stage_vs_fp64(device, N, prec):
     m = M.cube_mesh(4, 4, 4, periodic=True)
     st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, 2))
     u0 = smooth_u0(st, amp=0.4, noise=1e-6)
@@ -94,7 +94,7 @@ def test_p6_mixed_precision_vs_fp64(device, prec):
     a, b = lo.state(), ref.state()
     e2 = rel_l2(a, b)
     emax = float(np.max(np.abs(a - b)))
-    print(f"P=6 {prec} 10 steps: rel L2 {e2:.3e}, max {emax:.3e}")
+    print(f"P={N} {prec} 10 steps: rel L2 {e2:.3e}, max {emax:.3e}")
     assert 0.0 < e2 <= BOUND[prec]
 
 
