@@ -52,6 +52,13 @@ struct SCfg {
   // Waves per block: three per SIMD (four for BF16x3, which needs fewer
   // registers) to cover the latency of the trace gathers.
   static constexpr int W = PREC == kBF3 ? 16 : 12;
+  // Per-wave LDS buffer: a quarter tile (4 elements, row stride NPH) for the
+  // own traces, reused as the 16 x 17 update buffer.
+  static constexpr int NPH = NP | 1;
+  static constexpr int BUF = 4 * NPH > 16 * 17 ? 4 * NPH : 16 * 17;
+  static constexpr int NVH = (4 * NP + 63) / 64;  // quarter-tile values per lane
+  // Own traces staged through LDS (unit-stride loads) or gathered per lane.
+  static constexpr bool STAGE = PREC == kF32;
 };
 
 struct SurfArgs {
@@ -90,9 +97,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per tile, each wave: lane (c, g) loads the own and neighbour traces of
-// face g (all loads in flight together) and forms the LF jumps as its B
-// operand; then, per 16-row tile of LIFT: the rhs, u, u0 values this lane
+// Per tile, each wave: lane (c, g) gathers the neighbour trace of face g
+// (all loads in flight together), reads its own trace from the tile staged
+// in LDS a quarter at a time with unit-stride loads, and forms the LF jumps as its
+// B operand; then, per 16-row tile of LIFT: the rhs, u, u0 values this lane
 // will update are loaded, the product runs on matrix cores, and the result
 // goes through the wave's 16 x 17 LDS buffer so that each lane updates and
 // stores rows of 16 consecutive values.
@@ -100,10 +108,11 @@ template <int N, int PREC>
 __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel(SurfArgs a) {
   using C = SCfg<N, PREC>;
   constexpr int NP = C::NP, NFP = C::NFP, NF = C::NF, MTL = C::MTL, KSL = C::KSL, W = C::W;
-  constexpr int NPT = C::NPT, PARTS = C::PARTS;
+  constexpr int NPT = C::NPT, PARTS = C::PARTS, NPH = C::NPH, NVH = C::NVH;
+  constexpr bool STAGE = C::STAGE;
   constexpr dgk::FmaskTable<N> FM{};
   __shared__ __attribute__((aligned(16))) unsigned char sA[C::FRAG_BYTES];
-  __shared__ double sTall[W][16 * 17];
+  __shared__ double sTall[W][C::BUF];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[NF];
   {
@@ -145,10 +154,31 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
     const uint8_t* fn = sFN + (g * 24 + code) * NFP;
     const uint8_t* fm = sFM + g * NFP;
     double uP[NFP], uM[NFP];
+    // Own traces through LDS, a quarter tile at a time: unit-stride loads of
+    // elements 4 h .. 4 h + 3, then the lanes of those elements read theirs.
+#pragma unroll 1
+    for (int h = 0; h < (STAGE ? 4 : 0); ++h) {
+      const int lim = min(cnt, 4 * h + 4) * NP - 4 * h * NP;
+      wave_sync();  // previous readers of the buffer are done
+#pragma unroll
+      for (int j = 0; j < NVH; ++j) {
+        const int v = lane + 64 * j;
+        if (v < lim) {
+          const int e = v / NP;
+          sT[e * NPH + (v - e * NP)] = ut[4 * h * NP + v];
+        }
+      }
+      wave_sync();
+      if ((ce >> 2) == h) {
+        const double* se = sT + (ce - 4 * h) * NPH;
+#pragma unroll
+        for (int q = 0; q < NFP; ++q) uM[q] = se[fm[q]];
+      }
+    }
 #pragma unroll
     for (int q = 0; q < NFP; ++q) {
       uP[q] = a.u[nb + fn[q]];
-      uM[q] = ut[ce * NP + fm[q]];
+      if constexpr (!STAGE) uM[q] = ut[ce * NP + fm[q]];
     }
     // Fscale (n.f(u-) - n.f*), f(u) = u^2/2 (1,1,1), factored; zero past Nfp.
     auto jump = [&](int q) -> double {
