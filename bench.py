@@ -263,7 +263,10 @@ def bench_deriv(a, dev) -> dict:
     return out
 
 
-PREC_NAMES = ("f64", "f32", "bf16x3", "bf16x6")
+# Stage precision settings: Dr/Ds/Dt (and LIFT) precision; "_lift64" keeps
+# the LIFT product in FP64.
+STAGE_PRECS = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3,
+               "f32_lift64": 1 | 16, "bf16x3_lift64": 2 | 16, "bf16x6_lift64": 3 | 16}
 
 
 def bench_c4_stage(a, dev) -> dict:
@@ -294,11 +297,11 @@ def bench_c4_stage(a, dev) -> dict:
            "precisions": {}}
     solver = None
     hmin = None
-    precs = [p for p in PREC_NAMES if not a.precs or p == "f64" or p in a.precs.split(",")]
+    precs = [p for p in STAGE_PRECS if not a.precs or p == "f64" or p in a.precs.split(",")]
     for name in precs:
         solver = dg.BurgersSolver(dev, st, cfl=0.3, hmin=hmin)
         hmin = solver.hmin
-        solver.mesh.set_precision(PREC_NAMES.index(name))
+        solver.mesh.set_precision(STAGE_PRECS[name])
         solver.set_state(u0)
         solver.step(1)  # warm-up outside the timed run, restarted below
         solver.set_state(u0)

@@ -1487,10 +1487,13 @@ extern "C" int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int block
 extern "C" int dgk_burgers_set_precision(dgk_dgmesh m, int prec) {
   dgk::clear_error();
   DGK_REQUIRE(m, "dgk_burgers_set_precision: NULL mesh");
-  DGK_REQUIRE(prec >= 0 && prec <= 3, "dgk_burgers_set_precision: precision %d", prec);
-  if (prec != m->prec) {
+  const int vol = prec & ~DGK_PREC_LIFT_F64;
+  DGK_REQUIRE(vol >= 0 && vol <= 3, "dgk_burgers_set_precision: precision %d", prec);
+  const int lift = (prec & DGK_PREC_LIFT_F64) ? 0 : vol;
+  if (vol != m->prec || lift != m->lift_prec) {
     dgk::split_release(m);
-    m->prec = prec;
+    m->prec = vol;
+    m->lift_prec = lift;
   }
   return use_split(m, 0) ? dgk::split_prepare(m) : DGK_OK;
 }

@@ -86,10 +86,11 @@ struct dgk_dgmesh_s {
   std::vector<double> Dr_h, Ds_h, Dt_h, LIFT_h, r_h, s_h, t_h;
   // Split stage (orders above 4, or contractions below FP64): volume term
   // through a derivative plan into rhs, then flux + LIFT + RK update.
-  int prec = 0;                          // DGK_PREC_* of the D and LIFT contractions
+  int prec = 0;                          // DGK_PREC_* of the D contractions
+  int lift_prec = 0;                     // DGK_PREC_* of the LIFT contraction
   dgk_deriv_plan plan = nullptr;         // built on first use for (N, prec)
   double* rhs = nullptr;                 // state-sized FP64 volume term
-  void* lift_frag = nullptr;             // LIFT fragments for prec
+  void* lift_frag = nullptr;             // LIFT fragments for lift_prec
   int split_blocks = 0;                  // persistent grid of the surface kernel
 };
 

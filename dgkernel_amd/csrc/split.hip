@@ -377,7 +377,7 @@ int surface_launch(dgk_dgmesh m, const SurfArgs& a) {
 
 template <int N>
 int surface_n(dgk_dgmesh m, const SurfArgs* a) {
-  switch (m->prec) {
+  switch (m->lift_prec) {
     case kF64: return a ? surface_launch<N, kF64>(m, *a) : surface_blocks<N, kF64>(m);
     case kF32: return a ? surface_launch<N, kF32>(m, *a) : surface_blocks<N, kF32>(m);
     case kBF3: return a ? surface_launch<N, kBF3>(m, *a) : surface_blocks<N, kBF3>(m);
@@ -410,7 +410,7 @@ int dgk::split_prepare(dgk_dgmesh m) {
   const size_t state_bytes = (size_t)(m->state_off.empty() ? 0 : m->state_off.back()) * 8 + 16;
   rc = dgk_malloc(m->dev, state_bytes, nullptr, (void**)&m->rhs);
   if (rc != DGK_OK) return rc;
-  const std::vector<unsigned char> frag = lift_fragments(m->LIFT_h, m->N, m->prec);
+  const std::vector<unsigned char> frag = lift_fragments(m->LIFT_h, m->N, m->lift_prec);
   rc = dgk_malloc(m->dev, frag.size(), frag.data(), &m->lift_frag);
   if (rc != DGK_OK) return rc;
   return surface(m, nullptr);

@@ -158,7 +158,8 @@ class DGMesh:
         _lib.call("dgk_burgers_set_tile", self.h, elems_per_block, blocks_per_cu)
 
     def set_precision(self, prec: int) -> None:
-        """DGK_PREC_* of the D and LIFT contractions of the stage (state stays FP64)."""
+        """DGK_PREC_* of the D and LIFT contractions of the stage (state stays FP64);
+        | DGK_PREC_LIFT_F64 keeps the LIFT product in FP64."""
         _lib.call("dgk_burgers_set_precision", self.h, prec)
 
     def set_variant(self, variant: int) -> None:

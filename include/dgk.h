@@ -298,12 +298,15 @@ int dgk_burgers_set_variant(dgk_dgmesh m, int variant);
  * walks the tile list with a grid stride. */
 int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked);
 /* Precision of the Dr/Ds/Dt and LIFT contractions in the stage (DGK_PREC_*,
- * default DGK_PREC_F64). The state, geometry, flux, RK combination and dt
- * stay FP64. Below FP64, and for N = 5, 6 at any precision, the stage runs
- * split: a volume kernel (-div f(u) through a derivative plan, with the
- * linear-part removal of dgk_phys_deriv_affine) into a mesh-owned FP64
- * buffer, then a surface kernel (LF flux, LIFT on matrix cores, RK update).
- * The fused kernels (N <= 4, FP64) are used otherwise. */
+ * default DGK_PREC_F64); OR-ing DGK_PREC_LIFT_F64 keeps the LIFT product in
+ * FP64 while Dr/Ds/Dt run in the given precision. The state, geometry, flux,
+ * RK combination and dt stay FP64. Below FP64, and for N = 5, 6 at any
+ * precision, the stage runs split: a volume kernel (-div f(u) through a
+ * derivative plan, with the linear-part removal of dgk_phys_deriv_affine)
+ * into a mesh-owned FP64 buffer, then a surface kernel (LF flux, LIFT on
+ * matrix cores, RK update). The fused kernels (N <= 4, FP64) are used
+ * otherwise. */
+enum { DGK_PREC_LIFT_F64 = 16 };
 int dgk_burgers_set_precision(dgk_dgmesh m, int prec);
 
 /* ---------------------------------------------------------------------- */

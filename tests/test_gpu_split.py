@@ -98,6 +98,30 @@ def test_mixed_precision_stage_vs_fp64(device, N, prec):
     assert 0.0 < e2 <= BOUND[prec]
 
 
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+def test_lift_fp64_flag(device, prec):
+    """DGK_PREC_LIFT_F64: D in the low precision, LIFT in FP64; the state
+    stays within the low-precision bound of the FP64 run and differs from the
+    all-low-precision run."""
+    N = 6
+    m = M.cube_mesh(4, 4, 4, periodic=True)
+    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, 2))
+    u0 = smooth_u0(st, amp=0.4, noise=1e-6)
+    out = {}
+    hmin = None
+    for name, p in (("ref", _lib.DGK_PREC_F64), ("lo", PRECS[prec]),
+                    ("lo_l64", PRECS[prec] | _lib.DGK_PREC_LIFT_F64)):
+        s = dg.BurgersSolver(device, st, hmin=hmin)
+        hmin = s.hmin
+        s.mesh.set_precision(p)
+        s.set_state(u0)
+        s.step(5)
+        out[name] = s.state()
+    e = rel_l2(out["lo_l64"], out["ref"])
+    assert 0.0 < e <= BOUND[prec]
+    assert not np.array_equal(out["lo_l64"], out["lo"])
+
+
 def test_split_graph_replay(device):
     m = M.cube_mesh(4, 4, 4, periodic=True)
     st = dg.make_setup(m, 6, Kp=[200, 184])
