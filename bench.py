@@ -63,13 +63,13 @@ def stage_kernel_name(N: int, solver) -> str:
     return f"burgers_stage_pipe_kernel<{N}, {max(1, (e.get('tile') or 64) // 16)}, MODE, {kind}>"
 
 
-def read_traffic(name: str):
+def read_traffic(name: str, key: str = "traffic_bytes_per_launch"):
     """HBM bytes per launch recorded from rocprofv3 --pmc runs (profiles/<name>)."""
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.strip().startswith("traffic_bytes_per_launch:"):
+        if line.strip().startswith(key + ":"):
             try:
                 return float(line.split(":", 1)[1].strip().split()[0])
             except ValueError:
@@ -438,8 +438,11 @@ def main(argv=None) -> int:
     ap.add_argument("--variant", type=int, default=-1,
                     help="stage kernel (0 auto, 1 VALU, 2 MFMA pipelined, 3 split, 4 MFMA)")
     ap.add_argument("--xcd", type=int, default=-1, help="1: XCD-swept tile order")
-    ap.add_argument("--morton", action="store_true",
-                    help="renumber elements in Morton order within each partition")
+    ap.add_argument("--morton", dest="morton", action="store_true", default=None,
+                    help="renumber elements in Morton order within each partition "
+                         "(default for the P=3 stage, off for --mode c4)")
+    ap.add_argument("--no-morton", dest="morton", action="store_false",
+                    help="keep the generator's element order")
     ap.add_argument("--cpu-n", type=int, default=48, help="hexes per side of the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -490,7 +493,8 @@ def main(argv=None) -> int:
         return 0
 
     t_setup = time.perf_counter()
-    m, st, u0 = build_c3(a.n, a.order, a.parts, a.seed, morton=a.morton)
+    morton = a.morton is None or a.morton
+    m, st, u0 = build_c3(a.n, a.order, a.parts, a.seed, morton=morton)
     log(f"host setup {time.perf_counter() - t_setup:.1f} s, K={m.K}, parts={st.npart}")
     dev = _lib.Device()
     solver = dg.BurgersSolver(dev, st, cfl=0.3)
@@ -562,7 +566,9 @@ def main(argv=None) -> int:
     bpe = stage_bytes_per_elem(Np)
     achieved = bpe * K / (stage_ms.mean() * 1e-3) / 1e9
     flops = stage_flops_per_elem(Np, Nfp) * K / (stage_ms.mean() * 1e-3) / 1e12
-    traffic = read_traffic(f"burgers_stage_p{a.order}_{a.n}cube_pmc.md")
+    traffic = read_traffic(f"burgers_stage_p{a.order}_{a.n}cube_pmc.md",
+                           "traffic_bytes_per_launch" if morton
+                           else "traffic_bytes_per_launch_generator_order")
 
     cpu = None
     if not a.no_cpu:
@@ -586,9 +592,12 @@ def main(argv=None) -> int:
         "dtype": "f64",
         "data": "synthetic (smooth initial field on a generated periodic mesh)",
         "config": {"workload": f"batched Burgers SSP-RK3 stages, {a.n}^3x6={K} tets, P={a.order}, "
-                               f"{st.npart} slab partitions (+-40% imbalance), one persistent launch",
+                               f"{st.npart} slab partitions (+-40% imbalance), one persistent launch"
+                               + (", elements in Morton order within each partition" if morton
+                                  else ""),
                    "tets": K, "order": a.order, "Np": Np, "partitions": st.npart,
-                   "kpart_max": int(st.Kp.max()), "stages_timed": 3 * a.steps},
+                   "kpart_max": int(st.Kp.max()), "stages_timed": 3 * a.steps,
+                   "element_order": "morton" if morton else "generator"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": stage_kernel_name(a.order, solver),

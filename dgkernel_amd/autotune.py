@@ -7,7 +7,7 @@ warm-up launches first) and records the fastest in a JSON table,
 tuned entry for its order; BurgersSolver applies it when the table exists.
 
 Usage: python -m dgkernel_amd.autotune [--orders 1,2,3,4] [--tets 600000]
-                                        [--reps 12] [--out PATH]
+                                        [--reps 12] [--out PATH] [--morton] [--merge]
 
 Candidates (variant numbers of dgk_burgers_set_variant):
   1 VALU stage (one element per lane; 64/128/256 elements per block),
@@ -93,7 +93,8 @@ def time_stage(solver, reps: int) -> float:
     return t.ms() / (3 * reps)
 
 
-def tune(dev, orders, tets: int = 600_000, reps: int = 12, quick: bool = False, log=print) -> dict:
+def tune(dev, orders, tets: int = 600_000, reps: int = 12, quick: bool = False, log=print,
+         morton: bool = False) -> dict:
     from . import dg
     from . import mesh as M
     table = {"device": dev.name, "tets_target": tets, "reps": reps, "entries": {}, "measured": {}}
@@ -101,6 +102,8 @@ def tune(dev, orders, tets: int = 600_000, reps: int = 12, quick: bool = False, 
         n = max(2, int(round((tets / 6) ** (1 / 3))))
         m = M.cube_mesh(n, n, n, periodic=True)
         Kp = M.slab_partition(m.K, 8, 0.4, seed=1)
+        if morton:
+            m = M.morton_within_partitions(m, Kp)
         st = dg.make_setup(m, N, Kp=Kp)
         x, y, z = st.node_coords()
         u0 = 0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
@@ -113,7 +116,8 @@ def tune(dev, orders, tets: int = 600_000, reps: int = 12, quick: bool = False, 
             res.append(dict(cfg, ms=ms))
             log(f"N={N} K={m.K} {cfg}: {ms:.4f} ms/stage")
         best = min(res, key=lambda r: r["ms"])
-        table["entries"][str(N)] = dict(best, K=m.K, dof_stage_per_s=m.K * st.ref.Np / (best["ms"] * 1e-3))
+        table["entries"][str(N)] = dict(best, K=m.K, morton=morton,
+                                        dof_stage_per_s=m.K * st.ref.Np / (best["ms"] * 1e-3))
         table["measured"][str(N)] = res
         del solver
     return table
@@ -126,12 +130,22 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=12)
     ap.add_argument("--quick", action="store_true", help="default launch shapes only")
     ap.add_argument("--out", default=str(TABLE))
+    ap.add_argument("--morton", action="store_true",
+                    help="tune on a mesh renumbered in Morton order within partitions")
+    ap.add_argument("--merge", action="store_true",
+                    help="update only the tuned orders of an existing table at --out")
     a = ap.parse_args(argv)
     from . import _lib
     dev = _lib.Device()
     orders = [int(v) for v in a.orders.split(",") if v]
     table = tune(dev, orders, a.tets, a.reps, a.quick,
-                 log=lambda s: print(s, file=sys.stderr, flush=True))
+                 log=lambda s: print(s, file=sys.stderr, flush=True), morton=a.morton)
+    old = load_table(Path(a.out)) if a.merge else None
+    if old:
+        for key in ("entries", "measured"):
+            old.setdefault(key, {}).update(table[key])
+        old["device"] = table["device"]
+        table = old
     Path(a.out).write_text(json.dumps(table, indent=1) + "\n")
     print(json.dumps({k: v for k, v in table["entries"].items()}), flush=True)
     return 0
