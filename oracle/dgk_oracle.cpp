@@ -866,6 +866,53 @@ void burgersRHS(Solver& S, const double* u, double* rhs) {
   applyPerElement(el.LIFT, S.flux.data(), rhs, K, true);
 }
 
+// The same operator with the reference's OCCA-OpenMP schedule: one loop
+// iteration per partition (@outer -> omp parallel for), a serial element loop
+// inside it (@inner), so at most min(NPART, threads) threads are busy
+// (documentation/DGKernel/0 - Introduction/intro.md:42-45). Volume term,
+// LF flux and LIFT are evaluated element by element. poff[p] .. poff[p+1] are
+// the elements of partition p (contiguous, dense order).
+void burgersRHSPartitions(Solver& S, const double* u, double* rhs, const long* poff, int npart) {
+  const Element& el = S.el;
+  MeshData& m = S.m;
+  const int Np = el.Np, Nfp = el.Nfp, NF = 4 * Nfp;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int p = 0; p < npart; ++p) {
+    std::vector<double> F(Np), fl(NF);
+    for (long k = poff[p]; k < poff[p + 1]; ++k) {
+      const size_t b = (size_t)k * Np;
+      for (int i = 0; i < Np; ++i) F[i] = 0.5 * u[b + i] * u[b + i];
+      for (int i = 0; i < Np; ++i) {
+        double dr = 0.0, ds = 0.0, dt = 0.0;
+        for (int j = 0; j < Np; ++j) {
+          dr += el.Dr(i, j) * F[j];
+          ds += el.Ds(i, j) * F[j];
+          dt += el.Dt(i, j) * F[j];
+        }
+        const size_t q = b + i;
+        const double Fx = m.rx[q] * dr + m.sx[q] * ds + m.tx[q] * dt;
+        const double Fy = m.ry[q] * dr + m.sy[q] * ds + m.ty[q] * dt;
+        const double Fz = m.rz[q] * dr + m.sz[q] * ds + m.tz[q] * dt;
+        rhs[q] = -(Fx + Fy + Fz);
+      }
+      for (int f = 0; f < NF; ++f) {
+        const size_t q = (size_t)k * NF + f;
+        const double uM = u[m.vmapM[q]], uP = u[m.vmapP[q]];
+        const double sn = m.nx[q] + m.ny[q] + m.nz[q];
+        const double fM = 0.5 * uM * uM, fP = 0.5 * uP * uP;
+        const double alpha = std::max(std::fabs(uM), std::fabs(uP));
+        const double fstar = 0.5 * (fM + fP) * sn - 0.5 * alpha * std::fabs(sn) * (uP - uM);
+        fl[f] = m.Fscale[q] * (fM * sn - fstar);
+      }
+      for (int i = 0; i < Np; ++i) {
+        double sum = 0.0;
+        for (int j = 0; j < NF; ++j) sum += el.LIFT(i, j) * fl[j];
+        rhs[b + i] += sum;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Shock capturing (not in the reference; the BASELINE learned-viscosity
 // configuration). Written from:
@@ -1290,6 +1337,38 @@ int orc_ssprk3(void* h, double* u, int nsteps, double cfl, double hmin, double f
 #pragma omp parallel for schedule(static)
     for (long q = 0; q < (long)n; ++q)
       u[q] = (1.0 / 3.0) * u[q] + (2.0 / 3.0) * (u2[q] + dt * L[q]);
+  }
+  return 0;
+}
+
+int orc_burgers_rhs_partitions(void* h, const double* u, double* rhs, const long* poff,
+                               int npart) {
+  burgersRHSPartitions(*static_cast<Solver*>(h), u, rhs, poff, npart);
+  return 0;
+}
+
+// SSP-RK3 as orc_ssprk3 with the partition-parallel schedule of
+// burgersRHSPartitions for the operator and the stage updates.
+int orc_ssprk3_partitions(void* h, double* u, int nsteps, double cfl, double hmin,
+                          const long* poff, int npart, double* dts) {
+  Solver* S = static_cast<Solver*>(h);
+  const int Np = S->el.Np;
+  size_t n = (size_t)Np * S->m.K;
+  Vec u1(n), u2(n), L(n);
+  auto update = [&](auto&& body) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < npart; ++p)
+      for (size_t q = (size_t)poff[p] * Np; q < (size_t)poff[p + 1] * Np; ++q) body(q);
+  };
+  for (int step = 0; step < nsteps; ++step) {
+    const double dt = cfl * hmin / maxAbs(u, n);
+    if (dts) dts[step] = dt;
+    burgersRHSPartitions(*S, u, L.data(), poff, npart);
+    update([&](size_t q) { u1[q] = u[q] + dt * L[q]; });
+    burgersRHSPartitions(*S, u1.data(), L.data(), poff, npart);
+    update([&](size_t q) { u2[q] = 0.75 * u[q] + 0.25 * (u1[q] + dt * L[q]); });
+    burgersRHSPartitions(*S, u2.data(), L.data(), poff, npart);
+    update([&](size_t q) { u[q] = (1.0 / 3.0) * u[q] + (2.0 / 3.0) * (u2[q] + dt * L[q]); });
   }
   return 0;
 }

@@ -34,6 +34,9 @@ def load():
         lib.orc_min_node_distance.argtypes = [C.c_void_p]
         lib.orc_min_node_distance.restype = C.c_double
         lib.orc_ssprk3.argtypes = [C.c_void_p, _D, C.c_int, C.c_double, C.c_double, C.c_double, _D]
+        lib.orc_burgers_rhs_partitions.argtypes = [C.c_void_p, _D, _D, _L, C.c_int]
+        lib.orc_ssprk3_partitions.argtypes = [C.c_void_p, _D, C.c_int, C.c_double, C.c_double,
+                                              _L, C.c_int, _D]
         lib.orc_jacobi_gl.argtypes = [C.c_double, C.c_double, C.c_int, _D]
         lib.orc_jacobi_gq.argtypes = [C.c_double, C.c_double, C.c_int, _D, _D]
         lib.orc_jacobi_p.argtypes = [_D, C.c_int, C.c_double, C.c_double, C.c_int, _D]
@@ -214,6 +217,29 @@ class Setup:
         if hmin is None:
             hmin = self.min_node_distance()
         load().orc_ssprk3(self.h, _d(u), nsteps, cfl, hmin, fixed_dt, _d(dts))
+        return u, dts[:nsteps]
+
+    @staticmethod
+    def _poff(Kp) -> np.ndarray:
+        return np.concatenate([[0], np.cumsum(np.asarray(Kp, dtype=np.int64))]).astype(np.int64)
+
+    def burgers_rhs_partitions(self, u: np.ndarray, Kp) -> np.ndarray:
+        """burgers_rhs with the reference's OCCA-OpenMP schedule (parallel over
+        the contiguous partitions of sizes Kp, serial element loop inside)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(u)
+        poff = self._poff(Kp)
+        load().orc_burgers_rhs_partitions(self.h, _d(u), _d(out),
+                                          poff.ctypes.data_as(C.POINTER(C.c_long)), len(Kp))
+        return out
+
+    def ssprk3_partitions(self, u: np.ndarray, nsteps: int, cfl: float, hmin: float, Kp):
+        """ssprk3 with the partition-parallel schedule."""
+        u = np.array(u, dtype=np.float64, copy=True, order="C")
+        dts = np.zeros(max(nsteps, 1))
+        poff = self._poff(Kp)
+        load().orc_ssprk3_partitions(self.h, _d(u), nsteps, cfl, hmin,
+                                     poff.ctypes.data_as(C.POINTER(C.c_long)), len(Kp), _d(dts))
         return u, dts[:nsteps]
 
     def __del__(self):

@@ -319,6 +319,21 @@ def test_burgers_constant_state_and_conservation():
     assert abs(total) <= 1e-13 * (np.abs(rhs @ e["Mass"].T) * J).sum()
 
 
+def test_partition_schedule_equals_element_schedule():
+    """The reference's OCCA-OpenMP schedule (parallel over partitions, serial
+    element loop) computes the same operator as the element-parallel one, in
+    the same summation order: results are bit-identical, also over SSP-RK3."""
+    m, o, st = _burgers_setup(4, 2)
+    x, y, z = st.node_coords()
+    u = 0.5 + 0.3 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+    Kp = M.slab_partition(m.K, 5, 0.4, seed=3)
+    assert np.array_equal(o.burgers_rhs_partitions(u, Kp), o.burgers_rhs(u))
+    hmin = o.min_node_distance()
+    a, da = o.ssprk3(u, 3, 0.3, hmin)
+    b, db = o.ssprk3_partitions(u, 3, 0.3, hmin, Kp)
+    assert np.array_equal(a, b) and np.array_equal(da, db)
+
+
 def _exact_burgers(x, y, z, t, u0):
     """Characteristic solution u = u0(x - u t (1,1,1)) (before shocks), by Newton."""
     u = u0(x, y, z)
