@@ -457,7 +457,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-morton", dest="morton", action="store_false",
                     help="keep the generator's element order")
     ap.add_argument("--cpu-n", type=int, default=48, help="hexes per side of the CPU sample")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
