@@ -276,6 +276,58 @@ deriv_affine_kernel(AffineArgs a) {
     const double* U = sU[bu];
     constexpr int RW = is_bf(PREC) ? KS * 32 : NP;  // values per element row
     constexpr int NV = 16 * RW;
+    if constexpr (is_bf(PREC)) {
+      // Memory thread (e, t) handles value pairs t + TPE j of element e: the
+      // vertex values are read once, and each part is stored as one 32-bit
+      // word holding two bf16.
+      constexpr int TPE = NTM / 16, NPAIR = RW / 2, NJ = (NPAIR + TPE - 1) / TPE;
+      const int e = mt / TPE, t = mt % TPE;
+      const double* ue = U + e * NPU;
+      const double a0 = ue[a.vtx[0]];
+      const double a1 = ue[a.vtx[1]] - a0, a2 = ue[a.vtx[2]] - a0, a3 = ue[a.vtx[3]] - a0;
+      if (t == 0) {
+        sM[bl][e * 12 + 9] = 0.5 * a1;
+        sM[bl][e * 12 + 10] = 0.5 * a2;
+        sM[bl][e * 12 + 11] = 0.5 * a3;
+        if constexpr (EPI32) {
+          sMf[bl][e * 12 + 9] = (float)(0.5 * a1);
+          sMf[bl][e * 12 + 10] = (float)(0.5 * a2);
+          sMf[bl][e * 12 + 11] = (float)(0.5 * a3);
+        }
+      }
+      auto rem = [&](int i) -> double {
+        return i < NP ? ue[i] - (a0 + a1 * sLam[i] + a2 * sLam[NP + i] + a3 * sLam[2 * NP + i])
+                      : 0.0;
+      };
+      auto bits = [](__bf16 h) -> uint32_t { return (uint32_t)__builtin_bit_cast(uint16_t, h); };
+#pragma unroll 1
+      for (int j = 0; j < NJ; ++j) {
+        const int pr = t + TPE * j;
+        if (NPAIR % TPE == 0 || pr < NPAIR) {
+          const int i0 = 2 * pr;
+          const double x0 = rem(i0), x1 = rem(i0 + 1);
+          uint32_t* w0 = (uint32_t*)(sWb[bu][0] + e * KPB + i0);
+          uint32_t* w1 = (uint32_t*)(sWb[bu][1] + e * KPB + i0);
+          if constexpr (PREC == kBF3) {
+            // two parts carry ~16 bits: the FP32 residual is exact enough
+            const float f0 = (float)x0, f1 = (float)x1;
+            const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
+            *w0 = bits(h0) | (bits(h1) << 16);
+            *w1 = bits((__bf16)(f0 - (float)h0)) | (bits((__bf16)(f1 - (float)h1)) << 16);
+          } else {
+            const __bf16 h0 = (__bf16)(float)x0, h1 = (__bf16)(float)x1;
+            const double r0 = x0 - (double)(float)h0, r1 = x1 - (double)(float)h1;
+            const __bf16 m0 = (__bf16)(float)r0, m1 = (__bf16)(float)r1;
+            uint32_t* w2 = (uint32_t*)(sWb[bu][2] + e * KPB + i0);
+            *w0 = bits(h0) | (bits(h1) << 16);
+            *w1 = bits(m0) | (bits(m1) << 16);
+            *w2 = bits((__bf16)(float)(r0 - (double)(float)m0)) |
+                  (bits((__bf16)(float)(r1 - (double)(float)m1)) << 16);
+          }
+        }
+      }
+      return;
+    }
     for (int v = mt; v < NV; v += NTM) {
       const int e = v / RW, i = v - e * RW;
       double x = 0.0;
