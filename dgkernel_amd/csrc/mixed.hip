@@ -276,11 +276,12 @@ deriv_affine_kernel(AffineArgs a) {
     const double* U = sU[bu];
     constexpr int RW = is_bf(PREC) ? KS * 32 : NP;  // values per element row
     constexpr int NV = 16 * RW;
-    if constexpr (is_bf(PREC)) {
-      // Memory thread (e, t) handles value pairs t + TPE j of element e: the
-      // vertex values are read once, and each part is stored as one 32-bit
-      // word holding two bf16.
-      constexpr int TPE = NTM / 16, NPAIR = RW / 2, NJ = (NPAIR + TPE - 1) / TPE;
+    if constexpr (PREC != kF64) {
+      // Memory thread (e, t) handles values t + TPE j (FP32) or value pairs
+      // (bf16 parts, stored as one 32-bit word each) of element e: the vertex
+      // values are read once per element.
+      constexpr int TPE = NTM / 16;
+      static_assert(NTM % 16 == 0, "memory threads split evenly over 16 elements");
       const int e = mt / TPE, t = mt % TPE;
       const double* ue = U + e * NPU;
       const double a0 = ue[a.vtx[0]];
@@ -299,6 +300,16 @@ deriv_affine_kernel(AffineArgs a) {
         return i < NP ? ue[i] - (a0 + a1 * sLam[i] + a2 * sLam[NP + i] + a3 * sLam[2 * NP + i])
                       : 0.0;
       };
+      if constexpr (PREC == kF32) {
+        constexpr int NJ = (NP + TPE - 1) / TPE;
+#pragma unroll 1
+        for (int j = 0; j < NJ; ++j) {
+          const int i = t + TPE * j;
+          if (NP % TPE == 0 || i < NP) sWf[bu][e * NPF + i] = (float)rem(i);
+        }
+        return;
+      }
+      constexpr int NPAIR = RW / 2, NJ = (NPAIR + TPE - 1) / TPE;
       auto bits = [](__bf16 h) -> uint32_t { return (uint32_t)__builtin_bit_cast(uint16_t, h); };
 #pragma unroll 1
       for (int j = 0; j < NJ; ++j) {
