@@ -173,9 +173,9 @@ deriv_affine_kernel(AffineArgs a) {
   // r, s, t slopes, padded to 12 doubles so the epilogue reads it with six
   // 16-byte loads.
   __shared__ __attribute__((aligned(16))) double sM[4][16 * 12];
-  // BF16x3: the same per-element values in FP32 for an FP32 epilogue (its
-  // rounding is far below the split's ~1e-5).
-  constexpr bool EPI32 = PREC == kBF3 && !DIV;
+  // Derivative below FP64: the same per-element values in FP32 for an FP32
+  // epilogue (the outputs are FP32; its rounding adds a few FP32 ulps).
+  constexpr bool EPI32 = PREC != kF64 && !DIV;
   __shared__ __attribute__((aligned(16))) float sMf[EPI32 ? 4 : 1][EPI32 ? 16 * 12 : 1];
   __shared__ double sLam[MIXED ? 3 * NP : 1];
 
@@ -273,17 +273,14 @@ deriv_affine_kernel(AffineArgs a) {
   // lin: w = u minus the linear interpolant through the vertex nodes, in the
   // matrix precision (FP32, or bf16 parts with zero padding up to KS*32).
   auto lin = [&](int bu, int bl) {
-    const double* U = sU[bu];
-    constexpr int RW = is_bf(PREC) ? KS * 32 : NP;  // values per element row
-    constexpr int NV = 16 * RW;
-    if constexpr (PREC != kF64) {
+    static_assert(NTM % 16 == 0, "memory threads split evenly over 16 elements");
+    if constexpr (MIXED) {
       // Memory thread (e, t) handles values t + TPE j (FP32) or value pairs
-      // (bf16 parts, stored as one 32-bit word each) of element e: the vertex
-      // values are read once per element.
+      // (bf16 parts, stored as one 32-bit word each) of element e, so that the
+      // vertex values are read once per element.
       constexpr int TPE = NTM / 16;
-      static_assert(NTM % 16 == 0, "memory threads split evenly over 16 elements");
       const int e = mt / TPE, t = mt % TPE;
-      const double* ue = U + e * NPU;
+      const double* ue = sU[bu] + e * NPU;
       const double a0 = ue[a.vtx[0]];
       const double a1 = ue[a.vtx[1]] - a0, a2 = ue[a.vtx[2]] - a0, a3 = ue[a.vtx[3]] - a0;
       if (t == 0) {
@@ -302,77 +299,43 @@ deriv_affine_kernel(AffineArgs a) {
       };
       if constexpr (PREC == kF32) {
         constexpr int NJ = (NP + TPE - 1) / TPE;
-#pragma unroll 1
+  #pragma unroll 1
         for (int j = 0; j < NJ; ++j) {
           const int i = t + TPE * j;
           if (NP % TPE == 0 || i < NP) sWf[bu][e * NPF + i] = (float)rem(i);
         }
-        return;
-      }
-      constexpr int NPAIR = RW / 2, NJ = (NPAIR + TPE - 1) / TPE;
-      auto bits = [](__bf16 h) -> uint32_t { return (uint32_t)__builtin_bit_cast(uint16_t, h); };
-#pragma unroll 1
-      for (int j = 0; j < NJ; ++j) {
-        const int pr = t + TPE * j;
-        if (NPAIR % TPE == 0 || pr < NPAIR) {
-          const int i0 = 2 * pr;
-          const double x0 = rem(i0), x1 = rem(i0 + 1);
-          uint32_t* w0 = (uint32_t*)(sWb[bu][0] + e * KPB + i0);
-          uint32_t* w1 = (uint32_t*)(sWb[bu][1] + e * KPB + i0);
-          if constexpr (PREC == kBF3) {
-            // two parts carry ~16 bits: the FP32 residual is exact enough
-            const float f0 = (float)x0, f1 = (float)x1;
-            const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
-            *w0 = bits(h0) | (bits(h1) << 16);
-            *w1 = bits((__bf16)(f0 - (float)h0)) | (bits((__bf16)(f1 - (float)h1)) << 16);
-          } else {
-            const __bf16 h0 = (__bf16)(float)x0, h1 = (__bf16)(float)x1;
-            const double r0 = x0 - (double)(float)h0, r1 = x1 - (double)(float)h1;
-            const __bf16 m0 = (__bf16)(float)r0, m1 = (__bf16)(float)r1;
-            uint32_t* w2 = (uint32_t*)(sWb[bu][2] + e * KPB + i0);
-            *w0 = bits(h0) | (bits(h1) << 16);
-            *w1 = bits(m0) | (bits(m1) << 16);
-            *w2 = bits((__bf16)(float)(r0 - (double)(float)m0)) |
-                  (bits((__bf16)(float)(r1 - (double)(float)m1)) << 16);
+      } else {
+        // bf16 rows are zero-padded up to KS*32 values
+        constexpr int NPAIR = KS * 16, NJ = (NPAIR + TPE - 1) / TPE;
+        auto bits = [](__bf16 h) -> uint32_t { return (uint32_t)__builtin_bit_cast(uint16_t, h); };
+  #pragma unroll 1
+        for (int j = 0; j < NJ; ++j) {
+          const int pr = t + TPE * j;
+          if (NPAIR % TPE == 0 || pr < NPAIR) {
+            const int i0 = 2 * pr;
+            const double x0 = rem(i0), x1 = rem(i0 + 1);
+            uint32_t* w0 = (uint32_t*)(sWb[bu][0] + e * KPB + i0);
+            uint32_t* w1 = (uint32_t*)(sWb[bu][1] + e * KPB + i0);
+            if constexpr (PREC == kBF3) {
+              // two parts carry ~16 bits: the FP32 residual is exact enough
+              const float f0 = (float)x0, f1 = (float)x1;
+              const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
+              *w0 = bits(h0) | (bits(h1) << 16);
+              *w1 = bits((__bf16)(f0 - (float)h0)) | (bits((__bf16)(f1 - (float)h1)) << 16);
+            } else {
+              // three parts of the FP32 value hold it to ~2^-24: each
+              // difference below is exact in FP32
+              const float f0 = (float)x0, f1 = (float)x1;
+              const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
+              const float r0 = f0 - (float)h0, r1 = f1 - (float)h1;
+              const __bf16 m0 = (__bf16)r0, m1 = (__bf16)r1;
+              uint32_t* w2 = (uint32_t*)(sWb[bu][2] + e * KPB + i0);
+              *w0 = bits(h0) | (bits(h1) << 16);
+              *w1 = bits(m0) | (bits(m1) << 16);
+              *w2 = bits((__bf16)(r0 - (float)m0)) | (bits((__bf16)(r1 - (float)m1)) << 16);
+            }
           }
         }
-      }
-      return;
-    }
-    for (int v = mt; v < NV; v += NTM) {
-      const int e = v / RW, i = v - e * RW;
-      double x = 0.0;
-      if (i < NP) {
-        const double* ue = U + e * NPU;
-        const double a0 = ue[a.vtx[0]];
-        const double a1 = ue[a.vtx[1]] - a0, a2 = ue[a.vtx[2]] - a0, a3 = ue[a.vtx[3]] - a0;
-        x = ue[i] - (a0 + a1 * sLam[i] + a2 * sLam[NP + i] + a3 * sLam[2 * NP + i]);
-        if (i == a.vtx[0]) {
-          sM[bl][e * 12 + 9] = 0.5 * a1;
-          sM[bl][e * 12 + 10] = 0.5 * a2;
-          sM[bl][e * 12 + 11] = 0.5 * a3;
-          if constexpr (EPI32) {
-            sMf[bl][e * 12 + 9] = (float)(0.5 * a1);
-            sMf[bl][e * 12 + 10] = (float)(0.5 * a2);
-            sMf[bl][e * 12 + 11] = (float)(0.5 * a3);
-          }
-        }
-      }
-      if constexpr (PREC == kF32) {
-        sWf[bu][e * NPF + i] = (float)x;
-      } else if constexpr (PREC == kBF3) {
-        // two parts carry ~16 bits: the FP32 residual is exact enough
-        const float xf = (float)x;
-        const __bf16 h = (__bf16)xf;
-        sWb[bu][0][e * KPB + i] = h;
-        sWb[bu][1][e * KPB + i] = (__bf16)(xf - (float)h);
-      } else if constexpr (PREC == kBF6) {
-        const __bf16 h = (__bf16)(float)x;
-        const double r1 = x - (double)(float)h;
-        const __bf16 m = (__bf16)(float)r1;
-        sWb[bu][0][e * KPB + i] = h;
-        sWb[bu][1][e * KPB + i] = m;
-        sWb[bu][2][e * KPB + i] = (__bf16)(float)(r1 - (double)(float)m);
       }
     }
   };
