@@ -17,22 +17,25 @@
 //      coalesced 16-byte loads; the next tile's loads are issued before the
 //      matrix phase;
 //   2. low-precision modes subtract each element's linear interpolant through
-//      its four vertex nodes (w = u - L u, in FP64); D L u is the constant
-//      (u_v1 - u_v0, u_v2 - u_v0, u_v3 - u_v0)/2 in (r, s, t) and is added
-//      back in FP64 in the epilogue. Rounding errors then scale with
-//      |u - L u| = O(h^2 |u''|) instead of |u|;
+//      its four vertex nodes (w = u - L u, in FP64, a group of memory threads
+//      per element); D L u is the constant (u_v1 - u_v0, u_v2 - u_v0,
+//      u_v3 - u_v0)/2 in (r, s, t) and is added back in the epilogue.
+//      Rounding errors then scale with |u - L u| = O(h^2 |u''|) instead of
+//      |u|;
 //   3. every wave runs its MFMAs with B read from LDS, and parks the results
 //      in LDS;
-//   4. the epilogue applies the chain rule in FP64 and writes the three
-//      outputs with contiguous stores.
+//   4. the epilogue applies the chain rule and writes the outputs with
+//      contiguous stores: in FP64 for the FP64 plan and for the divergence
+//      (the split Burgers stage, FP64 output), in FP32 for the derivative
+//      plans below FP64 (FP32 outputs).
 //
 //   F64:     v_mfma_f64_16x16x4_f64.
 //   F32:     v_mfma_f32_16x16x4_f32 on FP32-rounded operator and w.
 //   BF16X3:  v_mfma_f32_16x16x32_bf16 on two-part bf16 splits,
 //            hi*hi + hi*lo + lo*hi (Henry, Tang & Heinecke, ARITH 2019);
 //            ~16 significant bits per operand.
-//   BF16X6:  three-part splits, hi*hi + hi*mid + mid*hi + mid*mid + hi*lo +
-//            lo*hi; ~24 bits, FP32-level accuracy.
+//   BF16X6:  three-part splits (of the FP32 value of w), hi*hi + hi*mid +
+//            mid*hi + mid*mid + hi*lo + lo*hi; ~24 bits, FP32-level accuracy.
 // All low-precision modes accumulate in FP32 and write FP32 outputs.
 #include <cmath>
 #include <cstring>

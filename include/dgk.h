@@ -179,15 +179,17 @@ int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int64_t* K_host
  *   outputs dense element-major, FP64 for DGK_PREC_F64, FP32 otherwise.
  * DGK_PREC_F64: FP64 MFMA (v_mfma_f64_16x16x4), FP64 everywhere.
  * Below FP64 each element's linear interpolant through its vertex nodes is
- * removed in FP64 before rounding and its (exact) derivative added back in
- * FP64, so rounding scales with |u - Lu| rather than |u|; r, s, t (the
- * reference nodes) locate the vertex nodes and are required.
+ * removed in FP64 before rounding and its derivative added back, so rounding
+ * scales with |u - Lu| rather than |u|; r, s, t (the reference nodes) locate
+ * the vertex nodes and are required.
  * DGK_PREC_F32: operator and remainder rounded to FP32, FP32 MFMA.
  * DGK_PREC_BF16X3: two-part bf16 splits, hi*hi + hi*lo + lo*hi
  *   (Henry, Tang & Heinecke, ARITH 2019), ~16 bits per operand.
- * DGK_PREC_BF16X6: three-part bf16 splits, six products, ~24 bits.
- * Low-precision modes accumulate in FP32 and write FP32 outputs; metrics
- * are applied in FP64. */
+ * DGK_PREC_BF16X6: three-part bf16 splits of the FP32 value, six products,
+ *   ~24 bits.
+ * Low-precision modes accumulate in FP32 and write FP32 outputs; the slopes
+ * of Lu and the metrics are applied in FP32 (a few FP32 ulps, below the
+ * products' own error). */
 enum { DGK_PREC_F64 = 0, DGK_PREC_F32 = 1, DGK_PREC_BF16X3 = 2, DGK_PREC_BF16X6 = 3 };
 typedef struct dgk_deriv_plan_s* dgk_deriv_plan;
 int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr_host, const double* Ds_host,
