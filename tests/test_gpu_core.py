@@ -255,6 +255,27 @@ def test_burgers_rhs_vs_oracle(device, N, n, parts, variant):
     assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
 
 
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_burgers_rhs_boundary_faces_vs_oracle(device, N, variant):
+    """Non-periodic jittered cube: a boundary face is its own neighbour, so its
+    LF jump is zero (VmapP = VmapM there, BuildMaps3D)."""
+    if variant in (5, 6) and N > 3:
+        pytest.skip("the deep-pipeline and register-operator kernels are built for N <= 3")
+    m = M.cube_mesh(4, 4, 4, periodic=False, jitter=0.05, seed=40 + N)
+    st = dg.make_setup(m, N, Kp=[150, 1, 233])
+    o = O.Setup.from_mesh(N, m)
+    assert int(np.sum(o.imap("EToE") == np.arange(m.K)[:, None])) == 6 * 2 * 4 * 4
+    u = smooth_u0(st, amp=0.8, noise=1e-2)
+    want = o.burgers_rhs(u)
+    solver = dg.BurgersSolver(device, st)
+    solver.mesh.set_variant(variant)
+    got = solver.rhs(u)
+    spread = max(float(np.max(np.abs(mm - mm[:, :1])) / np.max(np.abs(mm)))
+                 for mm in (o.field(k) for k in METRICS))
+    assert np.max(np.abs(got - want)) <= (1e-12 + 4.0 * spread) * np.max(np.abs(want))
+
+
 @pytest.mark.parametrize("variant", [1, 2, 5, 6])
 def test_config1_burgers_100_steps_vs_oracle(device, variant):
     """Config 1: 8^3 hexes x 6, P=2, 4 slabs, LF + SSP-RK3, CFL 0.3, 100 steps."""

@@ -122,6 +122,35 @@ def test_lift_fp64_flag(device, prec):
     assert not np.array_equal(out["lo_l64"], out["lo"])
 
 
+@pytest.mark.parametrize("N", [5, 6])
+def test_split_boundary_faces_vs_oracle(device, N):
+    """Non-periodic jittered cube: a boundary face is its own neighbour (zero
+    LF jump) in the surface kernel. FP64 rhs against the oracle; every
+    contraction below FP64 stays within its bound of the FP64 run after ten
+    steps (a wrong boundary trace would be an O(1) error)."""
+    m = M.cube_mesh(4, 4, 4, periodic=False, jitter=0.05, seed=50 + N)
+    st = dg.make_setup(m, N, Kp=[150, 1, 233])
+    o = O.Setup.from_mesh(N, m)
+    assert int(np.sum(o.imap("EToE") == np.arange(m.K)[:, None])) == 6 * 2 * 4 * 4
+    u = smooth_u0(st, amp=0.8, noise=1e-2)
+    want = o.burgers_rhs(u)
+    got = dg.BurgersSolver(device, st).rhs(u)
+    tol = 1e-12 + 4.0 * _metric_spread(o)
+    assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want))
+    u0 = smooth_u0(st, amp=0.4, noise=1e-6)
+    ref = dg.BurgersSolver(device, st)
+    ref.set_state(u0)
+    ref.step(10)
+    for prec in PRECS:
+        lo = dg.BurgersSolver(device, st, hmin=ref.hmin)
+        lo.mesh.set_precision(PRECS[prec])
+        lo.set_state(u0)
+        lo.step(10)
+        e2 = rel_l2(lo.state(), ref.state())
+        print(f"P={N} {prec} boundary mesh, 10 steps: rel L2 {e2:.3e}")
+        assert 0.0 < e2 <= 2.0 * BOUND[prec]
+
+
 def test_split_graph_replay(device):
     m = M.cube_mesh(4, 4, 4, periodic=True)
     st = dg.make_setup(m, 6, Kp=[200, 184])
