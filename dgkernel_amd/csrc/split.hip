@@ -49,9 +49,11 @@ struct SCfg {
   static constexpr size_t NFRAG = (size_t)MTL * KSL * 64;  // fragments per part
   static constexpr size_t FRAG_BYTES =
       PREC == kF64 ? NFRAG * 8 : (PREC == kF32 ? NFRAG * 4 : NFRAG * 16 * PARTS);
-  // Waves per block: three per SIMD (four for BF16x3, which needs fewer
-  // registers) to cover the latency of the trace gathers.
-  static constexpr int W = PREC == kBF3 ? 16 : 12;
+  // Waves per block: three per SIMD to cover the latency of the trace
+  // gathers. More waves only add contention for the L1's outstanding misses
+  // (four per SIMD made the BF16x3 kernel ~7 % slower at P=6), fewer leave
+  // latency uncovered.
+  static constexpr int W = 12;
   // Per-wave LDS buffer: a quarter tile (4 elements, row stride NPH) for the
   // own traces, reused as the 16 x 17 update buffer.
   static constexpr int NPH = NP | 1;
