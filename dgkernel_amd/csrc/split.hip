@@ -51,7 +51,7 @@ struct SCfg {
       PREC == kF64 ? NFRAG * 8 : (PREC == kF32 ? NFRAG * 4 : NFRAG * 16 * PARTS);
   // Waves per block: three per SIMD to cover the latency of the trace
   // gathers. More waves only add contention for the L1's outstanding misses
-  // (four per SIMD made the BF16x3 kernel ~7 % slower at P=6), fewer leave
+  // (four per SIMD made the BF16x3 kernel ~8 % slower at P=6), fewer leave
   // latency uncovered.
   static constexpr int W = 12;
   // Per-wave LDS buffer: a quarter tile (4 elements, row stride NPH) for the
