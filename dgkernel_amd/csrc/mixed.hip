@@ -25,9 +25,9 @@
 //   3. every wave runs its MFMAs with B read from LDS, and parks the results
 //      in LDS;
 //   4. the epilogue applies the chain rule and writes the outputs with
-//      contiguous stores: in FP64 for the FP64 plan and for the divergence
-//      (the split Burgers stage, FP64 output), in FP32 for the derivative
-//      plans below FP64 (FP32 outputs).
+//      contiguous stores: in FP64 for the FP64 plan and the BF16x6
+//      divergence, in FP32 for the other plans below FP64 (FP32 outputs; the
+//      divergence of the split Burgers stage is stored as FP64).
 //
 //   F64:     v_mfma_f64_16x16x4_f64.
 //   F32:     v_mfma_f32_16x16x4_f32 on FP32-rounded operator and w.
@@ -176,9 +176,11 @@ deriv_affine_kernel(AffineArgs a) {
   // r, s, t slopes, padded to 12 doubles so the epilogue reads it with six
   // 16-byte loads.
   __shared__ __attribute__((aligned(16))) double sM[4][16 * 12];
-  // Derivative below FP64: the same per-element values in FP32 for an FP32
-  // epilogue (the outputs are FP32; its rounding adds a few FP32 ulps).
-  constexpr bool EPI32 = PREC != kF64 && !DIV;
+  // Below FP64 the same per-element values in FP32, for an FP32 epilogue: its
+  // rounding adds a few FP32 ulps, below the products' own error -- except for
+  // the BF16x6 divergence (the split Burgers stage), whose ~2^-24 products
+  // keep the FP64 epilogue.
+  constexpr bool EPI32 = PREC == kF32 || PREC == kBF3 || (PREC == kBF6 && !DIV);
   __shared__ __attribute__((aligned(16))) float sMf[EPI32 ? 4 : 1][EPI32 ? 16 * 12 : 1];
   __shared__ double sLam[MIXED ? 3 * NP : 1];
 
@@ -426,12 +428,18 @@ deriv_affine_kernel(AffineArgs a) {
         const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2];
         const float ur = (float)re[i] + m2.y, us = (float)re[NP + i] + m2.z,
                     ut = (float)re[2 * NP + i] + m2.w;
-        __builtin_nontemporal_store(fmaf(m1.z, ut, fmaf(m0.w, us, m0.x * ur)),
-                                    (float*)a.ox + tr.vbase + v);
-        __builtin_nontemporal_store(fmaf(m1.w, ut, fmaf(m1.x, us, m0.y * ur)),
-                                    (float*)a.oy + tr.vbase + v);
-        __builtin_nontemporal_store(fmaf(m2.x, ut, fmaf(m1.y, us, m0.z * ur)),
-                                    (float*)a.oz + tr.vbase + v);
+        if constexpr (DIV) {
+          const float ga = m0.x + m0.y + m0.z, gb = m0.w + m1.x + m1.y, gc = m1.z + m1.w + m2.x;
+          __builtin_nontemporal_store(-(double)fmaf(gc, ut, fmaf(gb, us, ga * ur)),
+                                      (double*)a.ox + tr.vbase + v);
+        } else {
+          __builtin_nontemporal_store(fmaf(m1.z, ut, fmaf(m0.w, us, m0.x * ur)),
+                                      (float*)a.ox + tr.vbase + v);
+          __builtin_nontemporal_store(fmaf(m1.w, ut, fmaf(m1.x, us, m0.y * ur)),
+                                      (float*)a.oy + tr.vbase + v);
+          __builtin_nontemporal_store(fmaf(m2.x, ut, fmaf(m1.y, us, m0.z * ur)),
+                                      (float*)a.oz + tr.vbase + v);
+        }
       } else if (v < lim) {
         const int e = v / NP, i = v - e * NP;
         const R* re = sR[br] + e * ROWSP;
