@@ -212,6 +212,67 @@ def read_gambit_neutral(path: str) -> TetMesh:
                    EToP=etop, info={"file": path})
 
 
+def write_gambit_neutral(mesh: TetMesh, path: str, title: str = "dgkernel_amd tetrahedral mesh") -> None:
+    """Writes ``mesh`` as a Gambit neutral file that read_gambit_neutral (and the
+    reference's mesh/readers/gambit_reader.go:14-338) reads back to the same
+    vertices, elements and partitions.
+
+    Vertices and elements are numbered 1.. in array order; coordinates carry 17
+    significant digits, so they round-trip exactly. As in Gmsh's output, a
+    material group holds every element, and each partition follows as an
+    element group named ``p<N>``. A periodic identification (``EToVc``) has no
+    place in this format: use save_mesh for periodic meshes.
+    """
+    if mesh.EToVc is not mesh.EToV and not np.array_equal(mesh.EToVc, mesh.EToV):
+        raise ValueError("write_gambit_neutral: periodic meshes need save_mesh")
+    K, nv = mesh.K, int(mesh.VX.size)
+    parts = [] if mesh.EToP is None else [int(q) for q in np.unique(mesh.EToP) if q >= 0]
+    out = ["        CONTROL INFO 2.4.6", "** GAMBIT NEUTRAL FILE", title,
+           "PROGRAM:            dgkernel_amd     VERSION:  1.0", "",
+           "     NUMNP     NELEM     NGRPS    NBSETS     NDFCD     NDFVL",
+           f"{nv:10d}{K:10d}{len(parts) + 1:10d}{0:10d}{3:10d}{3:10d}", "ENDOFSECTION",
+           "   NODAL COORDINATES 2.4.6"]
+    for q in range(nv):
+        out.append(f"{q + 1:10d} {mesh.VX[q]:24.16e} {mesh.VY[q]:24.16e} {mesh.VZ[q]:24.16e}")
+    out += ["ENDOFSECTION", "      ELEMENTS/CELLS 2.4.6"]
+    for k in range(K):
+        v = mesh.EToV[k] + 1
+        out.append(f"{k + 1:8d} {6:2d} {4:2d} {v[0]:8d}{v[1]:8d}{v[2]:8d}{v[3]:8d}")
+    out.append("ENDOFSECTION")
+    groups = [("Material group 1", np.arange(1, K + 1))]
+    groups += [("p" + str(part), np.nonzero(mesh.EToP == part)[0] + 1) for part in parts]
+    for gi, (name, ids) in enumerate(groups):
+        out += ["       ELEMENT GROUP 2.4.6",
+                f"GROUP: {gi + 1:10d} ELEMENTS: {ids.size:10d} MATERIAL: {0:10d} NFLAGS: {1:10d}",
+                f"{name:>32s}", "       0"]
+        for a in range(0, ids.size, 10):
+            out.append("".join(f"{int(e):8d}" for e in ids[a:a + 10]))
+        out.append("ENDOFSECTION")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+def save_mesh(mesh: TetMesh, path: str) -> None:
+    """Stores a mesh, periodic identification and partitions included, as a
+    NumPy .npz (no pickled objects), so that separate runs -- CPU oracle and
+    GPU -- read identical inputs. ``info`` must be JSON-serialisable."""
+    import json
+    np.savez(path, VX=mesh.VX, VY=mesh.VY, VZ=mesh.VZ, EToV=mesh.EToV, EToVc=mesh.EToVc,
+             EToP=np.zeros(0, np.int64) if mesh.EToP is None else mesh.EToP,
+             has_etop=np.array(mesh.EToP is not None), period=np.array(mesh.period, np.float64),
+             info=np.array(json.dumps(mesh.info)))
+
+
+def load_mesh(path: str) -> TetMesh:
+    """Reads a mesh written by save_mesh (numpy.load with pickles refused)."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        etop = z["EToP"].astype(np.int64) if bool(z["has_etop"]) else None
+        return TetMesh(z["VX"].copy(), z["VY"].copy(), z["VZ"].copy(), z["EToV"].astype(np.int64),
+                       z["EToVc"].astype(np.int64), EToP=etop,
+                       period=tuple(float(x) for x in z["period"]), info=json.loads(str(z["info"])))
+
+
 def connect(mesh: TetMesh):
     """EToE, EToF [K,4] by matching sorted face vertex keys (ticonnect3d.go:24-122).
 
