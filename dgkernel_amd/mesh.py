@@ -491,5 +491,55 @@ def morton_within_partitions(mesh: TetMesh, Kp: np.ndarray) -> TetMesh:
                    period=mesh.period, info=dict(mesh.info, order="morton-within-partitions"))
 
 
+def element_centroids(mesh: TetMesh) -> np.ndarray:
+    """[K, 3] centroid of each element (affine-map coordinates)."""
+    return np.stack([mesh.VX[mesh.EToV].mean(axis=1), mesh.VY[mesh.EToV].mean(axis=1),
+                     mesh.VZ[mesh.EToV].mean(axis=1)], axis=1)
+
+
+def rcb_partition(mesh: TetMesh, counts) -> np.ndarray:
+    """Recursive coordinate bisection of the element centroids into
+    ``len(counts)`` partitions of exactly ``counts[p]`` elements; returns EToP.
+
+    A geometric stand-in for the reference's METIS partitioner
+    (mesh/partitioner/), which is not available here: each split cuts the
+    longest extent of the current element set at the element count the left
+    half of the partitions asks for, so partitions come out compact and the
+    number of faces between partitions -- the exchange volume -- stays near the
+    surface-to-volume minimum instead of growing like the slabs' cross-section.
+    """
+    counts = np.asarray(counts, dtype=np.int64)
+    if counts.ndim != 1 or counts.size == 0 or (counts < 0).any() or int(counts.sum()) != mesh.K:
+        raise ValueError("rcb_partition: counts must be non-negative and sum to K")
+    cen = element_centroids(mesh)
+    etop = np.empty(mesh.K, dtype=np.int64)
+
+    def split(idx: np.ndarray, parts: np.ndarray) -> None:
+        if parts.size == 1:
+            etop[idx] = parts[0]
+            return
+        half = parts.size // 2
+        nl = int(counts[parts[:half]].sum())
+        c = cen[idx]
+        axis = int(np.argmax(np.ptp(c, axis=0))) if idx.size else 0
+        order = np.argsort(c[:, axis], kind="stable")
+        split(idx[order[:nl]], parts[:half])
+        split(idx[order[nl:]], parts[half:])
+
+    split(np.arange(mesh.K), np.arange(counts.size))
+    return etop
+
+
+def order_by_partition(mesh: TetMesh, etop: np.ndarray) -> tuple[TetMesh, np.ndarray]:
+    """Renumbers elements partition-major (stable within each partition), the
+    order the partitioned state arrays and the stage's tile lists use; returns
+    the renumbered mesh (EToP set) and the element count of each partition."""
+    etop = np.asarray(etop, dtype=np.int64)
+    order = np.argsort(etop, kind="stable")
+    kp = np.bincount(etop, minlength=int(etop.max()) + 1 if etop.size else 0).astype(np.int64)
+    return (TetMesh(mesh.VX, mesh.VY, mesh.VZ, mesh.EToV[order], mesh.EToVc[order], EToP=etop[order],
+                    period=mesh.period, info=dict(mesh.info, order="partition-major")), kp)
+
+
 def etop_from_counts(kp: np.ndarray) -> np.ndarray:
     return np.repeat(np.arange(kp.size, dtype=np.int64), kp)

@@ -276,6 +276,28 @@ def test_burgers_rhs_boundary_faces_vs_oracle(device, N, variant):
     assert np.max(np.abs(got - want)) <= (1e-12 + 4.0 * spread) * np.max(np.abs(want))
 
 
+@pytest.mark.parametrize("N", [3, 5])
+def test_burgers_rhs_rcb_partitions_vs_oracle(device, N):
+    """Partitions from recursive coordinate bisection (uneven counts), elements
+    renumbered partition-major: every stage form against the oracle on the
+    same renumbered mesh."""
+    m = M.cube_mesh(5, 5, 5, periodic=True, jitter=0.05, seed=12)
+    kp = M.slab_partition(m.K, 6, 0.4, seed=3)
+    mo, kp = M.order_by_partition(m, M.rcb_partition(m, kp))
+    st = dg.make_setup(mo, N, Kp=kp)
+    u = smooth_u0(st, noise=1e-3)
+    o = O.Setup.from_mesh(N, mo)
+    want = o.burgers_rhs(u)
+    spread = max(float(np.max(np.abs(mm - mm[:, :1])) / np.max(np.abs(mm)))
+                 for mm in (o.field(k) for k in METRICS))
+    for variant in ((1, 2, 5) if N <= 3 else (0,)):
+        solver = dg.BurgersSolver(device, st)
+        if variant:
+            solver.mesh.set_variant(variant)
+        got = solver.rhs(u)
+        assert np.max(np.abs(got - want)) <= (1e-12 + 4.0 * spread) * np.max(np.abs(want)), variant
+
+
 @pytest.mark.parametrize("variant", [1, 2, 5, 6])
 def test_config1_burgers_100_steps_vs_oracle(device, variant):
     """Config 1: 8^3 hexes x 6, P=2, 4 slabs, LF + SSP-RK3, CFL 0.3, 100 steps."""
