@@ -1,0 +1,79 @@
+"""The benchmark configurations at their full sizes on the GPU, checked
+through properties that hold at any size (the oracle is too slow there):
+
+* the stage conserves the integral of u (periodic domain, conservative LF flux
+  in strong form; the oracle pins 1'MJ L(u) = 0 to round-off on small meshes);
+* graph replay of whole steps equals eager launches bit for bit;
+* a contraction below FP64 stays within its bound of the FP64 run.
+"""
+import numpy as np
+import pytest
+
+import bench
+from dgkernel_amd import _lib, dg
+from dgkernel_amd import mesh as M
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _jacobians(m):
+    """J of the affine map x = ((-(1+r+s+t)) v0 + (1+r) v1 + (1+s) v2 + (1+t) v3)/2:
+    det of the three edge vectors from v0, over 8."""
+    v = np.stack([m.VX[m.EToV], m.VY[m.EToV], m.VZ[m.EToV]], axis=2)
+    return np.abs(np.linalg.det(v[:, 1:, :] - v[:, :1, :])) / 8.0
+
+
+def _integral(st, J, u):
+    return float(np.sum(J * (u @ st.ref.Mass.sum(axis=1))))
+
+
+def test_c3_full_size_conservation_and_graph_replay(device):
+    """96^3 x 6 = 5,308,416 periodic tets, P=3, 64 slab partitions with +-40 %
+    imbalance, Morton order within partitions: two SSP-RK3 steps."""
+    m, st, u0 = bench.build_c3(96, 3, 64, 2024, morton=True)
+    assert m.K == 5308416 and st.npart == 64
+    J = _jacobians(m)
+    assert abs(J.sum() * 4.0 / 3.0 - 1.0) <= 1e-12  # the tets fill the unit cube
+    a = dg.BurgersSolver(device, st, cfl=0.3)
+    a.set_state(u0)
+    a.step(2)
+    ua = a.state()
+    assert np.isfinite(ua).all() and 0.0 < a.dt() < 1e-2
+    i0, i1 = _integral(st, J, u0), _integral(st, J, ua)
+    assert abs(i1 - i0) <= 1e-12 * abs(i0)
+    assert rel_l2(ua, u0) > 1e-4  # the state did move
+    b = dg.BurgersSolver(device, st, cfl=0.3, hmin=a.hmin)
+    b.set_state(u0)
+    b.step(2, graph=True)
+    assert np.array_equal(b.state(), ua)
+
+
+def test_c4_full_size_conservation_and_mixed_precision(device):
+    """64^3 x 6 = 1,572,864 periodic tets, P=6, 2 slab partitions: two FP64
+    steps conserve the integral; BF16x3 contractions with FP64 LIFT stay within
+    the BF16x3 bound of the FP64 run, eager and graph-replayed alike."""
+    m = M.cube_mesh(64, 64, 64, periodic=True)
+    st = dg.make_setup(m, 6, Kp=M.slab_partition(m.K, 2))
+    x, y, z = st.node_coords()
+    u0 = (0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+          + M.uniform(0, 7, x.size, -1e-6, 1e-6).reshape(x.shape))
+    del x, y, z
+    J = _jacobians(m)
+    ref = dg.BurgersSolver(device, st, cfl=0.3)
+    ref.set_state(u0)
+    ref.step(2)
+    ur = ref.state()
+    assert np.isfinite(ur).all()
+    i0, i1 = _integral(st, J, u0), _integral(st, J, ur)
+    assert abs(i1 - i0) <= 1e-12 * abs(i0)
+    out = []
+    for graph in (False, True):
+        lo = dg.BurgersSolver(device, st, cfl=0.3, hmin=ref.hmin)
+        lo.mesh.set_precision(_lib.DGK_PREC_BF16X3 | _lib.DGK_PREC_LIFT_F64)
+        lo.set_state(u0)
+        lo.step(2, graph=graph)
+        out.append(lo.state())
+    e = rel_l2(out[0], ur)
+    assert 0.0 < e <= 5e-3
+    assert np.array_equal(out[0], out[1])
