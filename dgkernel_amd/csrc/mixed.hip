@@ -478,6 +478,9 @@ deriv_affine_kernel(AffineArgs a) {
   // the operator registers of the matrix waves and the prefetch registers of
   // the memory waves are never live together.
   if (is_mma) {
+    // Matrix waves issue first when both roles are ready on a SIMD (wave
+    // priority, not a queue priority): 0.3-1 % faster across the plans.
+    __builtin_amdgcn_s_setprio(2);
     // Operator fragments of this wave's row tiles, resident for the launch.
     Frag<PREC> fa[TPW][KS];
 #pragma unroll
