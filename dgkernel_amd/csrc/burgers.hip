@@ -850,6 +850,9 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
     deep_meta(p, dn2, lane, nb2, pk2);
     __builtin_amdgcn_sched_barrier(0);
     wave_sync();  // sU holds this tile
+    // A wave in its matrix phase issues first (wave priority): the other wave
+    // of the SIMD does its loads and VALU work in the MFMA shadow.
+    __builtin_amdgcn_s_setprio(1);
     double vout[QN];
     {
       double4_t accV[MTV];
@@ -889,6 +892,7 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
         accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,
                                                          accL[tt], 0, 0, 0);
     }
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int q = 0; q < QN; ++q) {
       const double lf = (4 * q) / 16 < MTL ? accL[((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;
