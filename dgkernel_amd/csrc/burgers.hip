@@ -668,6 +668,7 @@ __device__ __forceinline__ void pipe_tiles(const StageParams& p, const double* s
     load_tile_in<N>(p, dnxt, lane, nxt);
     __builtin_amdgcn_sched_barrier(0);
     wave_sync();  // sU holds this tile
+    __builtin_amdgcn_s_setprio(1);  // matrix phase first (wave priority)
     // Volume: [Dr; Ds; Dt] F, folded at once with the affine metrics.
     double vout[QN];
     {
@@ -718,6 +719,7 @@ __device__ __forceinline__ void pipe_tiles(const StageParams& p, const double* s
             REGA ? ra[(MTV * KSV + tt * KSL + s) % NFR] : sAl[(tt * KSL + s) * 64 + lane], gm,
                                                          accL[tt], 0, 0, 0);
     }
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int q = 0; q < QN; ++q) {
       const double lf = (4 * q) / 16 < MTL ? accL[((4 * q) / 16) % MTL][((4 * q) % 16) / 4] : 0.0;

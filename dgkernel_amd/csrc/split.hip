@@ -254,6 +254,7 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
       AccT<PREC> acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = 0;
+      __builtin_amdgcn_s_setprio(1);  // matrix phase first (wave priority)
 #pragma unroll
       for (int s = 0; s < KSL; ++s) {
         const size_t q = ((size_t)tt * KSL + s) * 64 + lane;
@@ -278,6 +279,7 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
           }
         }
       }
+      __builtin_amdgcn_s_setprio(0);
       // Through the wave's 16 x 17 buffer to rows of 16 consecutive values.
       wave_sync();  // previous readers of the buffer are done
 #pragma unroll
