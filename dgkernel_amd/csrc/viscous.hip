@@ -573,10 +573,12 @@ __global__ void __launch_bounds__(64 * WPB) av_surf_kernel(AvArgs a, StageAv o) 
         p0[j] = MODE == 0 ? __builtin_nontemporal_load(o.u0 + pidx[j]) : 0.0;
       }
       double4v acc = (double4v){0.0, 0.0, 0.0, 0.0};
+      __builtin_amdgcn_s_setprio(1);  // matrix phase first (wave priority)
 #pragma unroll
       for (int s = 0; s < KSL; ++s)
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[(tt * KSL + s) * 64 + lane], bj[s], acc, 0,
                                                    0, 0);
+      __builtin_amdgcn_s_setprio(0);
       wave_fence();  // previous readers of the buffer are done
 #pragma unroll
       for (int r = 0; r < 4; ++r) sB[c * 17 + g + 4 * r] = acc[r];
