@@ -165,6 +165,30 @@ def _simulate_exchange(fc, u_parts, Nfp):
     return out
 
 
+def test_face_exchange_partial_lists_keep_untouched_slots(device):
+    """Lists that leave a P slot unfilled (one entry dropped from one pair)
+    take the pair-ordered path; the filled slots match the host simulation and
+    the dropped slot keeps its old content."""
+    import copy
+    m = M.cube_mesh(3, 3, 3, periodic=True)
+    st = dg.make_setup(m, 1, Kp=M.slab_partition(m.K, 3))
+    _, vp = st.vmaps()
+    fc = M.new_face_connector(m.K, 4, st.ref.Nfp, st.ref.Np, vp, M.etop_from_counts(st.Kp))
+    part = copy.copy(fc)
+    part.pick, part.place = dict(fc.pick), dict(fc.place)
+    part.pick[(0, 1)] = fc.pick[(0, 1)][:-1]
+    part.place[(1, 0)] = fc.place[(1, 0)][:-1]
+    u = M.uniform(8, 3, m.K * st.ref.Np, -1, 1).reshape(m.K, st.ref.Np)
+    u_parts = [u[fc.local_to_global[q]] for q in range(3)]
+    got = _exchange_gpu(device, part, u_parts, st.ref.Np, st.ref.Nfp)
+    want = _simulate_exchange(part, u_parts, st.ref.Nfp)
+    for q in range(3):
+        assert np.array_equal(got[q], want[q], equal_nan=True)
+    assert np.isnan(got[1][fc.place[(1, 0)][-1]])
+    full = _exchange_gpu(device, fc, u_parts, st.ref.Np, st.ref.Nfp)
+    assert not any(np.isnan(f).any() for f in full)
+
+
 def test_face_exchange_neu_five_partitions_bit_exact(device):
     """cube-partitioned.neu: groups p1..p4 give 5 partitions, p0 empty (face_connector.go:59-64)."""
     N = 2
