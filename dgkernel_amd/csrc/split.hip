@@ -52,8 +52,9 @@ struct SCfg {
   // Waves per block: three per SIMD to cover the latency of the trace
   // gathers. More waves only add contention for the L1's outstanding misses
   // (four per SIMD made the BF16x3 kernel ~8 % slower at P=6), fewer leave
-  // latency uncovered.
-  static constexpr int W = 12;
+  // latency uncovered. BF16x3 runs them as two blocks of six (its fragments
+  // leave room in LDS for two), which measured 0.6 % faster per step at P=6.
+  static constexpr int W = PREC == kBF3 ? 6 : 12;
   // Per-wave LDS buffer: a quarter tile (4 elements, row stride NPH) for the
   // own traces, reused as the 16 x 17 update buffer.
   static constexpr int NPH = NP | 1;
