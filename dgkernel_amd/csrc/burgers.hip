@@ -941,7 +941,7 @@ __global__ void __launch_bounds__(64 * WPB, KIND == 0 ? 3 : 2)
     burgers_stage_pipe_kernel(StageParams p, int nfull) {
   using SH = PipeShape<N>;
   constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, NPR = SH::NPR;
-  constexpr int KSV = SH::KSV, MTV = SH::MTV, MTL = SH::MTL, KSL = SH::KSL;
+  constexpr int KSV = SH::KSV, MTV = SH::MTV, KSL = SH::KSL;
   constexpr int NT = 64 * WPB;
   __shared__ __attribute__((aligned(16))) double sA[SH::NA];
   __shared__ __attribute__((aligned(16))) double sUall[WPB][SH::E * SH::S];

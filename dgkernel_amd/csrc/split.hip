@@ -145,7 +145,7 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
   }
   for (int t = t0; t < t1; t += ts) {
     const dgk::Tile tile = a.tiles[t];
-    const int cnt = tile.count, lim = cnt * NP;
+    const int cnt = tile.count;
     const int ce = c < cnt ? c : cnt - 1;  // clamped: every load stays in the tile
     const int64_t ge = tile.gbase + ce;
     const double* ut = a.u + tile.vbase;
