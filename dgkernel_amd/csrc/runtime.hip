@@ -106,9 +106,9 @@ int dgk_device_create(const char* props, dgk_device* out) {
 
 int dgk_device_destroy(dgk_device dev) {
   if (!dev) return DGK_OK;
-  hipSetDevice(dev->id);
-  hipStreamSynchronize(dev->stream);
-  hipStreamDestroy(dev->stream);
+  (void)hipSetDevice(dev->id);
+  (void)hipStreamSynchronize(dev->stream);
+  (void)hipStreamDestroy(dev->stream);
   delete dev;
   return DGK_OK;
 }
@@ -142,7 +142,7 @@ int dgk_malloc(dgk_device dev, size_t bytes, const void* init_host, void** out_d
   if (init_host && bytes) {
     hipError_t e = hipMemcpy(p, init_host, bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-      hipFree(p);
+      (void)hipFree(p);
       dgk::set_error("dgk_malloc: initial copy failed: %s", hipGetErrorString(e));
       return DGK_ERR_HIP;
     }
@@ -220,7 +220,7 @@ int dgk_graph_end(dgk_device dev, dgk_graph* out) {
   hipGraphExec_t ex;
   hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   if (e != hipSuccess) {
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     dgk::set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
     return DGK_ERR_HIP;
   }
@@ -236,8 +236,8 @@ int dgk_graph_launch(dgk_graph g) {
 
 int dgk_graph_destroy(dgk_graph g) {
   if (!g) return DGK_OK;
-  hipGraphExecDestroy(g->exec);
-  hipGraphDestroy(g->graph);
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
   delete g;
   return DGK_OK;
 }
@@ -282,8 +282,8 @@ int dgk_timer_elapsed_ms(dgk_timer t, double* ms) {
 
 int dgk_timer_destroy(dgk_timer t) {
   if (!t) return DGK_OK;
-  hipEventDestroy(t->a);
-  hipEventDestroy(t->b);
+  (void)hipEventDestroy(t->a);
+  (void)hipEventDestroy(t->b);
   delete t;
   return DGK_OK;
 }
