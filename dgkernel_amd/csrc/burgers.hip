@@ -805,7 +805,9 @@ __device__ __forceinline__ void deep_meta(const StageParams& p, const Tile& d, i
   nb = p.nbr[(lane >> 4) * p.K + g];
 }
 
-template <int N, int MODE, bool FULL>
+// DEPTH 1: the loads of tile t+1 are in flight during tile t. DEPTH 2 (stage
+// 1, which reads no u0 and has the registers): those of t+1 and t+2.
+template <int N, int MODE, bool FULL, int DEPTH = 1>
 __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* sA, double* sU,
                                            const uint8_t* sFN, int t0, int t1, int ts,
                                            double dt, double& lmax) {
@@ -827,16 +829,23 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
     own[w] = gq == 0 ? FPK.w[0][w] : gq == 1 ? FPK.w[1][w] : gq == 2 ? FPK.w[2][w] : FPK.w[3][w];
   const uint8_t* sfn_face = sFN + gq * 24 * NFP;
 
+  static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth 1 or 2");
   Tile dcur = tile_uniform(load_tile_raw(p.tiles, t0));
   Tile dn1 = tile_uniform(load_tile_raw(p.tiles, min(t0 + ts, t1 - 1)));
   Tile dn2 = tile_uniform(load_tile_raw(p.tiles, min(t0 + 2 * ts, t1 - 1)));
+  Tile dn3 = DEPTH == 2 ? tile_uniform(load_tile_raw(p.tiles, min(t0 + 3 * ts, t1 - 1))) : dn2;
   int32_t nb0, pk0, nb1, pk1;
   deep_meta(p, dcur, lane, nb0, pk0);
-  DeepIn<N, MODE> cur;
+  DeepIn<N, MODE> cur, nx1;
   deep_issue<N, MODE>(p, dcur, lane, nb0, pk0, sfn_face, cur);
   deep_meta(p, dn1, lane, nb1, pk1);
+  if constexpr (DEPTH == 2) {
+    // tile t+1 in flight from the start; the meta of t+2 ready
+    deep_issue<N, MODE>(p, dn1, lane, nb1, pk1, sfn_face, nx1);
+    deep_meta(p, dn2, lane, nb1, pk1);
+  }
   for (int t = t0; t < t1; t += ts) {
-    const int4 d3 = load_tile_raw(p.tiles, min(t + 3 * ts, t1 - 1));
+    const int4 d3 = load_tile_raw(p.tiles, min(t + (DEPTH + 2) * ts, t1 - 1));
     const int cnt = FULL ? E : dcur.count;
 #pragma unroll
     for (int j = 0; j < PV; ++j) {
@@ -846,10 +855,11 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
       sU[ok ? e * S + n : E * S - 1] = cur.u[j];
     }
     __builtin_amdgcn_sched_barrier(0);
+    // Loads of tile t+DEPTH (its meta was read a tile ago), meta of t+DEPTH+1.
     DeepIn<N, MODE> nx;
-    deep_issue<N, MODE>(p, dn1, lane, nb1, pk1, sfn_face, nx);
+    deep_issue<N, MODE>(p, DEPTH == 2 ? dn2 : dn1, lane, nb1, pk1, sfn_face, nx);
     int32_t nb2, pk2;
-    deep_meta(p, dn2, lane, nb2, pk2);
+    deep_meta(p, DEPTH == 2 ? dn3 : dn2, lane, nb2, pk2);
     __builtin_amdgcn_sched_barrier(0);
     wave_sync();  // sU holds this tile
     // A wave in its matrix phase issues first (wave priority): the other wave
@@ -923,12 +933,22 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
         lmax = fmax(lmax, fabs(o));
       }
     }
-    cur = nx;
+    if constexpr (DEPTH == 2) {
+      cur = nx1;
+      nx1 = nx;
+    } else {
+      cur = nx;
+    }
     nb1 = nb2;
     pk1 = pk2;
     dcur = dn1;
     dn1 = dn2;
-    dn2 = tile_uniform(d3);
+    if constexpr (DEPTH == 2) {
+      dn2 = dn3;
+      dn3 = tile_uniform(d3);
+    } else {
+      dn2 = tile_uniform(d3);
+    }
   }
 }
 
@@ -986,7 +1006,7 @@ __global__ void __launch_bounds__(64 * WPB, KIND == 0 ? 3 : 2)
     fs = NW / 8;
   }
   if (KIND == 1) {
-    deep_tiles<N, MODE, true>(p, sA, sU, sFN, f0, f1, fs, dt, lmax);
+    deep_tiles<N, MODE, true, MODE == 1 ? 2 : 1>(p, sA, sU, sFN, f0, f1, fs, dt, lmax);
     deep_tiles<N, MODE, false>(p, sA, sU, sFN, nfull + gw, p.ntiles, NW, dt, lmax);
   } else {
     pipe_tiles<N, MODE, true, KIND == 2>(p, sA, sU, sFN, f0, f1, fs, dt, lmax);
