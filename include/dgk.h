@@ -213,9 +213,12 @@ int dgk_matmul(dgk_device dev, int rows, int cols, const double* A_colmajor_dev,
 /* semantics; runtime exchange of utils/face_connector_test.go:104-156):   */
 /*   P_global[P_off[dst] + place[j]] = u_global[u_off[src] + pick[i]]       */
 /* for every (src, dst) pair, including src == dst, in one launch.         */
-/* A plan holds the concatenated lists on the device: pick lists src-major */
+/* The caller passes the concatenated lists: pick lists src-major         */
 /* (pick[src][dst] for src, then dst), place lists dst-major (place[dst]   */
-/* [src]); counts[src*npart + dst] gives each pair's length.               */
+/* [src]); counts[src*npart + dst] gives each pair's length. If the place  */
+/* lists fill every slot of every destination once (the FaceConnector's    */
+/* do), the plan stores the entries in destination-slot order and the      */
+/* launch writes P with unit stride; other lists are kept pair by pair.    */
 /* ---------------------------------------------------------------------- */
 typedef struct dgk_face_plan_s* dgk_face_plan;
 int dgk_face_plan_create(dgk_device dev, int npart, const int64_t* counts_host,
