@@ -75,5 +75,6 @@ def test_c4_full_size_conservation_and_mixed_precision(device):
         lo.step(2, graph=graph)
         out.append(lo.state())
     e = rel_l2(out[0], ur)
+    print(f"C4 BF16x3 (FP64 LIFT) 2 steps: rel L2 {e:.3e} vs FP64")
     assert 0.0 < e <= 5e-3
     assert np.array_equal(out[0], out[1])

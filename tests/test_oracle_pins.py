@@ -1,6 +1,7 @@
 """Pins of the CPU oracle against the reference's own known-answer tests,
 closed forms and invariants (no GPU). Each test names the reference test or
 the mathematical fact it restates."""
+import itertools
 import math
 
 import numpy as np
@@ -92,6 +93,188 @@ def test_nodes_two_restatements_agree(N):
     X, Y, Z = O.nodes3d(N)
     x, y, z = nudg.nodes3d(N)
     assert max(np.abs(X - x).max(), np.abs(Y - y).max(), np.abs(Z - z).max()) <= 1e-15
+
+
+# --- Nodes3D at P >= 3, pinned by what the reference's code reduces to -------
+#
+# The reference's warp & blend nodes (element/library/gonudg/nodes3d.go:17-323)
+# depart from the Hesthaven-Warburton text: alphastore is read 0-based
+# (:19-28) and evalwarp pairs the negated (descending) GL points with the
+# ascending equispaced points and multiplies by 1 - x^2 (:227-231, :295-319).
+# The closed forms below are derived by hand from that code; they are
+# evaluated here through numpy's Legendre class and a polynomial fit, not by
+# the oracle's Golub-Welsch / Lagrange-product route.
+#
+# Edge s = t = -1 (L1 = L2 = 0): the blends of faces 2 and 3 vanish
+# (Lb = L1 = 0), faces 0 and 1 overwrite the shift (La = 0, count = 2), so
+# only face 1's warp survives. In its evalshift call (L1, L2, L3) =
+# (0, (1-r)/2, (1+r)/2), so warp1 = (1-r)(1+r)/4 * 4 evalwarp(r), warp2 =
+# warp3 = 0 and the node moves along t1[1] = (1,0,0):
+#     r' = r + (1 - r^2)^2 sum_j l_j(r) (-x_j^GL - r_j),
+# l_j the Lagrange basis on the equispaced r_j, x^GL ascending. At the
+# equispaced nodes themselves l_j(r_k) = delta_jk. At P=3 this puts the edge
+# nodes at +-0.283395 where the textbook puts the GL points +-0.447214.
+#
+# Face t = -1 interior node at P=4, barycentrics (a, b, c) = (1/2, 1/4, 1/4)
+# of (v3, v1, v2): faces 1-3 have zero blend, face 0 has blend 1 and no
+# overwrite (count = 3). In evalshift w1 = 0 (c - b = 0, and the warp is odd),
+# w3 = -w2, so dx = 0 and dy = sqrt(3) w2 along t2[0] = (0,1,0), which moves
+# the node along the median to v3: lambda(v3) = 1/2 + w2 with
+#     w2 = a c 4 evalwarp(1/4) (1 + alpha^2 b^2) = (1/2)(1 + alpha^2/16)(15/16) W(1/4),
+# W(x) = sum_j l_j(x)(-x_j^GL - r_j), alpha = alphastore[4] = 1.1332 (0-based
+# read; the textbook's alpha(4) would be 0.1002).
+
+
+def _gl_points(P):
+    """Gauss-Lobatto-Legendre points, ascending: the ends and the roots of P_N'."""
+    inner = np.polynomial.legendre.Legendre.basis(P).deriv().roots()
+    return np.concatenate(([-1.0], np.sort(inner.real), [1.0]))
+
+
+def _warp_interp(P, x):
+    """W(x): the degree-P interpolant through (r_j, -x_j^GL - r_j), by polyfit."""
+    r = np.linspace(-1.0, 1.0, P + 1)
+    c = np.polynomial.polynomial.polyfit(r, -_gl_points(P) - r, P)
+    return np.polynomial.polynomial.polyval(x, c)
+
+
+def _bary(r, s, t):
+    """Barycentrics of the vertices (-1,-1,-1), (1,-1,-1), (-1,1,-1), (-1,-1,1)."""
+    return np.stack([-(1 + r + s + t) / 2, (1 + r) / 2, (1 + s) / 2, (1 + t) / 2], axis=1)
+
+
+def _edge_error(lam, N):
+    """Max deviation of the nodes on the six edges from the closed form."""
+    req = np.linspace(-1.0, 1.0, N + 1)
+    want = np.sort((1 + req + (1 - req ** 2) ** 2 * (-_gl_points(N) - req)) / 2)
+    worst = 0.0
+    for i, j in itertools.combinations(range(4), 2):
+        others = [k for k in range(4) if k not in (i, j)]
+        on = np.all(np.abs(lam[:, others]) <= 1e-12, axis=1)
+        got = np.sort(lam[on, j])
+        if got.size != N + 1:
+            return np.inf
+        worst = max(worst, float(np.abs(got - want).max()))
+    return worst
+
+
+def _face_p4_error(lam):
+    """Max deviation of the 12 face-interior nodes at P=4 from the closed form."""
+    w2 = 0.5 * (1 + 1.1332 ** 2 / 16) * (15 / 16) * _warp_interp(4, 0.25)
+    worst = 0.0
+    for f in range(4):
+        face = [k for k in range(4) if k != f]
+        on = (np.abs(lam[:, f]) <= 1e-12) & np.all(lam[:, face] > 1e-12, axis=1)
+        if on.sum() != 3:
+            return np.inf
+        for row in lam[on][:, face]:
+            want = np.sort([0.5 + w2, (0.5 - w2) / 2, (0.5 - w2) / 2])
+            worst = max(worst, float(np.abs(np.sort(row) - want).max()))
+    return worst
+
+
+def _symmetry_error(lam):
+    """Max distance from each vertex-permuted node to its nearest node."""
+    worst = 0.0
+    for p in itertools.permutations(range(4)):
+        d = np.abs(lam[:, p][:, None, :] - lam[None, :, :]).max(axis=2).min(axis=1).max()
+        worst = max(worst, float(d))
+    return worst
+
+
+def _node_sets(N):
+    e = O.element(N)
+    r, s, t = nudg.xyz_to_rst(*nudg.nodes3d(N))
+    return {"oracle": (e["r"], e["s"], e["t"]), "nudg": (r, s, t)}
+
+
+@pytest.mark.parametrize("N", range(3, 7))
+def test_nodes_edge_closed_form(N):
+    """Edge nodes on all six edges equal the reduction of nodes3d.go above."""
+    for name, (r, s, t) in _node_sets(N).items():
+        assert _edge_error(_bary(r, s, t), N) <= 2e-15, name
+    if N == 3:  # the textbook's edge nodes (the GL points) are not these
+        r, s, t = _node_sets(3)["oracle"]
+        edge = np.sort(r[(np.abs(s + 1) < 1e-12) & (np.abs(t + 1) < 1e-12)])
+        assert np.allclose(edge, [-1.0, -0.283395, 0.283395, 1.0], atol=1e-6)
+
+
+def test_nodes_face_closed_form_p4():
+    """Face-interior nodes at P=4 (they carry the 0-based alpha lookup)."""
+    for name, (r, s, t) in _node_sets(4).items():
+        assert _face_p4_error(_bary(r, s, t)) <= 2e-15, name
+
+
+@pytest.mark.parametrize("N", range(1, 7))
+def test_nodes_symmetric_under_vertex_permutations(N):
+    """The barycentric node set is invariant under all 24 vertex permutations;
+    at P=3 the face-interior nodes sit at the face centroids and at P=4 the
+    interior node at the tet centroid (the odd warp vanishes at 0)."""
+    for name, (r, s, t) in _node_sets(N).items():
+        lam = _bary(r, s, t)
+        assert _symmetry_error(lam) <= 2e-15, name
+        assert np.all(np.abs(lam.sum(axis=1) - 1) <= 1e-15)
+        if N == 3:
+            cen = np.all(np.abs(np.sort(lam, axis=1) - [0, 1 / 3, 1 / 3, 1 / 3]) <= 2e-15, axis=1)
+            assert cen.sum() == 4, name
+        if N == 4:
+            assert np.any(np.all(np.abs(lam - 0.25) <= 2e-15, axis=1)), name
+
+
+@pytest.mark.parametrize("N", range(3, 7))
+def test_nodes_in_tet_and_on_faces(N):
+    """nodes3d_test.go:68-130 (TestNodes3D: XYZtoRST of the nodes lies in the
+    reference tet, IsValidRST tolerance 1e-10, rsttoabc.go:104-107), extended to
+    P=3..6; and the warp is tangential: nodes of the equispaced set on a face
+    (a zero barycentric) stay on it."""
+    req, seq, teq = nudg.equi_nodes3d(N)
+    lam_eq = _bary(req, seq, teq)
+    for name, (r, s, t) in _node_sets(N).items():
+        tol = 1e-10
+        assert np.all((r >= -1 - tol) & (s >= -1 - tol) & (t >= -1 - tol) & (r + s + t <= -1 + tol)), name
+        lam = _bary(r, s, t)
+        zero = np.abs(lam_eq) <= 1e-15
+        assert np.all(np.abs(lam[zero]) <= 1e-14), name
+
+
+def test_evalwarp_zero_at_endpoints():
+    """nodes3d_test.go:176-206 (TestEvalwarp) at P=3..6."""
+    for P in range(3, 7):
+        xeq = np.linspace(-1.0, 1.0, P + 1)
+        w = nudg._evalwarp(P, -nudg.jacobi_gl(0.0, 0.0, P), xeq)
+        assert abs(w[0]) <= 1e-14 and abs(w[P]) <= 1e-14
+
+
+def _textbook_pairing(p, gauss_x, xs):
+    xeq = -1.0 + 2.0 * np.arange(p + 1) / p
+    c = np.polynomial.polynomial.polyfit(xeq, -gauss_x - xeq, p)  # ascending GL
+    return np.polynomial.polynomial.polyval(xs, c) * (1.0 - xs * xs)
+
+
+def _textbook_divide(p, gauss_x, xs):
+    xeq = -1.0 + 2.0 * np.arange(p + 1) / p
+    c = np.polynomial.polynomial.polyfit(xeq, gauss_x - xeq, p)
+    inner = np.abs(xs) < 1 - 1e-10
+    return np.where(inner, np.polynomial.polynomial.polyval(xs, c) / np.where(inner, 1 - xs * xs, 1), 0.0)
+
+
+@pytest.mark.parametrize("mutant", ["pairing", "divide", "alpha_1based"])
+def test_node_pins_reject_textbook_corrections(monkeypatch, mutant):
+    """Planted on the CPU: each plausible 'correction' of nodes3d.go toward the
+    textbook fails at least one of the pins above."""
+    if mutant == "pairing":
+        monkeypatch.setattr(nudg, "_evalwarp", _textbook_pairing)
+    elif mutant == "divide":
+        monkeypatch.setattr(nudg, "_evalwarp", _textbook_divide)
+    else:
+        monkeypatch.setattr(nudg, "_ALPHASTORE", (0.0,) + nudg._ALPHASTORE[:-1])
+    failures = 0
+    for N in (3, 4):
+        lam = _bary(*nudg.xyz_to_rst(*nudg.nodes3d(N)))
+        failures += _edge_error(lam, N) > 2e-15
+        if N == 4:
+            failures += _face_p4_error(lam) > 2e-15
+    assert failures >= 1
 
 
 @pytest.mark.parametrize("N", range(1, 7))
