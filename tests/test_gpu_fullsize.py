@@ -52,7 +52,7 @@ def test_c3_full_size_conservation_and_graph_replay(device):
 def test_c4_full_size_conservation_and_mixed_precision(device):
     """64^3 x 6 = 1,572,864 periodic tets, P=6, 2 slab partitions: two FP64
     steps conserve the integral; BF16x3 contractions with FP64 LIFT stay within
-    the BF16x3 bound of the FP64 run, eager and graph-replayed alike."""
+    5e-6 of the FP64 run (10x the measured error), eager and graph-replayed alike."""
     m = M.cube_mesh(64, 64, 64, periodic=True)
     st = dg.make_setup(m, 6, Kp=M.slab_partition(m.K, 2))
     x, y, z = st.node_coords()
@@ -76,5 +76,6 @@ def test_c4_full_size_conservation_and_mixed_precision(device):
         out.append(lo.state())
     e = rel_l2(out[0], ur)
     print(f"C4 BF16x3 (FP64 LIFT) 2 steps: rel L2 {e:.3e} vs FP64")
-    assert 0.0 < e <= 5e-3
+    # measured 5.2e-7 on MI355X; a plain-BF16 contraction would be >= 30x that
+    assert 0.0 < e <= 5e-6
     assert np.array_equal(out[0], out[1])

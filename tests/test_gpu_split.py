@@ -74,8 +74,20 @@ def test_p6_ssprk3_vs_oracle(device):
 # The contractions round the operators and the linear-part remainders to the
 # matrix precision; the state error after a few steps scales with dt times
 # that rounding. On this coarse mesh (h = 1/4) the remainder u - Lu is not
-# small, so the errors sit near the precision's unit round-off.
-BOUND = {"f32": 5e-5, "bf16x3": 5e-3, "bf16x6": 5e-5}
+# small, and at P >= 5 the reference's node set makes the operators badly
+# conditioned (row sums of |Dr| up to ~3800 at P=6), so the error grows with
+# P. Each bound is about 10x the error measured on MI355X (10 steps, 4^3x6
+# periodic tets, rel L2 vs the FP64 run):
+#   f32     P=3 2.3e-8   P=5 2.8e-7   P=6 4.5e-6
+#   bf16x3  P=3 9.9e-7   P=5 4.9e-5   P=6 5.7e-4
+#   bf16x6  P=3 7.1e-9   P=5 2.7e-7   P=6 2.0e-6
+# A contraction that dropped a split part (plain BF16 operands) carries at
+# least ~30x the BF16x3 error (tests/test_precision_model.py), so it fails.
+BOUND = {
+    (3, "f32"): 3e-7, (5, "f32"): 3e-6, (6, "f32"): 5e-5,
+    (3, "bf16x3"): 1e-5, (5, "bf16x3"): 5e-4, (6, "bf16x3"): 5e-3,
+    (3, "bf16x6"): 1e-7, (5, "bf16x6"): 3e-6, (6, "bf16x6"): 2e-5,
+}
 
 
 @pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
@@ -95,7 +107,7 @@ def test_mixed_precision_stage_vs_fp64(device, N, prec):
     e2 = rel_l2(a, b)
     emax = float(np.max(np.abs(a - b)))
     print(f"P={N} {prec} 10 steps: rel L2 {e2:.3e}, max {emax:.3e}")
-    assert 0.0 < e2 <= BOUND[prec]
+    assert 0.0 < e2 <= BOUND[N, prec]
 
 
 @pytest.mark.parametrize("prec", ["f32", "bf16x3"])
@@ -118,7 +130,8 @@ def test_lift_fp64_flag(device, prec):
         s.step(5)
         out[name] = s.state()
     e = rel_l2(out["lo_l64"], out["ref"])
-    assert 0.0 < e <= BOUND[prec]
+    print(f"P=6 {prec} with FP64 LIFT, 5 steps: rel L2 {e:.3e}")
+    assert 0.0 < e <= BOUND[N, prec]
     assert not np.array_equal(out["lo_l64"], out["lo"])
 
 
@@ -148,7 +161,7 @@ def test_split_boundary_faces_vs_oracle(device, N):
         lo.step(10)
         e2 = rel_l2(lo.state(), ref.state())
         print(f"P={N} {prec} boundary mesh, 10 steps: rel L2 {e2:.3e}")
-        assert 0.0 < e2 <= 2.0 * BOUND[prec]
+        assert 0.0 < e2 <= BOUND[N, prec]
 
 
 def test_split_graph_replay(device):
