@@ -37,10 +37,6 @@ class DgkError(RuntimeError):
         self.code = code
 
 
-class KernelConsts(C.Structure):
-    _fields_ = [("npart", C.c_int), ("kpart_max", C.c_int), ("int_size", C.c_int), ("np", C.c_int)]
-
-
 class _ArgVal(C.Union):
     _fields_ = [("mem", C.c_void_p), ("f64", C.c_double), ("f32", C.c_float), ("i64", C.c_int64),
                 ("i32", C.c_int32)]
@@ -80,7 +76,8 @@ _SIGS = {
     "dgk_copy_d2d": (_i32, [_vp, _vp, _vp, _sz]),
     "dgk_fill_bytes": (_i32, [_vp, _vp, _i32, _sz]),
     "dgk_kernel_list": (_i32, [C.c_char_p, _sz]),
-    "dgk_kernel_get": (_i32, [_vp, C.c_char_p, C.c_char_p, C.POINTER(KernelConsts), C.POINTER(_vp)]),
+    "dgk_kernel_build_from_source": (_i32, [_vp, C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
+    "dgk_kernel_describe": (_i32, [C.c_char_p, C.c_char_p, C.c_char_p, _sz]),
     "dgk_kernel_launch": (_i32, [_vp, _vp, _i32, C.POINTER(Arg)]),
     "dgk_kernel_free": (_i32, [_vp]),
     "dgk_graph_begin": (_i32, [_vp]),
@@ -116,7 +113,6 @@ _SIGS = {
     "dgk_burgers_set_variant": (_i32, [_vp, _i32]),
     "dgk_burgers_set_precision": (_i32, [_vp, _i32]),
     "dgk_burgers_set_order": (_i32, [_vp, _i32]),
-    "dgk_kernel_set_matrix": (_i32, [_vp, C.c_char_p, _i32, _i32, _pf64]),
     "dgk_av_create": (_i32, [_vp, _pf64, C.POINTER(AVParams), C.POINTER(_vp)]),
     "dgk_av_destroy": (_i32, [_vp]),
     "dgk_av_eps_dev": (_i32, [_vp, C.POINTER(_vp)]),
@@ -343,5 +339,56 @@ class Graph:
         try:
             if alive(self.dev):
                 load().dgk_graph_destroy(self.h)
+        except Exception:
+            pass
+
+
+def describe_kernel(source: str, name: str) -> str:
+    """The statement program libdgk would run for kernel ``name`` of ``source``
+    (parse only; needs no device)."""
+    buf = C.create_string_buffer(1 << 16)
+    call("dgk_kernel_describe", source.encode(), name.encode(), buf, C.c_size_t(len(buf)))
+    return buf.value.decode()
+
+
+class Kernel:
+    """A kernel built from the runner's full OKL text (``BuildKernelFromString``
+    replacement); ``launch`` is ``RunWithArgs`` with one value per signature
+    parameter: DeviceBuffer (or a raw device pointer) for pointers, and
+    (kind, value) pairs for scalars, e.g. (ARG_F64, 0.5)."""
+
+    def __init__(self, dev: Device, source: str, name: str):
+        h = C.c_void_p()
+        call("dgk_kernel_build_from_source", dev.h, source.encode(), name.encode(), C.byref(h))
+        self.h = h
+        self.dev = dev
+        self.name = name
+
+    def launch(self, args, host_K: np.ndarray | None = None) -> None:
+        arr = (Arg * len(args))()
+        for i, a in enumerate(args):
+            if isinstance(a, tuple):
+                kind, val = a
+                arr[i].kind = kind
+                field = {ARG_F64: "f64", ARG_F32: "f32", ARG_I64: "i64", ARG_I32: "i32",
+                         ARG_MEM: "mem"}[kind]
+                setattr(arr[i].v, field, val)
+            else:
+                arr[i].kind = ARG_MEM
+                arr[i].v.mem = a.ptr.value if isinstance(a, DeviceBuffer) else a
+        hk = None
+        if host_K is not None:
+            host_K = np.ascontiguousarray(host_K)
+            hk = ptr(host_K)
+        call("dgk_kernel_launch", self.h, hk, len(args), arr)
+
+    def free(self) -> None:
+        if self.h and alive(self.dev):
+            call("dgk_kernel_free", self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass

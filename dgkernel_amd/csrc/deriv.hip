@@ -1,4 +1,5 @@
-// Physical derivative and MATMUL kernels for gfx950.
+// Physical derivative and MATMUL kernels for gfx950 over the runner's pooled
+// partition layout.
 //
 // Reference semantics: the OKL kernel "differentiate"
 // (tetnudg/tet_nudg_mesh_physical_derivative_test.go:850-892) computes
@@ -9,82 +10,132 @@
 // and stride-Np loads. Here one block owns a tile of elements of one
 // partition: the tile's u is staged through LDS with contiguous (coalesced)
 // loads, Dr/Ds/Dt sit in LDS, each thread produces one node of one element
-// for a group of elements (register blocking over GROUP elements reuses each
-// D row), and the per-node metrics and outputs stream with unit stride.
-// Nothing goes back to HBM except the three outputs.
-#include "dgk_internal.h"
+// for a group of elements (register blocking over EPG elements reuses each
+// D row), and the per-node metrics and outputs stream with unit stride. The
+// temporaries Ur/Us/Ut are written only when the caller names them (a
+// source-built "differentiate" kernel does, the direct entry point does not).
+//
+// The operators are read through strides, D(n, j) = D[n*rs + j*cs], so the
+// runner's column-major device matrices (runner/runner.go:395-400) are used
+// in place.
+#include "parts.h"
+
+namespace dgk {
+
+int build_part_tiles(dgk_device dev, int npart, const int64_t* K, int per_tile, const void* kdev,
+                     int ksize, PartTiles& pt, dim3& grid) {
+  DGK_REQUIRE(npart >= 1, "partition count %d < 1", npart);
+  DGK_REQUIRE(per_tile >= 1, "tile size %d < 1", per_tile);
+  pt.npart = npart;
+  pt.mode2d = 0;
+  pt.kdev = nullptr;
+  pt.ksize = 8;
+  int64_t kmax = 0, tiles = 0;
+  for (int p = 0; p < npart; ++p) {
+    DGK_REQUIRE(K[p] >= 0 && K[p] < (1LL << 31), "K[%d] = %lld out of range", p, (long long)K[p]);
+    kmax = std::max<int64_t>(kmax, K[p]);
+    tiles += (K[p] + per_tile - 1) / per_tile;
+  }
+  if (npart <= kMaxPartsByValue) {
+    DGK_REQUIRE(tiles < (1LL << 31), "too many tiles (%lld)", (long long)tiles);
+    int t = 0;
+    for (int p = 0; p < npart; ++p) {
+      pt.start[p] = t;
+      pt.K[p] = (int)K[p];
+      t += (int)((K[p] + per_tile - 1) / per_tile);
+    }
+    pt.start[npart] = t;
+    grid = dim3((unsigned)t, 1, 1);
+    return DGK_OK;
+  }
+  DGK_REQUIRE(npart <= 65535, "partition count %d above 65535", npart);
+  pt.mode2d = 1;
+  if (kdev) {
+    DGK_REQUIRE(ksize == 4 || ksize == 8, "K element size %d", ksize);
+    pt.kdev = kdev;
+    pt.ksize = ksize;
+  } else {
+    size_t bytes = (size_t)npart * 8;
+    if (dev->kcache_bytes < bytes) {
+      if (dev->kcache) DGK_HIP(hipFree(dev->kcache));
+      dev->kcache = nullptr;
+      dev->kcache_bytes = 0;
+      DGK_HIP(hipMalloc(&dev->kcache, bytes));
+      dev->kcache_bytes = bytes;
+    }
+    DGK_HIP(hipMemcpyAsync(dev->kcache, K, bytes, hipMemcpyHostToDevice, dev->stream));
+    DGK_HIP(hipStreamSynchronize(dev->stream));
+    pt.kdev = dev->kcache;
+    pt.ksize = 8;
+  }
+  grid = dim3((unsigned)((kmax + per_tile - 1) / per_tile), (unsigned)npart, 1);
+  return DGK_OK;
+}
+
+}  // namespace dgk
 
 namespace {
 
-constexpr int kMaxPartsByValue = 256;
-
-struct PartTiles {
-  int npart;
-  int tiles_per_part_start[kMaxPartsByValue + 1];  // prefix sum of tiles
-  int K[kMaxPartsByValue];                          // elements per partition
-};
-
-__device__ __forceinline__ void find_tile(const PartTiles& pt, int tile, int per_tile, int& part,
-                                          int& first, int& count) {
-  int lo = 0, hi = pt.npart;  // last p with start[p] <= tile
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (pt.tiles_per_part_start[mid] <= tile) lo = mid; else hi = mid;
-  }
-  part = lo;
-  first = (tile - pt.tiles_per_part_start[lo]) * per_tile;
-  int rem = pt.K[lo] - first;
-  count = rem < per_tile ? rem : per_tile;
-}
-
-int build_part_tiles(int npart, const int64_t* K, int per_tile, PartTiles& pt, int& ntiles) {
-  DGK_REQUIRE(npart >= 1 && npart <= kMaxPartsByValue,
-              "partition count %d outside 1..%d", npart, kMaxPartsByValue);
-  pt.npart = npart;
-  int t = 0;
-  for (int p = 0; p < npart; ++p) {
-    DGK_REQUIRE(K[p] >= 0 && K[p] < (1LL << 31), "K[%d] = %lld out of range", p, (long long)K[p]);
-    pt.tiles_per_part_start[p] = t;
-    pt.K[p] = (int)K[p];
-    t += (int)((K[p] + per_tile - 1) / per_tile);
-  }
-  pt.tiles_per_part_start[npart] = t;
-  ntiles = t;
-  return DGK_OK;
-}
+using dgk::PartTiles;
+using dgk::find_tile;
 
 template <typename T>
 __device__ __forceinline__ T load_as(const void* p, int64_t i) {
   return static_cast<const T*>(p)[i];
 }
 
-struct Metrics9 {
+// Arrays of one physical-derivative launch. Each array has its own
+// partition offsets (value offsets, OFF wide), as the runner passes them.
+struct NodalArgs {
+  const double* D[3];  // Dr, Ds, Dt; D_q(n, j) = D[q][n*drs + j*dcs]
+  int drs, dcs;
+  const double* u;
+  const void* u_off;
   const void* m[9];  // rx ry rz sx sy sz tx ty tz
+  const void* m_off[9];
+  void* o[3];  // du/dx, du/dy, du/dz
+  const void* o_off[3];
+  double* t[3];  // Ur, Us, Ut (optional)
+  const void* t_off[3];
 };
 
-// One block: GROUP*EPG elements of one partition (EPG = elements per thread
-// group); threads = NP * GROUPS_PAD, thread (node n, group g) handles node n
-// of elements g, g+G, ..., g+(EPG-1)G.
+template <typename MT, typename OT, typename OFF>
+__device__ __forceinline__ void store_node(const NodalArgs& a, int part, int64_t li, double ar,
+                                           double as, double at) {
+  double m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q)
+    m[q] = (double)load_as<MT>(a.m[q], (int64_t)((const OFF*)a.m_off[q])[part] + li);
+  ((OT*)a.o[0])[(int64_t)((const OFF*)a.o_off[0])[part] + li] = (OT)(m[0] * ar + m[3] * as + m[6] * at);
+  ((OT*)a.o[1])[(int64_t)((const OFF*)a.o_off[1])[part] + li] = (OT)(m[1] * ar + m[4] * as + m[7] * at);
+  ((OT*)a.o[2])[(int64_t)((const OFF*)a.o_off[2])[part] + li] = (OT)(m[2] * ar + m[5] * as + m[8] * at);
+  if (a.t[0]) {
+    a.t[0][(int64_t)((const OFF*)a.t_off[0])[part] + li] = ar;
+    a.t[1][(int64_t)((const OFF*)a.t_off[1])[part] + li] = as;
+    a.t[2][(int64_t)((const OFF*)a.t_off[2])[part] + li] = at;
+  }
+}
+
+// One block: G*EPG elements of one partition; threads = NP * G (padded to a
+// wave multiple), thread (node n, group g) handles node n of elements g,
+// g+G, ..., g+(EPG-1)G.
 template <int NP, int G, int EPG, typename MT, typename OT, typename OFF>
 __global__ void __launch_bounds__(256)
-deriv_nodal_kernel(PartTiles pt, const double* __restrict__ Dr, const double* __restrict__ Ds,
-                   const double* __restrict__ Dt, const double* __restrict__ u,
-                   const OFF* __restrict__ u_off, Metrics9 met, const OFF* __restrict__ m_off,
-                   OT* __restrict__ ox, OT* __restrict__ oy, OT* __restrict__ oz,
-                   const OFF* __restrict__ o_off) {
+deriv_nodal_kernel(PartTiles pt, NodalArgs a) {
   constexpr int E = G * EPG;
   __shared__ double sD[3 * NP * NP];
   __shared__ double sU[E * NP];
   int part, first, count;
-  find_tile(pt, blockIdx.x, E, part, first, count);
+  if (!find_tile(pt, E, part, first, count)) return;
   const int tid = threadIdx.x;
   for (int i = tid; i < 3 * NP * NP; i += blockDim.x) {
-    int m = i / (NP * NP), r = i - m * NP * NP;
-    sD[i] = (m == 0 ? Dr : (m == 1 ? Ds : Dt))[r];
+    int q = i / (NP * NP), r = i - q * NP * NP;
+    int n = r / NP, j = r - n * NP;
+    sD[i] = a.D[q][n * a.drs + j * a.dcs];
   }
-  const int64_t ub = (int64_t)u_off[part] + (int64_t)first * NP;
+  const int64_t ub = (int64_t)((const OFF*)a.u_off)[part] + (int64_t)first * NP;
   const int nval = count * NP;
-  for (int i = tid; i < nval; i += blockDim.x) sU[i] = u[ub + i];
+  for (int i = tid; i < nval; i += blockDim.x) sU[i] = a.u[ub + i];
   __syncthreads();
   const int n = tid % NP, g = tid / NP;
   if (g >= G) return;
@@ -96,30 +147,20 @@ deriv_nodal_kernel(PartTiles pt, const double* __restrict__ Dr, const double* __
   const double* dt = sD + 2 * NP * NP + n * NP;
 #pragma unroll 5
   for (int j = 0; j < NP; ++j) {
-    double a = dr[j], b = ds[j], c = dt[j];
+    double x = dr[j], y = ds[j], z = dt[j];
 #pragma unroll
     for (int q = 0; q < EPG; ++q) {
       double uj = sU[(g + q * G) * NP + j];
-      ar[q] = fma(a, uj, ar[q]);
-      as[q] = fma(b, uj, as[q]);
-      at[q] = fma(c, uj, at[q]);
+      ar[q] = fma(x, uj, ar[q]);
+      as[q] = fma(y, uj, as[q]);
+      at[q] = fma(z, uj, at[q]);
     }
   }
-  const int64_t mb = (int64_t)m_off[part] + (int64_t)first * NP;
-  const int64_t obase = (int64_t)o_off[part] + (int64_t)first * NP;
 #pragma unroll
   for (int q = 0; q < EPG; ++q) {
     int e = g + q * G;
     if (e >= count) continue;
-    int64_t mi = mb + e * NP + n, oi = obase + e * NP + n;
-    double rx = (double)load_as<MT>(met.m[0], mi), ry = (double)load_as<MT>(met.m[1], mi),
-           rz = (double)load_as<MT>(met.m[2], mi), sx = (double)load_as<MT>(met.m[3], mi),
-           sy = (double)load_as<MT>(met.m[4], mi), sz = (double)load_as<MT>(met.m[5], mi),
-           tx = (double)load_as<MT>(met.m[6], mi), ty = (double)load_as<MT>(met.m[7], mi),
-           tz = (double)load_as<MT>(met.m[8], mi);
-    ox[oi] = (OT)(rx * ar[q] + sx * as[q] + tx * at[q]);
-    oy[oi] = (OT)(ry * ar[q] + sy * as[q] + ty * at[q]);
-    oz[oi] = (OT)(rz * ar[q] + sz * as[q] + tz * at[q]);
+    store_node<MT, OT, OFF>(a, part, (int64_t)(first + e) * NP + n, ar[q], as[q], at[q]);
   }
 }
 
@@ -127,38 +168,25 @@ deriv_nodal_kernel(PartTiles pt, const double* __restrict__ Dr, const double* __
 // u is staged in LDS.
 template <int NP, int G, typename MT, typename OT, typename OFF>
 __global__ void __launch_bounds__(256)
-deriv_nodal_big_kernel(PartTiles pt, const double* __restrict__ Dr, const double* __restrict__ Ds,
-                       const double* __restrict__ Dt, const double* __restrict__ u,
-                       const OFF* __restrict__ u_off, Metrics9 met, const OFF* __restrict__ m_off,
-                       OT* __restrict__ ox, OT* __restrict__ oy, OT* __restrict__ oz,
-                       const OFF* __restrict__ o_off) {
-  constexpr int E = G;
-  __shared__ double sU[E * NP];
+deriv_nodal_big_kernel(PartTiles pt, NodalArgs a) {
+  __shared__ double sU[G * NP];
   int part, first, count;
-  find_tile(pt, blockIdx.x, E, part, first, count);
+  if (!find_tile(pt, G, part, first, count)) return;
   const int tid = threadIdx.x;
-  const int64_t ub = (int64_t)u_off[part] + (int64_t)first * NP;
-  for (int i = tid; i < count * NP; i += blockDim.x) sU[i] = u[ub + i];
+  const int64_t ub = (int64_t)((const OFF*)a.u_off)[part] + (int64_t)first * NP;
+  for (int i = tid; i < count * NP; i += blockDim.x) sU[i] = a.u[ub + i];
   __syncthreads();
   for (int w = tid; w < count * NP; w += blockDim.x) {
     int e = w / NP, n = w - e * NP;
     double ar = 0, as = 0, at = 0;
     for (int j = 0; j < NP; ++j) {
       double uj = sU[e * NP + j];
-      ar = fma(Dr[n * NP + j], uj, ar);
-      as = fma(Ds[n * NP + j], uj, as);
-      at = fma(Dt[n * NP + j], uj, at);
+      int di = n * a.drs + j * a.dcs;
+      ar = fma(a.D[0][di], uj, ar);
+      as = fma(a.D[1][di], uj, as);
+      at = fma(a.D[2][di], uj, at);
     }
-    int64_t mi = (int64_t)m_off[part] + (int64_t)first * NP + w;
-    int64_t oi = (int64_t)o_off[part] + (int64_t)first * NP + w;
-    double rx = (double)load_as<MT>(met.m[0], mi), ry = (double)load_as<MT>(met.m[1], mi),
-           rz = (double)load_as<MT>(met.m[2], mi), sx = (double)load_as<MT>(met.m[3], mi),
-           sy = (double)load_as<MT>(met.m[4], mi), sz = (double)load_as<MT>(met.m[5], mi),
-           tx = (double)load_as<MT>(met.m[6], mi), ty = (double)load_as<MT>(met.m[7], mi),
-           tz = (double)load_as<MT>(met.m[8], mi);
-    ox[oi] = (OT)(rx * ar + sx * as + tx * at);
-    oy[oi] = (OT)(ry * ar + sy * as + ty * at);
-    oz[oi] = (OT)(rz * ar + sz * as + tz * at);
+    store_node<MT, OT, OFF>(a, part, (int64_t)first * NP + w, ar, as, at);
   }
 }
 
@@ -171,46 +199,38 @@ template <> struct NodalShape<3> { static constexpr int G = 12, EPG = 4; };
 template <> struct NodalShape<4> { static constexpr int G = 7, EPG = 4; };
 
 template <int N, typename MT, typename OT, typename OFF>
-int launch_nodal(dgk_device dev, int npart, const int64_t* K, const double* Dr, const double* Ds,
-                 const double* Dt, const double* u, const OFF* u_off, const void* const* metrics,
-                 const OFF* m_off, void* dx, void* dy, void* dz, const OFF* o_off) {
+int launch_nodal(dgk_device dev, int npart, const int64_t* K, const void* kdev, int ksize,
+                 const NodalArgs& a) {
   constexpr int NP = (N + 1) * (N + 2) * (N + 3) / 6;
-  Metrics9 met;
-  for (int i = 0; i < 9; ++i) met.m[i] = metrics[i];
   PartTiles pt;
-  int ntiles = 0;
+  dim3 grid;
   if constexpr (N <= 4) {
     constexpr int G = NodalShape<N>::G, EPG = NodalShape<N>::EPG;
-    int rc = build_part_tiles(npart, K, G * EPG, pt, ntiles);
+    int rc = dgk::build_part_tiles(dev, npart, K, G * EPG, kdev, ksize, pt, grid);
     if (rc) return rc;
-    if (ntiles == 0) return DGK_OK;
+    if (grid.x == 0) return DGK_OK;
     int threads = ((NP * G + 63) / 64) * 64;
-    hipLaunchKernelGGL((deriv_nodal_kernel<NP, G, EPG, MT, OT, OFF>), dim3(ntiles), dim3(threads), 0,
-                       dev->stream, pt, Dr, Ds, Dt, u, u_off, met, m_off, (OT*)dx, (OT*)dy, (OT*)dz,
-                       o_off);
+    hipLaunchKernelGGL((deriv_nodal_kernel<NP, G, EPG, MT, OT, OFF>), grid, dim3(threads), 0,
+                       dev->stream, pt, a);
   } else {
     constexpr int G = 16;
-    int rc = build_part_tiles(npart, K, G, pt, ntiles);
+    int rc = dgk::build_part_tiles(dev, npart, K, G, kdev, ksize, pt, grid);
     if (rc) return rc;
-    if (ntiles == 0) return DGK_OK;
-    hipLaunchKernelGGL((deriv_nodal_big_kernel<NP, G, MT, OT, OFF>), dim3(ntiles), dim3(256), 0,
-                       dev->stream, pt, Dr, Ds, Dt, u, u_off, met, m_off, (OT*)dx, (OT*)dy, (OT*)dz,
-                       o_off);
+    if (grid.x == 0) return DGK_OK;
+    hipLaunchKernelGGL((deriv_nodal_big_kernel<NP, G, MT, OT, OFF>), grid, dim3(256), 0, dev->stream,
+                       pt, a);
   }
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
 template <typename MT, typename OT, typename OFF>
-int dispatch_nodal_N(int N, dgk_device dev, int npart, const int64_t* K, const double* Dr,
-                     const double* Ds, const double* Dt, const double* u, const OFF* u_off,
-                     const void* const* metrics, const OFF* m_off, void* dx, void* dy, void* dz,
-                     const OFF* o_off) {
+int dispatch_nodal_N(int N, dgk_device dev, int npart, const int64_t* K, const void* kdev,
+                     int ksize, const NodalArgs& a) {
   switch (N) {
-#define DGK_CASE(n)                                                                             \
-  case n:                                                                                       \
-    return launch_nodal<n, MT, OT, OFF>(dev, npart, K, Dr, Ds, Dt, u, u_off, metrics, m_off, dx, \
-                                        dy, dz, o_off);
+#define DGK_CASE(n) \
+  case n:           \
+    return launch_nodal<n, MT, OT, OFF>(dev, npart, K, kdev, ksize, a);
     DGK_CASE(1) DGK_CASE(2) DGK_CASE(3) DGK_CASE(4) DGK_CASE(5) DGK_CASE(6)
 #undef DGK_CASE
     default:
@@ -219,71 +239,82 @@ int dispatch_nodal_N(int N, dgk_device dev, int npart, const int64_t* K, const d
   }
 }
 
-// Generic MATMUL over partitions: thread per (element, row).
-template <typename OFF>
+// MATMUL over partitions, thread per (element, row):
+//   OUT[e*rows + i] (= | +=) sum_j A(i,j) IN[e*cols + j],  A(i,j) = A[i*rs + j*cs].
+// The sum runs in the accumulator type AT of the runner's macro ("double sum"
+// or "float sum", runner/builder/builder.go:355-410): each step adds the
+// product (in the common type of A and IN) to sum and rounds to AT.
+template <typename MT, typename IT, typename OT, typename AT, typename OFF>
 __global__ void __launch_bounds__(256)
-matmul_kernel(PartTiles pt, int per_tile, int rows, int cols, const double* __restrict__ A,
-              const double* __restrict__ in, const OFF* __restrict__ in_off,
-              double* __restrict__ out, const OFF* __restrict__ out_off, int add) {
+matmul_kernel(PartTiles pt, int per_tile, int rows, int cols, const MT* __restrict__ A, int rs,
+              int cs, const IT* __restrict__ in, const OFF* __restrict__ in_off, OT* __restrict__ out,
+              const OFF* __restrict__ out_off, int add) {
   int part, first, count;
-  find_tile(pt, blockIdx.x, per_tile, part, first, count);
-  int64_t ib = (int64_t)in_off[part] + (int64_t)first * cols;
-  int64_t ob = (int64_t)out_off[part] + (int64_t)first * rows;
+  if (!find_tile(pt, per_tile, part, first, count)) return;
+  const int64_t ib = (int64_t)in_off[part] + (int64_t)first * cols;
+  const int64_t ob = (int64_t)out_off[part] + (int64_t)first * rows;
   for (int w = threadIdx.x; w < count * rows; w += blockDim.x) {
     int e = w / rows, i = w - e * rows;
-    double s = 0.0;
-    for (int j = 0; j < cols; ++j) s = fma(A[(int64_t)j * rows + i], in[ib + (int64_t)e * cols + j], s);
-    if (add) out[ob + w] += s; else out[ob + w] = s;
+    AT s = AT(0);
+    for (int j = 0; j < cols; ++j) s = (AT)(s + A[(int64_t)i * rs + (int64_t)j * cs] * in[ib + (int64_t)e * cols + j]);
+    if (add) out[ob + w] = (OT)(out[ob + w] + s); else out[ob + w] = (OT)s;
   }
 }
 
-// OUT[e*rows + i] = s[e*rows + i] * sum_j A(i,j) IN[e*cols + j], A column-major
-// (the runner's device matrix layout); s may be NULL (plain MATMUL).
-template <typename OFF, typename ST>
-__global__ void __launch_bounds__(256)
-scaled_matmul_kernel(PartTiles pt, int per_tile, int rows, int cols, const double* __restrict__ A,
-                     const double* __restrict__ in, const OFF* __restrict__ in_off,
-                     const ST* __restrict__ sc, const OFF* __restrict__ sc_off,
-                     double* __restrict__ out, const OFF* __restrict__ out_off) {
-  int part, first, count;
-  find_tile(pt, blockIdx.x, per_tile, part, first, count);
-  int64_t ib = (int64_t)in_off[part] + (int64_t)first * cols;
-  int64_t ob = (int64_t)out_off[part] + (int64_t)first * rows;
-  int64_t sb = sc ? (int64_t)sc_off[part] + (int64_t)first * rows : 0;
-  for (int w = threadIdx.x; w < count * rows; w += blockDim.x) {
-    int e = w / rows, i = w - e * rows;
-    double s = 0.0;
-    for (int j = 0; j < cols; ++j) s = fma(A[(int64_t)j * rows + i], in[ib + (int64_t)e * cols + j], s);
-    if (sc) s *= (double)sc[sb + w];
-    out[ob + w] = s;
-  }
-}
-
-__global__ void transpose_kernel(int rows, int cols, const double* __restrict__ in_colmajor,
-                                 double* __restrict__ out_rowmajor) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= rows * cols) return;
-  int i = w / cols, j = w - i * cols;
-  out_rowmajor[w] = in_colmajor[(int64_t)j * rows + i];
+template <typename MT, typename IT, typename OT, typename AT>
+int launch_matmul_t(dgk_device dev, const PartTiles& pt, dim3 grid, int per_tile, int rows, int cols,
+                    const void* A, int rs, int cs, const void* in, const void* in_off, void* out,
+                    const void* out_off, int add, int off_size) {
+  if (off_size == 8)
+    hipLaunchKernelGGL((matmul_kernel<MT, IT, OT, AT, int64_t>), grid, dim3(256), 0, dev->stream, pt,
+                       per_tile, rows, cols, (const MT*)A, rs, cs, (const IT*)in,
+                       (const int64_t*)in_off, (OT*)out, (const int64_t*)out_off, add);
+  else
+    hipLaunchKernelGGL((matmul_kernel<MT, IT, OT, AT, int32_t>), grid, dim3(256), 0, dev->stream, pt,
+                       per_tile, rows, cols, (const MT*)A, rs, cs, (const IT*)in,
+                       (const int32_t*)in_off, (OT*)out, (const int32_t*)out_off, add);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
 }
 
 }  // namespace
 
 namespace dgk {
 
-int phys_deriv_nodal_any(dgk_device dev, int N, int npart, const int64_t* K, const double* Dr,
-                         const double* Ds, const double* Dt, const double* u, const void* u_off,
-                         const void* const* metrics, int mtype, const void* m_off, void* dx,
-                         void* dy, void* dz, int otype, const void* o_off, int off_size) {
-  DGK_REQUIRE(dev && K && Dr && Ds && Dt && u && u_off && metrics && m_off && dx && dy && dz &&
-                  o_off,
+// Physical derivative over partitions. D rows/cols through (drs, dcs);
+// per-array offsets; temps t[3] (and t_off) may be NULL.
+int phys_deriv_nodal_full(dgk_device dev, int N, int npart, const int64_t* K, const void* kdev,
+                          int ksize, const double* const* D, int drs, int dcs, const double* u,
+                          const void* u_off, const void* const* metrics, const void* const* m_off,
+                          int mtype, void* const* outs, const void* const* o_off, int otype,
+                          double* const* temps, const void* const* t_off, int off_size) {
+  DGK_REQUIRE(dev && K && D && u && u_off && metrics && m_off && outs && o_off,
               "physical derivative: NULL argument");
-  for (int i = 0; i < 9; ++i) DGK_REQUIRE(metrics[i], "physical derivative: metric %d is NULL", i);
+  NodalArgs a{};
+  for (int q = 0; q < 3; ++q) {
+    DGK_REQUIRE(D[q], "physical derivative: operator %d is NULL", q);
+    DGK_REQUIRE(outs[q] && o_off[q], "physical derivative: output %d or its offsets is NULL", q);
+    a.D[q] = D[q];
+    a.o[q] = outs[q];
+    a.o_off[q] = o_off[q];
+    a.t[q] = temps ? temps[q] : nullptr;
+    a.t_off[q] = t_off ? t_off[q] : nullptr;
+    DGK_REQUIRE(!temps || (temps[q] && t_off && t_off[q]),
+                "physical derivative: temporary %d or its offsets is NULL", q);
+  }
+  for (int i = 0; i < 9; ++i) {
+    DGK_REQUIRE(metrics[i] && m_off[i], "physical derivative: metric %d or its offsets is NULL", i);
+    a.m[i] = metrics[i];
+    a.m_off[i] = m_off[i];
+  }
+  a.drs = drs;
+  a.dcs = dcs;
+  a.u = u;
+  a.u_off = u_off;
   DGK_REQUIRE(mtype == DGK_F64 || mtype == DGK_F32, "physical derivative: bad metric type");
   DGK_REQUIRE(otype == DGK_F64 || otype == DGK_F32, "physical derivative: bad output type");
-#define DGK_GO(MT, OT, OFF)                                                                   \
-  return dispatch_nodal_N<MT, OT, OFF>(N, dev, npart, K, Dr, Ds, Dt, u, (const OFF*)u_off,    \
-                                       metrics, (const OFF*)m_off, dx, dy, dz, (const OFF*)o_off)
+  DGK_REQUIRE(off_size == 8 || off_size == 4, "physical derivative: offset size %d", off_size);
+#define DGK_GO(MT, OT, OFF) return dispatch_nodal_N<MT, OT, OFF>(N, dev, npart, K, kdev, ksize, a)
   if (off_size == 8) {
     if (mtype == DGK_F64 && otype == DGK_F64) DGK_GO(double, double, int64_t);
     if (mtype == DGK_F32 && otype == DGK_F64) DGK_GO(float, double, int64_t);
@@ -298,58 +329,47 @@ int phys_deriv_nodal_any(dgk_device dev, int N, int npart, const int64_t* K, con
 #undef DGK_GO
 }
 
-int matmul_any(dgk_device dev, int rows, int cols, const double* A, int npart, const int64_t* K,
-               const double* in, const void* in_off, double* out, const void* out_off, int add,
-               int off_size) {
-  DGK_REQUIRE(dev && A && K && in && in_off && out && out_off, "matmul: NULL argument");
+// Typed MATMUL: mtype/itype/otype/atype are DGK_F64 or DGK_F32.
+int matmul_typed(dgk_device dev, int npart, const int64_t* K, const void* kdev, int ksize, int rows,
+                 int cols, const void* A, int rs, int cs, int mtype, const void* in,
+                 const void* in_off, int itype, void* out, const void* out_off, int otype,
+                 int atype, int add, int off_size) {
+  DGK_REQUIRE(dev && K && A && in && in_off && out && out_off, "matmul: NULL argument");
   DGK_REQUIRE(rows > 0 && cols > 0, "matmul: bad shape %dx%d", rows, cols);
+  DGK_REQUIRE(off_size == 8 || off_size == 4, "matmul: offset size %d", off_size);
   int per_tile = rows >= 256 ? 1 : 256 / rows;
   PartTiles pt;
-  int ntiles = 0;
-  int rc = build_part_tiles(npart, K, per_tile, pt, ntiles);
+  dim3 grid;
+  int rc = build_part_tiles(dev, npart, K, per_tile, kdev, ksize, pt, grid);
   if (rc) return rc;
-  if (!ntiles) return DGK_OK;
-  if (off_size == 8)
-    hipLaunchKernelGGL(matmul_kernel<int64_t>, dim3(ntiles), dim3(256), 0, dev->stream, pt, per_tile,
-                       rows, cols, A, in, (const int64_t*)in_off, out, (const int64_t*)out_off, add);
-  else
-    hipLaunchKernelGGL(matmul_kernel<int32_t>, dim3(ntiles), dim3(256), 0, dev->stream, pt, per_tile,
-                       rows, cols, A, in, (const int32_t*)in_off, out, (const int32_t*)out_off, add);
-  DGK_LAUNCH_CHECK();
-  return DGK_OK;
-}
-
-int scaled_matmul_any(dgk_device dev, int rows, int cols, const double* A_colmajor, int npart,
-                      const int64_t* K, const double* in, const void* in_off, const void* sc,
-                      int sc_type, const void* sc_off, double* out, const void* out_off,
-                      int off_size) {
-  DGK_REQUIRE(dev && A_colmajor && K && in && in_off && out && out_off, "matmul: NULL argument");
-  int per_tile = rows >= 256 ? 1 : 256 / rows;
-  PartTiles pt;
-  int ntiles = 0;
-  int rc = build_part_tiles(npart, K, per_tile, pt, ntiles);
-  if (rc) return rc;
-  if (!ntiles) return DGK_OK;
-#define DGK_GO(OFF, ST)                                                                          \
-  hipLaunchKernelGGL((scaled_matmul_kernel<OFF, ST>), dim3(ntiles), dim3(256), 0, dev->stream, pt, \
-                     per_tile, rows, cols, A_colmajor, in, (const OFF*)in_off, (const ST*)sc,      \
-                     (const OFF*)sc_off, out, (const OFF*)out_off)
-  if (off_size == 8) {
-    if (sc_type == DGK_F32) DGK_GO(int64_t, float); else DGK_GO(int64_t, double);
-  } else {
-    if (sc_type == DGK_F32) DGK_GO(int32_t, float); else DGK_GO(int32_t, double);
+  if (grid.x == 0) return DGK_OK;
+  const int code = (mtype == DGK_F32) * 8 + (itype == DGK_F32) * 4 + (otype == DGK_F32) * 2 +
+                   (atype == DGK_F32);
+  switch (code) {
+#define DGK_MM(c, MT, IT, OT, AT)                                                               \
+  case c:                                                                                       \
+    return launch_matmul_t<MT, IT, OT, AT>(dev, pt, grid, per_tile, rows, cols, A, rs, cs, in,  \
+                                           in_off, out, out_off, add, off_size);
+    DGK_MM(0, double, double, double, double)
+    DGK_MM(1, double, double, double, float)
+    DGK_MM(2, double, double, float, double)
+    DGK_MM(3, double, double, float, float)
+    DGK_MM(4, double, float, double, double)
+    DGK_MM(5, double, float, double, float)
+    DGK_MM(6, double, float, float, double)
+    DGK_MM(7, double, float, float, float)
+    DGK_MM(8, float, double, double, double)
+    DGK_MM(9, float, double, double, float)
+    DGK_MM(10, float, double, float, double)
+    DGK_MM(11, float, double, float, float)
+    DGK_MM(12, float, float, double, double)
+    DGK_MM(13, float, float, double, float)
+    DGK_MM(14, float, float, float, double)
+    DGK_MM(15, float, float, float, float)
+#undef DGK_MM
   }
-#undef DGK_GO
-  DGK_LAUNCH_CHECK();
-  return DGK_OK;
-}
-
-int transpose_colmajor(dgk_device dev, int rows, int cols, const double* in, double* out) {
-  int n = rows * cols;
-  hipLaunchKernelGGL(transpose_kernel, dim3((n + 255) / 256), dim3(256), 0, dev->stream, rows, cols,
-                     in, out);
-  DGK_LAUNCH_CHECK();
-  return DGK_OK;
+  dgk::set_error("matmul: bad type code");
+  return DGK_ERR_INVALID;
 }
 
 }  // namespace dgk
@@ -362,9 +382,15 @@ extern "C" int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int6
                                     void* dudy_dev, void* dudz_dev, int otype,
                                     const int64_t* o_off_dev) {
   dgk::clear_error();
-  return dgk::phys_deriv_nodal_any(dev, N, npart, K_host, Dr_dev, Ds_dev, Dt_dev, u_dev, u_off_dev,
-                                   metrics_dev, mtype, m_off_dev, dudx_dev, dudy_dev, dudz_dev,
-                                   otype, o_off_dev, 8);
+  DGK_REQUIRE(metrics_dev, "dgk_phys_deriv_nodal: NULL metrics");
+  const int np = dgk::np_of(N);
+  const double* D[3] = {Dr_dev, Ds_dev, Dt_dev};
+  const void* moff[9];
+  for (int i = 0; i < 9; ++i) moff[i] = m_off_dev;
+  void* outs[3] = {dudx_dev, dudy_dev, dudz_dev};
+  const void* ooff[3] = {o_off_dev, o_off_dev, o_off_dev};
+  return dgk::phys_deriv_nodal_full(dev, N, npart, K_host, nullptr, 8, D, np, 1, u_dev, u_off_dev,
+                                    metrics_dev, moff, mtype, outs, ooff, otype, nullptr, nullptr, 8);
 }
 
 extern "C" int dgk_matmul(dgk_device dev, int rows, int cols, const double* A_colmajor_dev,
@@ -372,6 +398,7 @@ extern "C" int dgk_matmul(dgk_device dev, int rows, int cols, const double* A_co
                           const int64_t* in_off_dev, double* out_dev, const int64_t* out_off_dev,
                           int add) {
   dgk::clear_error();
-  return dgk::matmul_any(dev, rows, cols, A_colmajor_dev, npart, K_host, in_dev, in_off_dev, out_dev,
-                         out_off_dev, add, 8);
+  return dgk::matmul_typed(dev, npart, K_host, nullptr, 8, rows, cols, A_colmajor_dev, 1, rows,
+                           DGK_F64, in_dev, in_off_dev, DGK_F64, out_dev, out_off_dev, DGK_F64,
+                           DGK_F64, add, 8);
 }

@@ -25,6 +25,10 @@ struct dgk_device_s {
   int arch_major = 0, arch_minor = 0;
   char name[256] = {0};
   bool capturing = false;
+  // Device copy of a host K array for launches over more than 256
+  // partitions (grown on demand, freed with the device).
+  void* kcache = nullptr;
+  size_t kcache_bytes = 0;
 };
 
 #define DGK_HIP(call)                                                                   \

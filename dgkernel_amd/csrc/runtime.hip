@@ -109,6 +109,7 @@ int dgk_device_destroy(dgk_device dev) {
   (void)hipSetDevice(dev->id);
   (void)hipStreamSynchronize(dev->stream);
   (void)hipStreamDestroy(dev->stream);
+  if (dev->kcache) (void)hipFree(dev->kcache);
   delete dev;
   return DGK_OK;
 }

@@ -6,8 +6,9 @@
  *
  *  1. The device/memory/kernel/graph layer that replaces the gocca API the
  *     reference's runner calls through cgo (citations are the reference call
- *     sites each function replaces). A registry of hand-written kernels keyed
- *     by kernel name + the runner-generated signature replaces OCCA's JIT.
+ *     sites each function replaces). Kernels are built from the same OKL
+ *     text the runner hands to gocca, read by a front end that maps its
+ *     statements onto hand-written gfx950 kernels (no JIT compiler).
  *
  *  2. Hot-path entry points, callable directly (and from inside a graph
  *     capture): physical derivative, FaceConnector exchange, fused
@@ -89,21 +90,22 @@ int dgk_copy_d2d(dgk_device dev, void* dst_dev, const void* src_dev, size_t byte
 int dgk_fill_bytes(dgk_device dev, void* dst_dev, int value, size_t bytes);
 
 /* ---------------------------------------------------------------------- */
-/* Kernel registry: replaces OCCADevice.BuildKernelFromString              */
-/* (runner/runner.go:106-109) and OCCAKernel.RunWithArgs                   */
-/* (runner/kernel_execution.go:57). signature is the string the runner     */
-/* generates (runner/kernel_config.go:186-246): "const int_t* K, const     */
-/* double* Dr_NudgTet2, ..., const double* U_global, const int_t*          */
-/* U_offsets, ...". Arguments are matched to the native kernel by name.    */
-/* Known kernels: see dgk_kernel_list().                                    */
+/* Source-built kernels: replace OCCADevice.BuildKernelFromString          */
+/* (runner/runner.go:88-123, called with the runner's preamble + "\n" +    */
+/* the caller's kernel source, the kernel name, and OCCA props) and        */
+/* OCCAKernel.RunWithArgs (runner/kernel_execution.go:57).                 */
+/* There is no OKL compiler: the text is read by a front end that knows    */
+/* the runner's preamble (runner/builder/builder.go:172-410: int_t, NPART, */
+/* KpartMax, static matrices, X_PART and MATMUL_X / MATMUL_ADD_X macros)   */
+/* and these statements in the @outer partition loop: X_PART aliases,      */
+/* MATMUL macros, element-node and flat-node pointwise loops, and the      */
+/* element matvec loop (dgk_kernel_list prints the forms). Each statement  */
+/* runs as one launch over all partitions on hand-written kernels; the     */
+/* reference's "differentiate" body (three MATMULs + chain rule,           */
+/* tetnudg/tet_nudg_mesh_physical_derivative_test.go:850-892, :240-278)    */
+/* runs as one launch of the physical-derivative kernel. Other kernels     */
+/* return DGK_ERR_UNSUPPORTED with the offending source line.              */
 /* ---------------------------------------------------------------------- */
-typedef struct {
-  int npart;      /* NPART */
-  int kpart_max;  /* KpartMax */
-  int int_size;   /* sizeof(int_t): 8 (INT64 default) or 4 (INT32) */
-  int np;         /* nodes per element of the element arrays (NP) */
-} dgk_kernel_consts;
-
 enum { DGK_ARG_MEM = 0, DGK_ARG_F64 = 1, DGK_ARG_F32 = 2, DGK_ARG_I64 = 3, DGK_ARG_I32 = 4 };
 
 typedef struct {
@@ -118,18 +120,22 @@ typedef struct {
   } v;
 } dgk_arg;
 
-/* Newline-separated registry names into buf (size bytes). */
+/* The statement forms the front end knows, newline-separated. */
 int dgk_kernel_list(char* buf, size_t size);
-int dgk_kernel_get(dgk_device dev, const char* name, const char* signature,
-                   const dgk_kernel_consts* consts, dgk_kernel* out);
-/* A static matrix the runner embeds in the OKL preamble instead of passing it
- * (runner/builder/builder.go:215-277, MATMUL_<name> with a static matrix):
- * the shim hands its values over once, rows x cols, row-major. */
-int dgk_kernel_set_matrix(dgk_kernel k, const char* name, int rows, int cols,
-                          const double* values_rowmajor);
-/* Enqueue one launch. host_K: the partition sizes (the runner's K array,
- * npart values of int_size bytes) -- used to size the launch without a
- * device read. args: in signature order. */
+/* Read source (the full text gocca would compile) and prepare kernel name.
+ * Static matrices are copied to the device here (allocates: call outside
+ * graph capture). */
+int dgk_kernel_build_from_source(dgk_device dev, const char* source, const char* name,
+                                 dgk_kernel* out);
+/* Parse only (no device): the statement program the library would run,
+ * one line per launch, into buf. */
+int dgk_kernel_describe(const char* source, const char* name, char* buf, size_t size);
+/* Enqueue the kernel's launches. args: one per signature parameter, in
+ * signature order (runner/kernel_execution.go:150-227): device memory for
+ * pointers (non-NULL), DGK_ARG_F64/F32/I64/I32 matching each scalar's type.
+ * host_K: the partition sizes (NPART int_t values) if the caller has them;
+ * NULL reads them from the K argument (a synchronous copy, as the runner's
+ * launch is synchronous anyway). */
 int dgk_kernel_launch(dgk_kernel k, const void* host_K, int nargs, const dgk_arg* args);
 int dgk_kernel_free(dgk_kernel k);
 
