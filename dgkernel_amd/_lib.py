@@ -91,6 +91,8 @@ _SIGS = {
     "dgk_timer_destroy": (_i32, [_vp]),
     "dgk_phys_deriv_nodal": (_i32, [_vp, _i32, _i32, _pi64, _vp, _vp, _vp, _vp, _vp,
                                     C.POINTER(_vp), _i32, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "dgk_phys_deriv_nodal_noalloc": (_i32, [_vp, _i32, _i32, _pi64, _vp, _vp, _vp, _vp, _vp,
+                                            C.POINTER(_vp), _i32, _vp, _vp, _vp, _vp, _i32, _vp]),
     "dgk_deriv_plan_create": (_i32, [_vp, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pf64, _i32,
                                      C.POINTER(_vp)]),
     "dgk_deriv_plan_destroy": (_i32, [_vp]),

@@ -23,7 +23,7 @@
 namespace dgk {
 
 int build_part_tiles(dgk_device dev, int npart, const int64_t* K, int per_tile, const void* kdev,
-                     int ksize, PartTiles& pt, dim3& grid) {
+                     int ksize, PartTiles& pt, dim3& grid, bool force2d) {
   DGK_REQUIRE(npart >= 1, "partition count %d < 1", npart);
   DGK_REQUIRE(per_tile >= 1, "tile size %d < 1", per_tile);
   pt.npart = npart;
@@ -36,7 +36,8 @@ int build_part_tiles(dgk_device dev, int npart, const int64_t* K, int per_tile, 
     kmax = std::max<int64_t>(kmax, K[p]);
     tiles += (K[p] + per_tile - 1) / per_tile;
   }
-  if (npart <= kMaxPartsByValue) {
+  DGK_REQUIRE(!force2d || kdev, "two-dimensional tiling needs a device K");
+  if (npart <= kMaxPartsByValue && !force2d) {
     DGK_REQUIRE(tiles < (1LL << 31), "too many tiles (%lld)", (long long)tiles);
     int t = 0;
     for (int p = 0; p < npart; ++p) {
@@ -200,13 +201,13 @@ template <> struct NodalShape<4> { static constexpr int G = 7, EPG = 4; };
 
 template <int N, typename MT, typename OT, typename OFF>
 int launch_nodal(dgk_device dev, int npart, const int64_t* K, const void* kdev, int ksize,
-                 const NodalArgs& a) {
+                 bool force2d, const NodalArgs& a) {
   constexpr int NP = (N + 1) * (N + 2) * (N + 3) / 6;
   PartTiles pt;
   dim3 grid;
   if constexpr (N <= 4) {
     constexpr int G = NodalShape<N>::G, EPG = NodalShape<N>::EPG;
-    int rc = dgk::build_part_tiles(dev, npart, K, G * EPG, kdev, ksize, pt, grid);
+    int rc = dgk::build_part_tiles(dev, npart, K, G * EPG, kdev, ksize, pt, grid, force2d);
     if (rc) return rc;
     if (grid.x == 0) return DGK_OK;
     int threads = ((NP * G + 63) / 64) * 64;
@@ -214,7 +215,7 @@ int launch_nodal(dgk_device dev, int npart, const int64_t* K, const void* kdev, 
                        dev->stream, pt, a);
   } else {
     constexpr int G = 16;
-    int rc = dgk::build_part_tiles(dev, npart, K, G, kdev, ksize, pt, grid);
+    int rc = dgk::build_part_tiles(dev, npart, K, G, kdev, ksize, pt, grid, force2d);
     if (rc) return rc;
     if (grid.x == 0) return DGK_OK;
     hipLaunchKernelGGL((deriv_nodal_big_kernel<NP, G, MT, OT, OFF>), grid, dim3(256), 0, dev->stream,
@@ -226,11 +227,11 @@ int launch_nodal(dgk_device dev, int npart, const int64_t* K, const void* kdev, 
 
 template <typename MT, typename OT, typename OFF>
 int dispatch_nodal_N(int N, dgk_device dev, int npart, const int64_t* K, const void* kdev,
-                     int ksize, const NodalArgs& a) {
+                     int ksize, bool force2d, const NodalArgs& a) {
   switch (N) {
 #define DGK_CASE(n) \
   case n:           \
-    return launch_nodal<n, MT, OT, OFF>(dev, npart, K, kdev, ksize, a);
+    return launch_nodal<n, MT, OT, OFF>(dev, npart, K, kdev, ksize, force2d, a);
     DGK_CASE(1) DGK_CASE(2) DGK_CASE(3) DGK_CASE(4) DGK_CASE(5) DGK_CASE(6)
 #undef DGK_CASE
     default:
@@ -287,7 +288,8 @@ int phys_deriv_nodal_full(dgk_device dev, int N, int npart, const int64_t* K, co
                           int ksize, const double* const* D, int drs, int dcs, const double* u,
                           const void* u_off, const void* const* metrics, const void* const* m_off,
                           int mtype, void* const* outs, const void* const* o_off, int otype,
-                          double* const* temps, const void* const* t_off, int off_size) {
+                          double* const* temps, const void* const* t_off, int off_size,
+                          bool force2d) {
   DGK_REQUIRE(dev && K && D && u && u_off && metrics && m_off && outs && o_off,
               "physical derivative: NULL argument");
   NodalArgs a{};
@@ -314,7 +316,8 @@ int phys_deriv_nodal_full(dgk_device dev, int N, int npart, const int64_t* K, co
   DGK_REQUIRE(mtype == DGK_F64 || mtype == DGK_F32, "physical derivative: bad metric type");
   DGK_REQUIRE(otype == DGK_F64 || otype == DGK_F32, "physical derivative: bad output type");
   DGK_REQUIRE(off_size == 8 || off_size == 4, "physical derivative: offset size %d", off_size);
-#define DGK_GO(MT, OT, OFF) return dispatch_nodal_N<MT, OT, OFF>(N, dev, npart, K, kdev, ksize, a)
+#define DGK_GO(MT, OT, OFF) \
+  return dispatch_nodal_N<MT, OT, OFF>(N, dev, npart, K, kdev, ksize, force2d, a)
   if (off_size == 8) {
     if (mtype == DGK_F64 && otype == DGK_F64) DGK_GO(double, double, int64_t);
     if (mtype == DGK_F32 && otype == DGK_F64) DGK_GO(float, double, int64_t);
@@ -390,7 +393,90 @@ extern "C" int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int6
   void* outs[3] = {dudx_dev, dudy_dev, dudz_dev};
   const void* ooff[3] = {o_off_dev, o_off_dev, o_off_dev};
   return dgk::phys_deriv_nodal_full(dev, N, npart, K_host, nullptr, 8, D, np, 1, u_dev, u_off_dev,
-                                    metrics_dev, moff, mtype, outs, ooff, otype, nullptr, nullptr, 8);
+                                    metrics_dev, moff, mtype, outs, ooff, otype, nullptr, nullptr, 8,
+                                    false);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free physical derivative.
+//
+// dgk_phys_deriv_nodal keeps the partition sizes of a launch over more than
+// 256 partitions in a per-device buffer it allocates on first need. The
+// variant below keeps them in `scratch`, a file-scope table of
+// kNoAllocMaxParts int64 values (~512 KiB) that every GPU the library runs on
+// holds for the life of the process. Host code never passes the array's name
+// to a kernel: it uses the address hipGetSymbolAddress returns for each
+// device (dgk::noalloc_init at device creation, kept in dev->noalloc_k). The
+// host K is staged into the table by small kernels that carry 256 values each
+// in their arguments, so the call makes no allocation, no copy from host
+// memory and no synchronisation (graph-capturable). Every call on one GPU
+// shares the table and overwrites its first npart entries; calls that overlap
+// on one GPU (two streams) race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocMaxParts = 65535;  // grid.y limit of the 2-D launch
+constexpr int kStageChunk = 256;
+
+static __device__ __align__(256) int64_t scratch[kNoAllocMaxParts];
+
+struct KChunk {
+  int64_t k[kStageChunk];
+};
+
+__global__ void stage_k_kernel(int64_t* __restrict__ dst, int first, int n, KChunk c) {
+  const int i = threadIdx.x;
+  if (i < n) dst[first + i] = c.k[i];
+}
+
+}  // namespace
+
+int dgk::noalloc_init(dgk_device dev) {
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    dgk::set_error("hipGetSymbolAddress(partition table) failed on device %d: %s", dev->id,
+                   hipGetErrorString(e));
+    return DGK_ERR_HIP;
+  }
+  dev->noalloc_k = (int64_t*)p;
+  return DGK_OK;
+}
+
+extern "C" int dgk_phys_deriv_nodal_noalloc(dgk_device dev, int N, int npart,
+                                            const int64_t* K_host, const double* Dr_dev,
+                                            const double* Ds_dev, const double* Dt_dev,
+                                            const double* u_dev, const int64_t* u_off_dev,
+                                            const void* const* metrics_dev, int mtype,
+                                            const int64_t* m_off_dev, void* dudx_dev,
+                                            void* dudy_dev, void* dudz_dev, int otype,
+                                            const int64_t* o_off_dev) {
+  dgk::clear_error();
+  DGK_REQUIRE(dev && K_host && metrics_dev, "dgk_phys_deriv_nodal_noalloc: NULL argument");
+  DGK_REQUIRE(dev->noalloc_k, "dgk_phys_deriv_nodal_noalloc: partition table not resolved");
+  DGK_REQUIRE(npart >= 1 && npart <= kNoAllocMaxParts,
+              "dgk_phys_deriv_nodal_noalloc: %d partitions outside 1..%d (the table's size)", npart,
+              kNoAllocMaxParts);
+  for (int p = 0; p < npart; ++p)
+    DGK_REQUIRE(K_host[p] >= 0 && K_host[p] < (1LL << 31),
+                "dgk_phys_deriv_nodal_noalloc: K[%d] = %lld out of range", p, (long long)K_host[p]);
+  for (int first = 0; first < npart; first += kStageChunk) {
+    KChunk c;
+    const int n = std::min(kStageChunk, npart - first);
+    for (int i = 0; i < n; ++i) c.k[i] = K_host[first + i];
+    hipLaunchKernelGGL(stage_k_kernel, dim3(1), dim3(kStageChunk), 0, dev->stream, dev->noalloc_k,
+                       first, n, c);
+    DGK_LAUNCH_CHECK();
+  }
+  const int np = dgk::np_of(N);
+  const double* D[3] = {Dr_dev, Ds_dev, Dt_dev};
+  const void* moff[9];
+  for (int i = 0; i < 9; ++i) moff[i] = m_off_dev;
+  void* outs[3] = {dudx_dev, dudy_dev, dudz_dev};
+  const void* ooff[3] = {o_off_dev, o_off_dev, o_off_dev};
+  return dgk::phys_deriv_nodal_full(dev, N, npart, K_host, dev->noalloc_k, 8, D, np, 1, u_dev,
+                                    u_off_dev, metrics_dev, moff, mtype, outs, ooff, otype, nullptr,
+                                    nullptr, 8, true);
 }
 
 extern "C" int dgk_matmul(dgk_device dev, int rows, int cols, const double* A_colmajor_dev,

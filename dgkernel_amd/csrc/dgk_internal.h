@@ -14,6 +14,9 @@ namespace dgk {
 
 void set_error(const char* fmt, ...);
 void clear_error();
+// Resolves this device's address of the allocation-free partition table;
+// called once per device handle at creation, with the device current.
+int noalloc_init(dgk_device dev);
 
 }  // namespace dgk
 
@@ -29,6 +32,9 @@ struct dgk_device_s {
   // partitions (grown on demand, freed with the device).
   void* kcache = nullptr;
   size_t kcache_bytes = 0;
+  // This device's address of the allocation-free partition table (set at
+  // device creation, see deriv.hip).
+  int64_t* noalloc_k = nullptr;
 };
 
 #define DGK_HIP(call)                                                                   \

@@ -60,7 +60,8 @@ int phys_deriv_nodal_full(dgk_device dev, int N, int npart, const int64_t* K, co
                           int ksize, const double* const* D, int drs, int dcs, const double* u,
                           const void* u_off, const void* const* metrics, const void* const* m_off,
                           int mtype, void* const* outs, const void* const* o_off, int otype,
-                          double* const* temps, const void* const* t_off, int off_size);
+                          double* const* temps, const void* const* t_off, int off_size,
+                          bool force2d);
 int matmul_typed(dgk_device dev, int npart, const int64_t* K, const void* kdev, int ksize, int rows,
                  int cols, const void* A, int rs, int cs, int mtype, const void* in,
                  const void* in_off, int itype, void* out, const void* out_off, int otype,
@@ -1831,7 +1832,8 @@ extern "C" int dgk_kernel_launch(dgk_kernel k, const void* host_K, int nargs, co
         rc = dgk::phys_deriv_nodal_full(k->dev, np_order(np), k->npart, K.data(), kdev, k->int_size, D, 1,
                                         np, (const double*)b.G(o.u), b.O(o.u), mets, moff,
                                         dgk_type(k->arrays[o.mets[0]].type), outs, ooff,
-                                        dgk_type(k->arrays[o.outs[0]].type), tmp, toff, k->int_size);
+                                        dgk_type(k->arrays[o.outs[0]].type), tmp, toff, k->int_size,
+                                        false);
         break;
       }
     }

@@ -54,7 +54,9 @@ __device__ __forceinline__ bool find_tile(const PartTiles& pt, int per_tile, int
 // K if the caller has one (the runner's K array); else, above
 // kMaxPartsByValue partitions, K is copied into a per-device buffer
 // (synchronous; that call is not graph-capturable). grid.x == 0: no work.
+// force2d: use the two-dimensional grid (K from kdev, which must be given)
+// whatever the partition count.
 int build_part_tiles(dgk_device dev, int npart, const int64_t* K, int per_tile, const void* kdev,
-                     int ksize, PartTiles& pt, dim3& grid);
+                     int ksize, PartTiles& pt, dim3& grid, bool force2d = false);
 
 }  // namespace dgk

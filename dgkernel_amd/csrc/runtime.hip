@@ -100,6 +100,12 @@ int dgk_device_create(const char* props, dgk_device* out) {
     dgk::set_error("hipStreamCreateWithFlags: %s", hipGetErrorString(e));
     return DGK_ERR_HIP;
   }
+  int rc = dgk::noalloc_init(d);
+  if (rc != DGK_OK) {
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return rc;
+  }
   *out = d;
   return DGK_OK;
 }

@@ -177,6 +177,23 @@ int dgk_phys_deriv_nodal(dgk_device dev, int N, int npart, const int64_t* K_host
                          const int64_t* m_off_dev, void* dudx_dev, void* dudy_dev, void* dudz_dev,
                          int otype, const int64_t* o_off_dev);
 
+/* Allocation-free form of dgk_phys_deriv_nodal, with the same arguments and
+ * results. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops). dgk_phys_deriv_nodal keeps the partition sizes of a launch over more
+ * than 256 partitions in a per-device buffer it allocates (and fills with a
+ * synchronous copy) on first need; this form stages them into a fixed table
+ * the library holds on every GPU it runs on (65535 partitions at most, ~512
+ * KiB) with small kernels, so it makes no allocation, no host copy and no
+ * synchronisation. All calls on one GPU share that table: calls that overlap
+ * on one GPU (different streams) race. npart above 65535 is rejected. */
+int dgk_phys_deriv_nodal_noalloc(dgk_device dev, int N, int npart, const int64_t* K_host,
+                                 const double* Dr_dev, const double* Ds_dev, const double* Dt_dev,
+                                 const double* u_dev, const int64_t* u_off_dev,
+                                 const void* const* metrics_dev, int mtype,
+                                 const int64_t* m_off_dev, void* dudx_dev, void* dudy_dev,
+                                 void* dudz_dev, int otype, const int64_t* o_off_dev);
+
 /* Affine elements (native layout): the operator is prepared once per order
  * and precision (MFMA fragments), then applied to K elements:
  *   u_dev: dense element-major [k*Np + n], FP64, 16-byte aligned;
