@@ -1,10 +1,13 @@
-"""The benchmark configurations at their full sizes on the GPU, checked
-through properties that hold at any size (the oracle is too slow there):
+"""The benchmark configurations at their full sizes on the GPU:
 
-* the stage conserves the integral of u (periodic domain, conservative LF flux
-  in strong form; the oracle pins 1'MJ L(u) = 0 to round-off on small meshes);
-* graph replay of whole steps equals eager launches bit for bit;
-* a contraction below FP64 stays within its bound of the FP64 run.
+* against the CPU oracle (the reference's partition schedule, all host
+  threads): one stage's L(u) and two SSP-RK3 steps of the 5.3M-tet P=3
+  problem; the 2^20-tet P=4 physical derivative through the runner's own
+  differentiate kernel and through the affine matrix-core plan;
+* through properties that hold at any size: the stage conserves the integral
+  of u (periodic domain, conservative LF flux in strong form); graph replay of
+  whole steps equals eager launches bit for bit; a contraction below FP64
+  stays within its bound of the FP64 run.
 """
 import numpy as np
 import pytest
@@ -12,7 +15,91 @@ import pytest
 import bench
 from dgkernel_amd import _lib, dg
 from dgkernel_amd import mesh as M
+from oracle import oracle as O
+from tests import okl_kernels as KS
 from tests.helpers import rel_l2
+from tests.runner_text import DeviceRun
+
+
+@pytest.fixture(scope="module")
+def c3():
+    """96^3 x 6 periodic tets, P=3, 64 slab partitions with +-40 % imbalance,
+    Morton order within partitions (the benchmark's workload)."""
+    return bench.build_c3(96, 3, 64, 2024, morton=True)
+
+
+def test_c3_full_size_vs_oracle(device, c3):
+    m, st, u0 = c3
+    o = O.Setup.from_mesh(3, m)
+    hmin = o.min_node_distance()
+    s = dg.BurgersSolver(device, st, cfl=0.3, hmin=hmin)
+    want = o.burgers_rhs_partitions(u0, st.Kp)
+    got = s.rhs(u0)
+    err = float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    e2 = rel_l2(got, want)
+    print(f"C3 L(u) vs oracle: max rel {err:.3e}, rel L2 {e2:.3e}")
+    # round-off: at h = 1/96 the volume and surface terms are ~1/h larger than
+    # their sum L(u), so the max-norm bound carries that factor
+    assert e2 <= 1e-12 and err <= 1e-11
+    del got, want
+    s.set_state(u0)
+    s.step(2)
+    want, dts = o.ssprk3_partitions(u0, 2, 0.3, hmin, st.Kp)
+    e2 = rel_l2(s.state(), want)
+    print(f"C3 two SSP-RK3 steps vs oracle: rel L2 {e2:.3e}, dt {s.dt():.17g} vs {dts[-1]:.17g}")
+    assert e2 <= 1e-12
+    assert abs(s.dt() - dts[-1]) <= 1e-12 * dts[-1]
+
+
+def test_c2_derivative_full_size_vs_oracle(device):
+    """First 2^20 tets of a 64x64x43-hex cube, vertices jittered +-10 % of h,
+    P=4, u ~ U[-1, 1]: the reference's differentiate kernel (three MATMULs +
+    chain rule, per-node metrics) built from its own text, and the affine FP64
+    plan, against calcPhysicalDerivative."""
+    N = 4
+    m = M.cube_mesh(64, 64, 43, jitter=0.1, seed=2024, kmax=2 ** 20)
+    assert m.K == 2 ** 20
+    o = O.Setup(N, m.VX, m.VY, m.VZ, m.EToV)
+    assert o.status == 0
+    e = O.element(N)
+    Np = e["Np"]
+    U = M.uniform(2024, 1, m.K * Np, -1.0, 1.0).reshape(m.K, Np)
+    mets = {nm: o.field(nm.lower()) for nm in KS.METRICS}
+    order9 = [mets[k] for k in ("Rx", "Ry", "Rz", "Sx", "Sy", "Sz", "Tx", "Ty", "Tz")]
+    want = O.phys_deriv(e["Dr"], e["Ds"], e["Dt"], U, order9)
+    case = KS.differentiate_full([e["Dr"], e["Ds"], e["Dt"]], U.reshape(-1),
+                                 {k: v.reshape(-1) for k, v in mets.items()}, [m.K], short="NudgTet4")
+    run = DeviceRun(case[0], device)
+    k = _lib.Kernel(device, case[1], case[2])
+    k.launch(run.args(case[3]))
+    device.sync()
+    for q, nm in enumerate(("DuDx", "DuDy", "DuDz")):
+        got = run.result(nm).reshape(m.K, Np)
+        err = float(np.max(np.abs(got - want[q])) / np.abs(want[q]).max())
+        print(f"C2 {nm} (runner kernel) vs oracle: max rel {err:.3e}")
+        assert err <= 1e-13
+    del run, k
+    geo = np.stack([mm[:, 0] for mm in order9])
+    plan = dg.DerivPlan(device, N, (e["Dr"], e["Ds"], e["Dt"]), _lib.DGK_PREC_F64,
+                        (e["r"], e["s"], e["t"]))
+    du = device.upload(np.ascontiguousarray(U))
+    dgeo = device.upload(np.ascontiguousarray(geo))
+    outs = [device.zeros(m.K * Np) for _ in range(3)]
+    plan.apply(m.K, du, dgeo.ptr, m.K, outs)
+    device.sync()
+    # The affine plan uses one metric value per element; the oracle's per-node
+    # metrics (Dr X etc.) agree with it to round-off ~ cond(V) eps, which the
+    # random u (|Dr u| up to ~90) amplifies. The reference value is the chain
+    # rule with the element's value.
+    del want
+    want_e = O.phys_deriv(e["Dr"], e["Ds"], e["Dt"], U,
+                          [np.repeat(g[:, None], Np, axis=1) for g in geo])
+    for q in range(3):
+        got = outs[q].to_host().reshape(m.K, Np)
+        err = float(np.max(np.abs(got - want_e[q])) / np.abs(want_e[q]).max())
+        print(f"C2 d{q} (affine FP64 plan) vs oracle: max rel {err:.3e}")
+        assert err <= 1e-13
+    plan.free()
 
 pytestmark = pytest.mark.gpu
 
@@ -28,10 +115,10 @@ def _integral(st, J, u):
     return float(np.sum(J * (u @ st.ref.Mass.sum(axis=1))))
 
 
-def test_c3_full_size_conservation_and_graph_replay(device):
+def test_c3_full_size_conservation_and_graph_replay(device, c3):
     """96^3 x 6 = 5,308,416 periodic tets, P=3, 64 slab partitions with +-40 %
     imbalance, Morton order within partitions: two SSP-RK3 steps."""
-    m, st, u0 = bench.build_c3(96, 3, 64, 2024, morton=True)
+    m, st, u0 = c3
     assert m.K == 5308416 and st.npart == 64
     J = _jacobians(m)
     assert abs(J.sum() * 4.0 / 3.0 - 1.0) <= 1e-12  # the tets fill the unit cube
