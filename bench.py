@@ -212,7 +212,8 @@ def bench_deriv(a, dev) -> dict:
     ref64 = None
     res = {}
     precs = [("affine_f64", _lib.DGK_PREC_F64), ("affine_f32", _lib.DGK_PREC_F32),
-             ("affine_bf16x3", _lib.DGK_PREC_BF16X3), ("affine_bf16x6", _lib.DGK_PREC_BF16X6)]
+             ("affine_bf16x3", _lib.DGK_PREC_BF16X3), ("affine_bf16x6", _lib.DGK_PREC_BF16X6),
+             ("affine_i8x6", _lib.DGK_PREC_I8X6)]
     rst = (st.ref.r, st.ref.s, st.ref.t)
     if a.precs:
         keep = set(a.precs.split(","))
@@ -221,7 +222,7 @@ def bench_deriv(a, dev) -> dict:
         plan = dg.DerivPlan(dev, N, D, prec, rst)
         outs = [dev.empty(K * Np, plan.out_dtype) for _ in range(3)]
         ms = _time_ms(dev, lambda: plan.apply(K, du, geo_ptr, K, outs), a.reps)
-        ob = 8 if prec == _lib.DGK_PREC_F64 else 4
+        ob = 8 if prec in (_lib.DGK_PREC_F64, _lib.DGK_PREC_I8X6) else 4
         nbytes = (Np * 8 + 3 * Np * ob + 72) * K
         got = [o.to_host().astype(np.float64) for o in outs]
         if ref64 is None:
@@ -300,7 +301,8 @@ def bench_deriv(a, dev) -> dict:
 # Stage precision settings: Dr/Ds/Dt (and LIFT) precision; "_lift64" keeps
 # the LIFT product in FP64.
 STAGE_PRECS = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3,
-               "f32_lift64": 1 | 16, "bf16x3_lift64": 2 | 16, "bf16x6_lift64": 3 | 16}
+               "f32_lift64": 1 | 16, "bf16x3_lift64": 2 | 16, "bf16x6_lift64": 3 | 16,
+               "i8x6": 4}
 
 
 def bench_c4_stage(a, dev) -> dict:

@@ -24,7 +24,7 @@ _pf32 = C.POINTER(C.c_float)
 
 DGK_OK = 0
 DGK_F64, DGK_F32 = 0, 1
-DGK_PREC_F64, DGK_PREC_F32, DGK_PREC_BF16X3, DGK_PREC_BF16X6 = 0, 1, 2, 3
+DGK_PREC_F64, DGK_PREC_F32, DGK_PREC_BF16X3, DGK_PREC_BF16X6, DGK_PREC_I8X6 = 0, 1, 2, 3, 4
 DGK_PREC_LIFT_F64 = 16  # flag: LIFT product in FP64 whatever the D precision
 ARG_MEM, ARG_F64, ARG_F32, ARG_I64, ARG_I32 = 0, 1, 2, 3, 4
 

@@ -1514,8 +1514,8 @@ extern "C" int dgk_burgers_set_precision(dgk_dgmesh m, int prec) {
   dgk::clear_error();
   DGK_REQUIRE(m, "dgk_burgers_set_precision: NULL mesh");
   const int vol = prec & ~DGK_PREC_LIFT_F64;
-  DGK_REQUIRE(vol >= 0 && vol <= 3, "dgk_burgers_set_precision: precision %d", prec);
-  const int lift = (prec & DGK_PREC_LIFT_F64) ? 0 : vol;
+  DGK_REQUIRE(vol >= 0 && vol <= DGK_PREC_I8X6, "dgk_burgers_set_precision: precision %d", prec);
+  const int lift = ((prec & DGK_PREC_LIFT_F64) || vol == DGK_PREC_I8X6) ? 0 : vol;
   if (vol != m->prec || lift != m->lift_prec) {
     dgk::split_release(m);
     m->prec = vol;

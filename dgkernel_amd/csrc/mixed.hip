@@ -36,7 +36,17 @@
 //            ~16 significant bits per operand.
 //   BF16X6:  three-part splits (of the FP32 value of w), hi*hi + hi*mid +
 //            mid*hi + mid*mid + hi*lo + lo*hi; ~24 bits, FP32-level accuracy.
-// All low-precision modes accumulate in FP32 and write FP32 outputs.
+//   I8X6:    exact-integer split (Ozaki, Ogita & Oishi, Numer. Algorithms
+//            2012; Ootomo, Ozaki & Yokota, IJHPCA 2024, on integer matrix
+//            units): each operator row and each element's w is scaled by a
+//            power of two to |x| <= 1 and rounded to a 46-bit integer, which
+//            is written as six signed base-256 digits (int8). The digit pairs
+//            of weight a + b >= 5 (21 of 36) run as v_mfma_i32_16x16x64_i8
+//            with exact int32 accumulation, one accumulator per weight; the
+//            weights are combined and scaled back in FP64. Error ~2^-46 of the
+//            row and element magnitudes: FP64-level results (~3e-12 rel.).
+// BF16 and FP32 modes accumulate in FP32 and write FP32 outputs; I8X6 writes
+// FP64.
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -51,10 +61,21 @@ typedef unsigned int uintx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int intx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kF64 = 0, kF32 = 1, kBF3 = 2, kBF6 = 3;
+constexpr int kF64 = 0, kF32 = 1, kBF3 = 2, kBF6 = 3, kI8 = 4;
+// Exact-integer split: digits per operand, and the lowest pair weight kept.
+constexpr int kDig = 6, kThr = 5;
+constexpr int kDiag = 2 * kDig - 1 - kThr;  // accumulators (pair weights kThr..2 kDig-2)
+constexpr int kBits = 8 * kDig - 2;          // integer bits of a scaled operand
+// Integer plan K steps: 16x16x64 steps, and a final 16x16x32 step when the
+// rest of the Np columns fits in 32 (same issue cost, half the registers).
+constexpr int i8_k64(int np) { return np / 64 + (np % 64 > 32 ? 1 : 0); }
+constexpr int i8_t32(int np) { return (np % 64 != 0 && np % 64 <= 32) ? 1 : 0; }
 
-constexpr int parts_of(int prec) { return prec == kBF3 ? 2 : (prec == kBF6 ? 3 : 1); }
+constexpr int parts_of(int prec) {
+  return prec == kBF3 ? 2 : (prec == kBF6 ? 3 : (prec == kI8 ? kDig : 1));
+}
 constexpr bool is_bf(int prec) { return prec == kBF3 || prec == kBF6; }
 constexpr int freg_of(int prec) { return prec == kF64 ? 2 : (prec == kF32 ? 1 : 4); }
 
@@ -63,8 +84,11 @@ struct ACfg {
   static constexpr int NP = (N + 1) * (N + 2) * (N + 3) / 6;
   static constexpr int ROWS = 3 * NP;
   static constexpr int MT = (ROWS + 15) / 16;
-  static constexpr int KSTEP = is_bf(PREC) ? 32 : 4;
-  static constexpr int KS = (NP + KSTEP - 1) / KSTEP;
+  static constexpr int KSTEP = is_bf(PREC) ? 32 : (PREC == kI8 ? 64 : 4);
+  static constexpr int K64 = PREC == kI8 ? i8_k64(NP) : 0;  // integer plan: x64 steps
+  static constexpr int T32 = PREC == kI8 ? i8_t32(NP) : 0;  // integer plan: x32 tail step
+  static constexpr int KS = PREC == kI8 ? K64 + T32 : (NP + KSTEP - 1) / KSTEP;
+  static constexpr int KPAD = 64 * K64 + 32 * T32;           // integer plan: padded K
   static constexpr int PARTS = parts_of(PREC);
   static constexpr int FRAG_REGS = MT * KS * PARTS * freg_of(PREC);
   // G matrix waves hold the operator; G memory waves stage, convert and
@@ -73,15 +97,22 @@ struct ACfg {
   // SIMD) so that two blocks share a CU and one block's matrix phase
   // overlaps the other's memory phase. FP64 needs ~170 fragment and
   // accumulator registers per matrix wave and runs one block per CU.
+  // The integer plan keeps six operator digits in registers (72 at P=6
+  // with two row tiles per wave), so eight matrix waves share them.
   static constexpr int G =
-      MT < 4 ? MT : ((PREC == kF64 && MT >= 8) || (FRAG_REGS / 4 > 100 && PREC != kF64) ? 8 : 4);
+      MT < 4 ? MT
+             : ((PREC == kF64 && MT >= 8) || (PREC == kI8 && MT >= 8) ||
+                        (FRAG_REGS / 4 > 100 && PREC != kF64 && PREC != kI8)
+                    ? 8
+                    : 4);
   static constexpr int WPE = PREC == kF64 && G < 8 ? 2 : 4;
   static constexpr int TPW = (MT + G - 1) / G;
-  static constexpr int NT = 128 * G;                     // threads per block
-  static constexpr int NTM = 64 * G;                     // memory threads
+  static constexpr int NTM = 64 * G;   // memory threads
+  static constexpr int NT = 64 * G + NTM;  // threads per block
   static constexpr int NPU = NP | 1;                     // LDS row stride of u, in doubles
   static constexpr int NPF = NP | 1;                     // LDS row stride of FP32 w, in floats
   static constexpr int KPB = KS * 32 + 8;                // LDS row stride of bf16 w parts
+  static constexpr int KPI = KPAD + 16;                  // LDS row stride of int8 w digits, bytes
   static constexpr int ROWSP = MT * 16 + 1;              // LDS row stride of results
   static constexpr int NV2 = 8 * NP;                     // double2 per 16-element tile
   static constexpr int NL = (NV2 + NTM - 1) / NTM;       // double2 loads per memory thread
@@ -97,8 +128,8 @@ HostCfg host_cfg(int N, int prec) {
   HostCfg c;
   c.NP = dgk::np_of(N);
   c.MT = (3 * c.NP + 15) / 16;
-  c.KSTEP = is_bf(prec) ? 32 : 4;
-  c.KS = (c.NP + c.KSTEP - 1) / c.KSTEP;
+  c.KSTEP = is_bf(prec) ? 32 : (prec == kI8 ? 64 : 4);
+  c.KS = prec == kI8 ? i8_k64(c.NP) + i8_t32(c.NP) : (c.NP + c.KSTEP - 1) / c.KSTEP;
   c.PARTS = parts_of(prec);
   return c;
 }
@@ -121,11 +152,16 @@ template <>
 struct Frag<kBF6> {
   bf16x8 v[3];
 };
+template <>
+struct Frag<kI8> {
+  intx4 v[kDig];  // 16x16x64 step: digit a has weight 256^a
+  long t[kDig];   // 16x16x32 tail step
+};
 
 template <int PREC>
 using AccT = typename std::conditional<PREC == kF64, doublex4, floatx4>::type;
 template <int PREC>
-using ResT = typename std::conditional<PREC == kF64, double, float>::type;
+using ResT = typename std::conditional<PREC == kF64 || PREC == kI8, double, float>::type;
 
 struct AffineArgs {
   int64_t K;                // dense mode: elements; tile mode: unused
@@ -133,6 +169,8 @@ struct AffineArgs {
   int64_t ntiles;
   const void* frag;    // operator fragments, [part][MT][KS][64]
   const double* lam;   // [3][NP] barycentric weights l1, l2, l3 of each node (mixed modes)
+  const int* rowexp;   // I8X6: power-of-two scale of each operator row [MT*16]
+  const double* rowcorr;  // I8X6: [MT*16][4] D applied to 1, l1, l2, l3 minus its exact value
   int vtx[4];          // indices of the vertex nodes (mixed modes)
   const double* u;
   const double* geo;
@@ -152,7 +190,7 @@ __attribute__((amdgpu_waves_per_eu(ACfg<N, PREC>::WPE)))
 deriv_affine_kernel(AffineArgs a) {
   using C = ACfg<N, PREC>;
   constexpr int NP = C::NP, MT = C::MT, KS = C::KS, TPW = C::TPW, G = C::G, NTM = C::NTM;
-  constexpr int NPU = C::NPU, NPF = C::NPF, KPB = C::KPB, ROWSP = C::ROWSP;
+  constexpr int NPU = C::NPU, NPF = C::NPF, KPB = C::KPB, KPI = C::KPI, ROWSP = C::ROWSP;
   constexpr int NV2 = C::NV2, NL = C::NL, NIT = C::NIT, PARTS = C::PARTS;
   constexpr bool MIXED = PREC != kF64;
   using R = ResT<PREC>;
@@ -172,6 +210,22 @@ deriv_affine_kernel(AffineArgs a) {
   __shared__ __attribute__((aligned(16))) __bf16 sWb[is_bf(PREC) ? 2 : 1][PARTS]
                                                     [is_bf(PREC) ? 16 * KPB : 1];
   __shared__ R sR[2][16 * ROWSP];
+  // I8X6: the digits of w, [parity][digit][element][KPI bytes], and each
+  // element's power-of-two scale
+  __shared__ __attribute__((aligned(16))) int8_t sWi[PREC == kI8 ? 2 : 1][PREC == kI8 ? kDig : 1]
+                                                     [PREC == kI8 ? 16 * KPI : 1];
+  __shared__ int sF[2][16];
+  // I8X6: each element's vertex values (a0, a1 - a0, a2 - a0, a3 - a0), and
+  // the per-row correction that applies the stored operator to the linear
+  // part (the exact slopes are added in the epilogue)
+  __shared__ __attribute__((aligned(16))) double sA[PREC == kI8 ? 2 : 1][PREC == kI8 ? 64 : 1];
+  __shared__ __attribute__((aligned(16))) double sDel[PREC == kI8 ? MT * 64 : 1];
+  __shared__ int sRx[PREC == kI8 ? MT * 16 : 1];  // I8X6 row scales
+  // I8X6: the operator's least significant digit (weight 1), which enters a
+  // single pair (with w's top digit), sits in LDS; the matrix waves keep the
+  // other five digits in registers.
+  __shared__ intx4 sOpx[PREC == kI8 ? MT * (C::K64 > 0 ? C::K64 : 1) * 64 : 1];
+  __shared__ long sOpt[PREC == kI8 && C::T32 ? MT * 64 : 1];
   // Per element: 9 metrics rx..tz, then (below FP64) half the linear part's
   // r, s, t slopes, padded to 12 doubles so the epilogue reads it with six
   // 16-byte loads.
@@ -187,7 +241,7 @@ deriv_affine_kernel(AffineArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const bool is_mma = w < G;
-  const int mt = tid - NTM;  // memory-thread index (valid when !is_mma)
+  const int mt = tid - 64 * G;  // memory-thread index (valid when !is_mma)
   const int c = lane & 15, g = lane >> 4;
 
   // All setup loads complete here, so that the wait counters inside the
@@ -309,6 +363,59 @@ deriv_affine_kernel(AffineArgs a) {
           const int i = t + TPE * j;
           if (NP % TPE == 0 || i < NP) sWf[bu][e * NPF + i] = (float)rem(i);
         }
+      } else if constexpr (PREC == kI8) {
+        // Thread t of the element's TPE threads takes groups of four values
+        // t, t + TPE, ...; the element's scale 2^E (|w| < 2^E) comes from a
+        // max over those threads (consecutive lanes).
+        constexpr int NG = C::KPAD / 4, GPT = (NG + TPE - 1) / TPE;
+        static_assert(64 % TPE == 0, "an element's threads sit in one wave");
+        double x[GPT][4];
+        double mx = 0.0;
+#pragma unroll
+        for (int j = 0; j < GPT; ++j)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int i = 4 * (t + TPE * j) + b;
+            x[j][b] = rem(i);
+            mx = fmax(mx, fabs(x[j][b]));
+          }
+#pragma unroll
+        for (int o = TPE / 2; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+        int E = 0;
+        if (mx > 0.0) (void)frexp(mx, &E);
+        if (t == 0) {
+          sF[bu][e] = E;
+          sA[bu][4 * e] = a0;
+          sA[bu][4 * e + 1] = a1;
+          sA[bu][4 * e + 2] = a2;
+          sA[bu][4 * e + 3] = a3;
+        }
+        // y = round(w 2^(kBits - E)), |y| <= 2^46. The bytes of y + B, B =
+        // 0x808080808080, each minus 128 (XOR 0x80), are the balanced
+        // base-256 digits of y; split at 2^24 into exact uint32 halves, and
+        // transpose the bytes of four values into one word per digit.
+        const double B = 141289400074368.0;
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+          const int gi = t + TPE * j;
+          if (NG % TPE != 0 && gi >= NG) continue;
+          uint32_t L[4], H[4];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const double z = rint(ldexp(x[j][b], kBits - E)) + B;
+            const double zh = floor(z * 0x1p-24);
+            L[b] = (uint32_t)(z - zh * 0x1p24) ^ 0x808080u;
+            H[b] = (uint32_t)zh ^ 0x808080u;
+          }
+#pragma unroll
+          for (int d = 0; d < kDig; ++d) {
+            const uint32_t* S = d < 3 ? L : H;
+            const uint32_t k = (uint32_t)(d % 3);
+            const uint32_t lo = __builtin_amdgcn_perm(S[1], S[0], 0x0c0c0000u | ((4u + k) << 8) | k);
+            const uint32_t hi = __builtin_amdgcn_perm(S[3], S[2], ((4u + k) << 24) | (k << 16) | 0x0c0cu);
+            *(uint32_t*)(sWi[bu][d] + e * KPI + 4 * gi) = lo | hi;
+          }
+        }
       } else {
         // bf16 rows are zero-padded up to KS*32 values
         constexpr int NPAIR = KS * 16, NJ = (NPAIR + TPE - 1) / TPE;
@@ -369,6 +476,72 @@ deriv_affine_kernel(AffineArgs a) {
             acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[j][s].v, b, acc[j], 0, 0, 0);
         }
       }
+    } else if constexpr (PREC == kI8) {
+      // Per row tile: kDiag int32 accumulators, one per pair weight; B digits
+      // are read from LDS one at a time, so only one is live.
+      constexpr int K64 = C::K64;
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) {
+        intx4 ac[kDiag];
+#pragma unroll
+        for (int d = 0; d < kDiag; ++d) ac[d] = intx4{0, 0, 0, 0};
+        // One B digit ahead is in flight; the scheduling barriers keep the
+        // compiler from hoisting more (registers: the operator digits fill
+        // most of the budget).
+#pragma unroll
+        for (int s = 0; s < K64; ++s) {
+          const int8_t* wb = sWi[bu][0] + c * KPI + 64 * s + 16 * g;
+          intx4 bn = *(const intx4*)wb;
+#pragma unroll
+          for (int q = 0; q < kDig; ++q) {
+            const intx4 b = bn;
+            if (q + 1 < kDig) bn = *(const intx4*)(wb + (q + 1) * 16 * KPI);
+#pragma unroll
+            for (int p = kDig - 1; p >= 0; --p)
+              if (p + q >= kThr) {
+                const intx4 af = p > 0 ? fa[j][s].v[p] : sOpx[((w * TPW + j) * K64 + s) * 64 + lane];
+                ac[p + q - kThr] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(af, b, ac[p + q - kThr], 0, 0, 0);
+              }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        if constexpr (C::T32) {
+          const int8_t* wb = sWi[bu][0] + c * KPI + 64 * K64 + 8 * g;
+          long bn = *(const long*)wb;
+#pragma unroll
+          for (int q = 0; q < kDig; ++q) {
+            const long b = bn;
+            if (q + 1 < kDig) bn = *(const long*)(wb + (q + 1) * 16 * KPI);
+#pragma unroll
+            for (int p = kDig - 1; p >= 0; --p)
+              if (p + q >= kThr) {
+                const long af = p > 0 ? fa[j][K64].t[p] : sOpt[(w * TPW + j) * 64 + lane];
+                ac[p + q - kThr] =
+                    __builtin_amdgcn_mfma_i32_16x16x32_i8(af, b, ac[p + q - kThr], 0, 0, 0);
+              }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        const int tt = w * TPW + j;
+        if (tt < MT) {
+          const int E = sF[bu][c] + 8 * kThr - 2 * kBits;
+          const doublex2 av01 = *(const doublex2*)(sA[bu] + 4 * c);
+          const doublex2 av23 = *(const doublex2*)(sA[bu] + 4 * c + 2);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = 16 * tt + 4 * g + r;
+            double y = (double)ac[kDiag - 1][r];
+#pragma unroll
+            for (int d = kDiag - 2; d >= 0; --d) y = fma(y, 256.0, (double)ac[d][r]);
+            const doublex2 d01 = *(const doublex2*)(sDel + 4 * row);
+            const doublex2 d23 = *(const doublex2*)(sDel + 4 * row + 2);
+            const double corr = av01.x * d01.x + av01.y * d01.y + av23.x * d23.x + av23.y * d23.y;
+            sR[bu][c * ROWSP + row] = ldexp(y, sRx[row] + E) + corr;
+          }
+        }
+      }
+      return;
     } else {
       bf16x8 bb[KS][PARTS];
 #pragma unroll
@@ -460,7 +633,7 @@ deriv_affine_kernel(AffineArgs a) {
           const double dx = m01.x * ur + m23.y * us + m67.x * ut;
           const double dy = m01.y * ur + m45.x * us + m67.y * ut;
           const double dz = m23.x * ur + m45.y * us + m8l.x * ut;
-          if constexpr (PREC == kF64) {
+          if constexpr (PREC == kF64 || PREC == kI8) {
             __builtin_nontemporal_store(dx, (double*)a.ox + tr.vbase + v);
             __builtin_nontemporal_store(dy, (double*)a.oy + tr.vbase + v);
             __builtin_nontemporal_store(dz, (double*)a.oz + tr.vbase + v);
@@ -493,6 +666,13 @@ deriv_affine_kernel(AffineArgs a) {
           fa[j][s].v = tt < MT ? ((const double*)a.frag)[q] : 0.0;
         } else if constexpr (PREC == kF32) {
           fa[j][s].v = tt < MT ? ((const float*)a.frag)[q] : 0.f;
+        } else if constexpr (PREC == kI8) {
+#pragma unroll
+          for (int p = 1; p < kDig; ++p) {
+            const intx4* F = (const intx4*)a.frag + (size_t)p * C::NFRAG;
+            if (s < C::K64) fa[j][s].v[p] = tt < MT ? F[q] : intx4{0, 0, 0, 0};
+            else fa[j][s].t[p] = tt < MT ? ((const long*)(F + q))[0] : 0L;
+          }
         } else {
 #pragma unroll
           for (int p = 0; p < PARTS; ++p) {
@@ -520,6 +700,17 @@ deriv_affine_kernel(AffineArgs a) {
   } else {
     if constexpr (MIXED) {
       for (int v = mt; v < 3 * NP; v += NTM) sLam[v] = a.lam[v];
+    }
+    if constexpr (PREC == kI8) {
+      for (int v = mt; v < MT * 64; v += NTM) sDel[v] = a.rowcorr[v];
+      for (int v = mt; v < MT * 16; v += NTM) sRx[v] = a.rowexp[v];
+      // digit 0 slots [tile][kstep][lane] (16 bytes; the tail step uses 8)
+      const intx4* F0 = (const intx4*)a.frag;
+      for (int v = mt; v < MT * KS * 64; v += NTM) {
+        const int tt = v / (KS * 64), ks = (v / 64) % KS, l = v % 64;
+        if (ks < C::K64) sOpx[(tt * C::K64 + ks) * 64 + l] = F0[v];
+        else if constexpr (C::T32) sOpt[tt * 64 + l] = ((const long*)(F0 + v))[0];
+      }
     }
     doublex2 pre[PD][NL];
     double pm[PD];  // metrics: memory thread v < 144 holds row v / 16 of element v % 16
@@ -562,6 +753,8 @@ struct dgk_deriv_plan_s {
   int N = 0, prec = 0;
   void* frag = nullptr;
   double* lam = nullptr;
+  int* rowexp = nullptr;  // I8X6 row scales
+  double* rowcorr = nullptr;  // I8X6 [MT*16][4] operator on the linear part minus exact
   int vtx[4] = {0, 0, 0, 0};
   int blocks = 0;  // persistent grid size
 };
@@ -621,6 +814,7 @@ int dispatch_n(dgk_deriv_plan pl, const AffineArgs* a, int what) {
     case kF64: return dispatch_p<N, kF64>(pl, a, what);
     case kF32: return dispatch_p<N, kF32>(pl, a, what);
     case kBF3: return dispatch_p<N, kBF3>(pl, a, what);
+    case kI8: return dispatch_p<N, kI8>(pl, a, what);
     default: return dispatch_p<N, kBF6>(pl, a, what);
   }
 }
@@ -646,7 +840,7 @@ extern "C" int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr, co
   dgk::clear_error();
   DGK_REQUIRE(dev && Dr && Ds && Dt && out, "dgk_deriv_plan_create: NULL argument");
   DGK_REQUIRE(N >= 1 && N <= 6, "dgk_deriv_plan_create: N=%d outside 1..6", N);
-  DGK_REQUIRE(prec >= kF64 && prec <= kBF6, "dgk_deriv_plan_create: precision %d", prec);
+  DGK_REQUIRE(prec >= kF64 && prec <= kI8, "dgk_deriv_plan_create: precision %d", prec);
   DGK_REQUIRE(prec == kF64 || (r && s && t),
               "dgk_deriv_plan_create: node coordinates r, s, t are required below FP64");
   *out = nullptr;
@@ -660,7 +854,40 @@ extern "C" int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr, co
   };
   const size_t nfrag = (size_t)hc.MT * hc.KS * 64;
   std::vector<unsigned char> buf;
-  if (prec == kF64 || prec == kF32) {
+  std::vector<int> rowexp;
+  if (prec == kI8) {
+    // Row r scaled by 2^-E_r (|A_rk| < 2^E_r), rounded to kBits bits, written
+    // as kDig balanced base-256 digits. Digit a of (row, k) sits in the
+    // 16-byte slot [a][tile][kstep][lane], lane = row % 16 + 16 g: byte
+    // k % 16 with g = (k % 64) / 16 for a 16x16x64 step, byte k % 8 with
+    // g = (k % 32) / 8 for the 16x16x32 tail (the same K placement the
+    // kernel uses for w).
+    const int K64 = i8_k64(NP);
+    rowexp.assign((size_t)hc.MT * 16, 0);
+    buf.assign(nfrag * 16 * kDig, 0);
+    for (int row = 0; row < hc.MT * 16; ++row) {
+      double mx = 0.0;
+      for (int k = 0; k < NP; ++k) mx = std::max(mx, std::fabs(val(row, k)));
+      int E = 0;
+      if (mx > 0.0) (void)std::frexp(mx, &E);
+      rowexp[row] = E;
+      const int tt = row / 16, lr = row % 16;
+      for (int k = 0; k < NP; ++k) {
+        long long X = std::llrint(std::ldexp(val(row, k), kBits - E));
+        const int ks = k < 64 * K64 ? k / 64 : K64;
+        const int l = ks < K64 ? lr + 16 * ((k % 64) / 16) : lr + 16 * ((k - 64 * K64) / 8);
+        const int b = ks < K64 ? k % 16 : (k - 64 * K64) % 8;
+        for (int d = 0; d < kDig; ++d) {
+          long long v = X & 255, cy = v >> 7;
+          const int dg = d < kDig - 1 ? (int)(v - (cy << 8)) : (int)X;
+          if (d < kDig - 1) X = (X >> 8) + cy;
+          DGK_REQUIRE(dg >= -128 && dg <= 127, "dgk_deriv_plan_create: digit out of range");
+          const size_t q = (((size_t)d * hc.MT + tt) * hc.KS + ks) * 64 + l;
+          buf[q * 16 + b] = (unsigned char)(signed char)dg;
+        }
+      }
+    }
+  } else if (prec == kF64 || prec == kF32) {
     buf.resize(nfrag * (prec == kF64 ? 8 : 4));
     for (int tt = 0; tt < hc.MT; ++tt)
       for (int ks = 0; ks < hc.KS; ++ks)
@@ -713,6 +940,28 @@ extern "C" int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr, co
       lam[2 * NP + i] = 0.5 * (1.0 + t[i]);
     }
   }
+  // I8X6: Lu = a0 + a1 l1 + a2 l2 + a3 l3 enters as its exact slopes (a1, a2,
+  // a3)/2 in the epilogue; the stored operator gives D Lu, which differs by
+  // its round-off (D 1 is ~1e-11, not 0, at P=6). The row corrections
+  // [D 1, D l1 - e1, D l2 - e2, D l3 - e3] restore the stored operator's
+  // action, so the plan agrees with FP64 to its own rounding.
+  std::vector<double> rowcorr;
+  if (prec == kI8) {
+    rowcorr.assign((size_t)hc.MT * 64, 0.0);
+    for (int row = 0; row < 3 * NP; ++row) {
+      const int blk = row / NP;
+      long double c[4] = {0, 0, 0, 0};
+      for (int k = 0; k < NP; ++k) {
+        const long double v = val(row, k);
+        c[0] += v;
+        c[1] += v * lam[k];
+        c[2] += v * lam[NP + k];
+        c[3] += v * lam[2 * NP + k];
+      }
+      c[1 + blk] -= 0.5L;
+      for (int q = 0; q < 4; ++q) rowcorr[(size_t)row * 4 + q] = (double)c[q];
+    }
+  }
   dgk_deriv_plan p = new dgk_deriv_plan_s();
   p->dev = dev;
   p->N = N;
@@ -721,10 +970,16 @@ extern "C" int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr, co
   int rc = dgk_malloc(dev, buf.size(), buf.data(), &p->frag);
   if (rc == DGK_OK && !lam.empty())
     rc = dgk_malloc(dev, lam.size() * sizeof(double), lam.data(), (void**)&p->lam);
+  if (rc == DGK_OK && !rowexp.empty())
+    rc = dgk_malloc(dev, rowexp.size() * sizeof(int), rowexp.data(), (void**)&p->rowexp);
+  if (rc == DGK_OK && !rowcorr.empty())
+    rc = dgk_malloc(dev, rowcorr.size() * sizeof(double), rowcorr.data(), (void**)&p->rowcorr);
   if (rc == DGK_OK) rc = dispatch(p, nullptr, 0);
   if (rc != DGK_OK) {
     if (p->frag) dgk_free(dev, p->frag);
     if (p->lam) dgk_free(dev, p->lam);
+    if (p->rowexp) dgk_free(dev, p->rowexp);
+    if (p->rowcorr) dgk_free(dev, p->rowcorr);
     delete p;
     return rc;
   }
@@ -736,6 +991,8 @@ extern "C" int dgk_deriv_plan_destroy(dgk_deriv_plan p) {
   if (!p) return DGK_OK;
   dgk_free(p->dev, p->frag);
   if (p->lam) dgk_free(p->dev, p->lam);
+  if (p->rowexp) dgk_free(p->dev, p->rowexp);
+  if (p->rowcorr) dgk_free(p->dev, p->rowcorr);
   delete p;
   return DGK_OK;
 }
@@ -756,6 +1013,8 @@ extern "C" int dgk_phys_deriv_affine(dgk_deriv_plan p, int64_t K, const double* 
   a.ntiles = 0;
   a.frag = p->frag;
   a.lam = p->lam;
+  a.rowexp = p->rowexp;
+  a.rowcorr = p->rowcorr;
   std::memcpy(a.vtx, p->vtx, sizeof a.vtx);
   a.u = u_dev;
   a.geo = geo_dev;
@@ -775,6 +1034,8 @@ int dgk::deriv_plan_divergence(dgk_deriv_plan p, const dgk::Tile* tiles, int nti
   a.ntiles = ntiles;
   a.frag = p->frag;
   a.lam = p->lam;
+  a.rowexp = p->rowexp;
+  a.rowcorr = p->rowcorr;
   std::memcpy(a.vtx, p->vtx, sizeof a.vtx);
   a.u = u;
   a.geo = geo;

@@ -253,8 +253,8 @@ class DerivPlan:
     """Affine physical derivative on matrix cores (dgk_deriv_plan).
 
     ``prec`` is one of ``_lib.DGK_PREC_F64``, ``DGK_PREC_F32``,
-    ``DGK_PREC_BF16X3``, ``DGK_PREC_BF16X6``; outputs are FP64 for the first
-    and FP32 otherwise. ``D`` is (Dr, Ds, Dt), Np x Np row-major; ``rst`` the
+    ``DGK_PREC_BF16X3``, ``DGK_PREC_BF16X6``, ``DGK_PREC_I8X6``; outputs are
+    FP64 for F64 and I8X6 and FP32 otherwise. ``D`` is (Dr, Ds, Dt), Np x Np row-major; ``rst`` the
     reference node coordinates (required below FP64).
     """
 
@@ -267,7 +267,8 @@ class DerivPlan:
         _lib.call("dgk_deriv_plan_create", dev.h, N, *(P(m) for m in D), *(P(v) for v in R), prec,
                   C.byref(h))
         self.h = h
-        self.out_dtype = np.float64 if prec == _lib.DGK_PREC_F64 else np.float32
+        self.out_dtype = (np.float64 if prec in (_lib.DGK_PREC_F64, _lib.DGK_PREC_I8X6)
+                          else np.float32)
 
     @classmethod
     def from_ref(cls, dev: _lib.Device, ref: RefTet, prec: int = _lib.DGK_PREC_F64):

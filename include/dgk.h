@@ -210,10 +210,21 @@ int dgk_phys_deriv_nodal_noalloc(dgk_device dev, int N, int npart, const int64_t
  *   (Henry, Tang & Heinecke, ARITH 2019), ~16 bits per operand.
  * DGK_PREC_BF16X6: three-part bf16 splits of the FP32 value, six products,
  *   ~24 bits.
- * Low-precision modes accumulate in FP32 and write FP32 outputs; the slopes
- * of Lu and the metrics are applied in FP32 (a few FP32 ulps, below the
- * products' own error). */
-enum { DGK_PREC_F64 = 0, DGK_PREC_F32 = 1, DGK_PREC_BF16X3 = 2, DGK_PREC_BF16X6 = 3 };
+ * These accumulate in FP32 and write FP32 outputs; the slopes of Lu and the
+ * metrics are applied in FP32 (a few FP32 ulps, below the products' own
+ * error).
+ * DGK_PREC_I8X6: exact-integer split on the INT8 matrix cores (Ozaki scheme):
+ *   operator rows and each element's remainder scaled by powers of two and
+ *   written as six signed 8-bit digits (46 bits); the 21 digit pairs of
+ *   highest weight accumulate exactly in int32 and are combined in FP64.
+ *   FP64 outputs, ~3e-12 relative to FP64. */
+enum {
+  DGK_PREC_F64 = 0,
+  DGK_PREC_F32 = 1,
+  DGK_PREC_BF16X3 = 2,
+  DGK_PREC_BF16X6 = 3,
+  DGK_PREC_I8X6 = 4
+};
 typedef struct dgk_deriv_plan_s* dgk_deriv_plan;
 int dgk_deriv_plan_create(dgk_device dev, int N, const double* Dr_host, const double* Ds_host,
                           const double* Dt_host, const double* r_host, const double* s_host,
@@ -327,7 +338,8 @@ int dgk_burgers_set_variant(dgk_dgmesh m, int variant);
 int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked);
 /* Precision of the Dr/Ds/Dt and LIFT contractions in the stage (DGK_PREC_*,
  * default DGK_PREC_F64); OR-ing DGK_PREC_LIFT_F64 keeps the LIFT product in
- * FP64 while Dr/Ds/Dt run in the given precision. The state, geometry, flux,
+ * FP64 while Dr/Ds/Dt run in the given precision (DGK_PREC_I8X6 always
+ * keeps LIFT in FP64). The state, geometry, flux,
  * RK combination and dt stay FP64. Below FP64, and for N = 5, 6 at any
  * precision, the stage runs split: a volume kernel (-div f(u) through a
  * derivative plan, with the linear-part removal of dgk_phys_deriv_affine)

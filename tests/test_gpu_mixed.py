@@ -14,13 +14,16 @@ pytestmark = pytest.mark.gpu
 
 METRICS = ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz")
 PRECS = {"f64": _lib.DGK_PREC_F64, "f32": _lib.DGK_PREC_F32, "bf16x3": _lib.DGK_PREC_BF16X3,
-         "bf16x6": _lib.DGK_PREC_BF16X6}
+         "bf16x6": _lib.DGK_PREC_BF16X6, "i8x6": _lib.DGK_PREC_I8X6}
 # Bounds on the error against FP64 (relative L2, max error scaled by max |du|)
 # for FP32-accumulated products of length Np <= 84 on O(1) data. FP32
 # operands carry 24 bits (measured ~1e-7); the two-part bf16 split carries
 # 16 bits per operand and drops lo*lo, so BF16x3 sits near 2^-16 / sqrt(Np)
 # (measured ~2e-6); the three-part split (six products) is at FP32 level.
-BOUND = {"f32": (1e-6, 1e-5), "bf16x3": (1e-5, 5e-5), "bf16x6": (1e-6, 1e-5)}
+# The exact-integer split (I8X6) carries 46 bits per operand and exact int32
+# sums: FP64-level, bounded here at 1e-11 (L2) and 1e-10 (max).
+BOUND = {"f32": (1e-6, 1e-5), "bf16x3": (1e-5, 5e-5), "bf16x6": (1e-6, 1e-5),
+         "i8x6": (1e-11, 1e-10)}
 
 
 def _run(device, N, D, U, geo, prec, rst):
@@ -66,7 +69,7 @@ def test_affine_fp64_vs_oracle(device, N):
         assert np.max(np.abs(got[d] - host[d])) <= 1e-13 * scale
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6", "i8x6"])
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_affine_mixed_error_bounds(device, N, prec):
     el, U, geo, host = _mesh_case(N, seed=22)
@@ -81,7 +84,7 @@ def test_affine_mixed_error_bounds(device, N, prec):
         assert e2 > 0.0
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3", "bf16x6", "i8x6"])
 @pytest.mark.parametrize("N", [4, 6])
 def test_affine_grid_stride_and_ragged(device, prec, N):
     """K above one sweep of the persistent grid, K % 16 != 0 (K Np odd at N=4),

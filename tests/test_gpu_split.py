@@ -21,7 +21,8 @@ def _metric_spread(o):
     return max(float(np.max(np.abs(mm - mm[:, :1])) / np.max(np.abs(mm)))
                for mm in (o.field(k) for k in METRICS))
 
-PRECS = {"f32": _lib.DGK_PREC_F32, "bf16x3": _lib.DGK_PREC_BF16X3, "bf16x6": _lib.DGK_PREC_BF16X6}
+PRECS = {"f32": _lib.DGK_PREC_F32, "bf16x3": _lib.DGK_PREC_BF16X3, "bf16x6": _lib.DGK_PREC_BF16X6,
+         "i8x6": _lib.DGK_PREC_I8X6}
 
 
 @pytest.mark.parametrize("N,n,parts", [(5, 4, [100, 284]), (6, 4, [384]), (6, 5, [250, 1, 499])])
@@ -81,16 +82,18 @@ def test_p6_ssprk3_vs_oracle(device):
 #   f32     P=3 2.3e-8   P=5 2.8e-7   P=6 4.5e-6
 #   bf16x3  P=3 9.9e-7   P=5 4.9e-5   P=6 5.7e-4
 #   bf16x6  P=3 7.1e-9   P=5 2.7e-7   P=6 2.0e-6
+#   i8x6    P=3 1.2e-14  P=5 2.7e-13  P=6 6.9e-12
 # A contraction that dropped a split part (plain BF16 operands) carries at
 # least ~30x the BF16x3 error (tests/test_precision_model.py), so it fails.
 BOUND = {
     (3, "f32"): 3e-7, (5, "f32"): 3e-6, (6, "f32"): 5e-5,
     (3, "bf16x3"): 1e-5, (5, "bf16x3"): 5e-4, (6, "bf16x3"): 5e-3,
     (3, "bf16x6"): 1e-7, (5, "bf16x6"): 3e-6, (6, "bf16x6"): 2e-5,
+    (3, "i8x6"): 2e-13, (5, "i8x6"): 3e-12, (6, "i8x6"): 7e-11,
 }
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6", "i8x6"])
 @pytest.mark.parametrize("N", [3, 5, 6])
 def test_mixed_precision_stage_vs_fp64(device, N, prec):
     m = M.cube_mesh(4, 4, 4, periodic=True)
