@@ -362,6 +362,74 @@ def bench_c4_stage(a, dev) -> dict:
     return out
 
 
+def p6_section(dev, seed: int, steps: int = 5, reps: int = 10) -> dict:
+    """The P=6 half of the metric, for the default bench line: the 64^3 x 6 =
+    1,572,864-tet periodic P=6 problem (2 slab partitions, u0 = 0.5 + 0.4
+    sin(2 pi x) cos(2 pi y) sin(2 pi z) + U(+-1e-6)). SSP-RK3 steps with the
+    Dr/Ds/Dt contractions in FP64, in the exact-integer split (I8X6) and in
+    BF16x3 (LIFT in FP64 for both), each after a sustained warm-up and compared
+    with the FP64 run after the same steps; and the physical derivative alone
+    (three outputs) in the same three plans."""
+    from dgkernel_amd import _lib, dg
+    from dgkernel_amd import mesh as M
+    N = 6
+    m = M.cube_mesh(64, 64, 64, periodic=True)
+    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, 2, 0.0, seed=seed))
+    x, y, z = st.node_coords()
+    u0 = (0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+          + M.uniform(seed, 7, x.size, -1e-6, 1e-6).reshape(x.shape))
+    del x, y, z
+    K, Np = m.K, st.ref.Np
+    out = {"workload": f"Burgers SSP-RK3, 64^3x6={K} periodic tets, P=6, 2 slab partitions, "
+                       f"{steps} steps per plan", "stage": {}, "derivative": {}}
+    ref, hmin = None, None
+    for name, prec in (("f64", 0), ("i8x6", 4), ("bf16x3_lift64", 2 | 16)):
+        solver = dg.BurgersSolver(dev, st, cfl=0.3, hmin=hmin)
+        hmin = solver.hmin
+        solver.mesh.set_precision(prec)
+        solver.set_state(u0)
+        _warm(dev, lambda: solver.step(1), min_calls=1, min_ms=WARM_STAGE_MS)
+        solver.set_state(u0)
+        dev.sync()
+        t = dev.timer()
+        t.start()
+        solver.step(steps)
+        t.stop()
+        ms = t.ms() / steps
+        u = solver.state()
+        rec = {"ms_per_step": ms, "value": K * Np * 3 / (ms * 1e-3)}
+        if ref is None:
+            ref = u
+        else:
+            rec["err_vs_f64"] = _errs(u, ref)
+        out["stage"][name] = rec
+        log(f"p6 stage {name}: {ms:.3f} ms/step {rec.get('err_vs_f64', '')}")
+        del solver
+    mesh = dg.DGMesh(dev, st)
+    geo_ptr = mesh.geometry_dev_ptr()
+    du = dev.upload(u0.reshape(-1))
+    D, rst = (st.ref.Dr, st.ref.Ds, st.ref.Dt), (st.ref.r, st.ref.s, st.ref.t)
+    ref = None
+    for name, prec in (("f64", 0), ("i8x6", 4), ("bf16x3", 2)):
+        plan = dg.DerivPlan(dev, N, D, prec, rst)
+        outs = [dev.empty(K * Np, plan.out_dtype) for _ in range(3)]
+        ms = _time_ms(dev, lambda: plan.apply(K, du, geo_ptr, K, outs), reps)
+        got = [o.to_host().astype(np.float64) for o in outs]
+        rec = {"ms": ms}
+        if ref is None:
+            ref = got
+        else:
+            e = [_errs(g, r) for g, r in zip(got, ref)]
+            rec["err_vs_f64"] = {"rel_l2": max(q["rel_l2"] for q in e),
+                                 "max_rel": max(q["max_rel"] for q in e)}
+            rec["speedup_vs_f64"] = out["derivative"]["f64"]["ms"] / ms
+        out["derivative"][name] = rec
+        log(f"p6 derivative {name}: {ms:.4f} ms {rec.get('err_vs_f64', '')}")
+        del outs, got
+        plan.free()
+    return out
+
+
 def bench_c5(a, dev) -> dict:
     """Learned artificial viscosity: 64x64x85 hexes x 6 = 2,088,960 periodic
     tets (2^21 is not a multiple of 6), P=4, 2 slab partitions,
@@ -486,6 +554,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-p6", action="store_true",
+                    help="leave the P=6 section (C4 stage and derivative) out of the stage line")
     ap.add_argument("--mode", choices=["stage", "deriv", "c4", "c5"], default="stage")
     ap.add_argument("--c5-n", type=int, default=64, help="hexes in x and y (c5)")
     ap.add_argument("--nz", type=int, default=85, help="hexes in z (c5)")
@@ -611,6 +681,12 @@ def main(argv=None) -> int:
                            "traffic_bytes_per_launch" if morton
                            else "traffic_bytes_per_launch_generator_order")
 
+    kname = stage_kernel_name(a.order, solver)
+    del solver
+    p6 = None
+    if not a.no_p6:
+        p6 = p6_section(dev, a.seed)
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -643,7 +719,7 @@ def main(argv=None) -> int:
                    "element_order": "morton" if morton else "generator"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": stage_kernel_name(a.order, solver),
+                     "kernel": kname,
                      "bytes_per_elem": bpe,
                      "stage_ms_mean": float(stage_ms.mean()),
                      "fp64_tflops": flops, "fp64_frac": flops / FP64_PEAK_TFLOPS},
@@ -651,6 +727,7 @@ def main(argv=None) -> int:
         "graph_ms_per_step": None if graph_ms is None else graph_ms / a.steps,
         "graph_value": None if graph_ms is None else dof_stage / (graph_ms * 1e-3),
         "state_finite": finite,
+        "p6": p6,
         "device": dev.name,
     }
     print(json.dumps(out), flush=True)
