@@ -430,6 +430,46 @@ def p6_section(dev, seed: int, steps: int = 5, reps: int = 10) -> dict:
     return out
 
 
+def bench_partition(a) -> dict:
+    """Partition quality on the n^3 x 6 periodic cube (host only): faces
+    between partitions (the exchange's face count), communication volume
+    (per element, the other partitions among its face neighbours), load
+    imbalance and wall time, for the stage bench's slabs (+-40 %), equal
+    slabs, coordinate bisection and the k-way graph partitioner (cut and
+    volume objectives, 5 % imbalance)."""
+    from dgkernel_amd import mesh as M
+    from dgkernel_amd import partitioner as P
+    m = M.cube_mesh(a.n, a.n, a.n, periodic=True)
+    k = a.parts
+    EToE, _ = M.connect(m)
+    own = np.repeat(np.arange(m.K), 4).reshape(m.K, 4)
+
+    def quality(etop, secs):
+        c = (EToE != own) & (etop[EToE] != etop[own])
+        pe = np.sort(np.where(c, etop[EToE], -1), axis=1)
+        first = np.concatenate([np.ones((m.K, 1), bool), pe[:, 1:] != pe[:, :-1]], axis=1)
+        cnt = np.bincount(etop, minlength=k)
+        return {"cut_faces": int(c.sum()) // 2, "volume": int(((pe >= 0) & first).sum()),
+                "imbalance": float(cnt.max() / cnt.mean() - 1.0), "seconds": secs}
+
+    out = {"metric": "partition quality", "workload": f"{a.n}^3x6={m.K} periodic tets, {k} partitions",
+           "partitions": {}}
+    kp40 = M.slab_partition(m.K, k, 0.4, seed=a.seed)
+    kp = M.slab_partition(m.K, k, 0.0)
+    out["partitions"]["slabs_imbalanced_40pct"] = quality(M.etop_from_counts(kp40), 0.0)
+    out["partitions"]["slabs"] = quality(M.etop_from_counts(kp), 0.0)
+    t = time.perf_counter()
+    e = M.rcb_partition(m, kp)
+    out["partitions"]["rcb"] = quality(e, time.perf_counter() - t)
+    for obj in ("cut", "vol"):
+        t = time.perf_counter()
+        e = P.kway_partition(m, k, obj, seed=a.seed)
+        out["partitions"][f"kway_{obj}"] = quality(e, time.perf_counter() - t)
+    for name, q in out["partitions"].items():
+        log(f"{name}: {q}")
+    return out
+
+
 def bench_c5(a, dev) -> dict:
     """Learned artificial viscosity: 64x64x85 hexes x 6 = 2,088,960 periodic
     tets (2^21 is not a multiple of 6), P=4, 2 slab partitions,
@@ -556,7 +596,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-p6", action="store_true",
                     help="leave the P=6 section (C4 stage and derivative) out of the stage line")
-    ap.add_argument("--mode", choices=["stage", "deriv", "c4", "c5"], default="stage")
+    ap.add_argument("--mode", choices=["stage", "deriv", "c4", "c5", "partition"], default="stage")
     ap.add_argument("--c5-n", type=int, default=64, help="hexes in x and y (c5)")
     ap.add_argument("--nz", type=int, default=85, help="hexes in z (c5)")
     ap.add_argument("--t-end", type=float, default=0.4, help="final time (c5)")
@@ -581,6 +621,9 @@ def main(argv=None) -> int:
 
     from dgkernel_amd import _lib, dg
 
+    if a.mode == "partition":
+        print(json.dumps(bench_partition(a)), flush=True)
+        return 0
     if a.copy_calib:
         dev = _lib.Device()
         src, dst = dev.zeros(a.copy_calib // 8), dev.zeros(a.copy_calib // 8)

@@ -99,6 +99,8 @@ _SIGS = {
     "dgk_phys_deriv_affine": (_i32, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
     "dgk_matmul": (_i32, [_vp, _i32, _i32, _vp, _i32, _pi64, _vp, _vp, _vp, _vp, _i32]),
     "dgk_face_plan_create": (_i32, [_vp, _i32, _pi64, _pi32, _pi32, C.POINTER(_vp)]),
+    "dgk_partition_kway": (_i32, [_i64, _pi64, _pi64, _pi64, _pi64, _i32, _f64, _i32, C.c_uint64,
+                                  _pi32, _pi64]),
     "dgk_face_plan_destroy": (_i32, [_vp]),
     "dgk_face_exchange": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "dgk_dgmesh_create": (_i32, [_vp, C.POINTER(DGMeshDesc), C.POINTER(_vp)]),

@@ -47,7 +47,7 @@ def _run(cmd: list[str]) -> None:
 
 def build_lib(jobs: int = 8) -> Path:
     BUILD.mkdir(exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
+    sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     objs = [BUILD / (s.stem + ".o") for s in sources]
     cmds = [[HIPCC, *HIP_FLAGS, "-c", str(s), "-o", str(o)] for s, o in zip(sources, objs)]
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:

@@ -431,6 +431,24 @@ int dgk_mlp_grad(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t coun
 int dgk_mlp_train_step(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t ndata,
                        int batch, uint64_t seed, float lr, float* loss_acc_dev);
 
+/* ---------------------------------------------------------------------- */
+/* Element-graph partitioning (host only: no device is needed or touched). */
+/* Replaces metis.PartGraphKwayWeighted as MeshPartitioner.Partition calls */
+/* it (mesh/partitioner/mesh_partitioner.go:87-142; graph from             */
+/* buildMetisGraph, :145-184).                                              */
+/* ---------------------------------------------------------------------- */
+/* Multilevel k-way partition of the CSR graph (xadj [nv+1], adjncy        */
+/* [xadj[nv]], symmetric); vwgt [nv] and adjwgt [xadj[nv]] may be NULL      */
+/* (weights 1). part[v] in [0, nparts); every part weighs at most           */
+/* max(ubfactor * total / nparts, ceil(total / nparts)) unless the vertex   */
+/* weights make that impossible. objective 0 minimises the edge cut (sum of */
+/* adjwgt over edges between parts, each once), 1 the communication volume */
+/* (sum over vertices of the number of other parts among the neighbours);  */
+/* *objval (may be NULL) receives that value. Deterministic for a seed.     */
+int dgk_partition_kway(int64_t nv, const int64_t* xadj, const int64_t* adjncy,
+                       const int64_t* vwgt, const int64_t* adjwgt, int nparts, double ubfactor,
+                       int objective, uint64_t seed, int32_t* part, int64_t* objval);
+
 #ifdef __cplusplus
 }
 #endif

@@ -322,6 +322,32 @@ def test_burgers_rhs_rcb_partitions_vs_oracle(device, N):
         assert np.max(np.abs(got - want)) <= (1e-12 + 4.0 * spread) * np.max(np.abs(want)), variant
 
 
+@pytest.mark.parametrize("N", [2, 3])
+def test_burgers_kway_partitions_vs_oracle(device, N):
+    """Partitions from the multilevel k-way graph partitioner (7 parts, volume
+    objective), elements renumbered partition-major: the stage against the
+    oracle's L(u), and 10 SSP-RK3 steps against the oracle's."""
+    from dgkernel_amd import partitioner as P
+    m = M.cube_mesh(6, 6, 6, periodic=True, jitter=0.05, seed=4)
+    mo, kp = M.order_by_partition(m, P.kway_partition(m, 7, "vol", seed=2))
+    assert kp.size == 7 and kp.min() > 0
+    st = dg.make_setup(mo, N, Kp=kp)
+    u = smooth_u0(st, noise=1e-3)
+    o = O.Setup.from_mesh(N, mo)
+    want = o.burgers_rhs(u)
+    spread = max(float(np.max(np.abs(mm - mm[:, :1])) / np.max(np.abs(mm)))
+                 for mm in (o.field(k) for k in METRICS))
+    solver = dg.BurgersSolver(device, st)
+    got = solver.rhs(u)
+    assert np.max(np.abs(got - want)) <= (1e-12 + 4.0 * spread) * np.max(np.abs(want))
+    hmin = o.min_node_distance()
+    want, _ = o.ssprk3(u, 10, 0.3, hmin)
+    solver = dg.BurgersSolver(device, st, cfl=0.3, hmin=hmin)
+    solver.set_state(u)
+    solver.step(10)
+    assert rel_l2(solver.state(), want) <= 1e-11
+
+
 @pytest.mark.parametrize("variant", [1, 2, 5, 6])
 def test_config1_burgers_100_steps_vs_oracle(device, variant):
     """Config 1: 8^3 hexes x 6, P=2, 4 slabs, LF + SSP-RK3, CFL 0.3, 100 steps."""
