@@ -32,6 +32,7 @@ namespace {
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kF64 = 0, kF32 = 1, kBF3 = 2, kBF6 = 3;
 constexpr bool is_bf(int prec) { return prec == kBF3 || prec == kBF6; }
@@ -41,27 +42,41 @@ template <int N, int PREC>
 struct SCfg {
   static constexpr int NP = dgk::Tet<N>::Np, NFP = dgk::Tet<N>::Nfp, NF = 4 * NFP;
   static constexpr int MTL = (NP + 15) / 16;  // 16-row tiles of LIFT
-  // MFMA steps over the face points: one point per face (k = 4) or eight
-  // points per face (bf16, k = 32, padded past Nfp with zero columns).
-  static constexpr int KSL = is_bf(PREC) ? (NFP + 7) / 8 : NFP;
-  static constexpr int NPT = is_bf(PREC) ? 8 * KSL : NFP;  // padded points per face
+  // LIFT columns face by face: the MFMA steps of face f take its points four
+  // at a time (FP64/FP32, k = 4) or 32 at a time (bf16, k = 32), with zero
+  // columns past Nfp.
+  static constexpr int KB = is_bf(PREC) ? 32 : 4;
+  static constexpr int KSF = (NFP + KB - 1) / KB;  // steps per face
+  static constexpr int NPT = KB * KSF;             // padded points per face
   static constexpr int PARTS = parts_of(PREC);
-  static constexpr size_t NFRAG = (size_t)MTL * KSL * 64;  // fragments per part
+  static constexpr size_t NFRAG = (size_t)MTL * 4 * KSF * 64;  // fragments per part
   static constexpr size_t FRAG_BYTES =
       PREC == kF64 ? NFRAG * 8 : (PREC == kF32 ? NFRAG * 4 : NFRAG * 16 * PARTS);
-  // Waves per block: three per SIMD to cover the latency of the trace
-  // gathers. More waves only add contention for the L1's outstanding misses
-  // (four per SIMD made the BF16x3 kernel ~8 % slower at P=6), fewer leave
-  // latency uncovered. BF16x3 runs them as two blocks of six (its fragments
-  // leave room in LDS for two), which measured 0.6 % faster per step at P=6.
-  static constexpr int W = PREC == kBF3 ? 6 : 12;
-  // Per-wave LDS buffer: a quarter tile (4 elements, row stride NPH) for the
-  // own traces, reused as the 16 x 17 update buffer.
-  static constexpr int NPH = NP | 1;
-  static constexpr int BUF = 4 * NPH > 16 * 17 ? 4 * NPH : 16 * 17;
-  static constexpr int NVH = (4 * NP + 63) / 64;  // quarter-tile values per lane
-  // Own traces staged through LDS (unit-stride loads) or gathered per lane.
-  static constexpr bool STAGE = PREC == kF32;
+  // Waves per block: two per SIMD, in one block per CU, or for BF16x3 in
+  // two blocks of four (its fragments leave room in LDS for two). The
+  // kernel keeps its own loads in flight two phases ahead; a third wave per
+  // SIMD would not fit the registers that takes (it spilled, and ran 20-30 %
+  // slower).
+  static constexpr int W = PREC == kBF3 ? 4 : 8;
+  // Trace gathers of one face: lanes along its 16 x Nfp (element, point)
+  // pairs, point fastest, so one load instruction touches the nodes of two
+  // or three neighbours rather than one per lane.
+  static constexpr int NG = (16 * NFP + 63) / 64;
+  // The jumps of one face in LDS, [point][element] with row stride 17 (the
+  // point-major writes and element-major reads are both conflict-free), in
+  // FP64 where the product needs more than FP32 carries; the same per-wave
+  // buffer serves as the 16 x 17 update buffer.
+  using JT = typename std::conditional<PREC == kF64 || PREC == kBF6, double, float>::type;
+  static constexpr size_t BUF_BYTES =
+      NPT * 17 * sizeof(JT) > 16 * 17 * 8 ? NPT * 17 * sizeof(JT) : 16 * 17 * 8;
+  static constexpr int BUF = (int)(BUF_BYTES / 8);
+};
+
+// Per-wave face record for the gathers: value offset of the neighbour's
+// node 0, row of the orientation table, flux factors.
+struct FaceRec {
+  int32_t nb, fnrow;
+  double fa, fb;
 };
 
 struct SurfArgs {
@@ -100,38 +115,60 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per tile, each wave: lane (c, g) gathers the neighbour trace of face g
-// (all loads in flight together), reads its own trace from the tile staged
-// in LDS a quarter at a time with unit-stride loads, and forms the LF jumps as its
-// B operand; then, per 16-row tile of LIFT: the rhs, u, u0 values this lane
-// will update are loaded, the product runs on matrix cores, and the result
-// goes through the wave's 16 x 17 LDS buffer so that each lane updates and
-// stores rows of 16 consecutive values.
+// Per tile, each wave:
+//   1. lanes along each face's (element, point) pairs hold the own and
+//      neighbour traces (face 0 was gathered during the previous tile; the
+//      others are gathered two faces ahead of their use) and
+//      form the LF jumps face by face into the LDS buffer, from which each
+//      lane reads its B operands of the face's MFMA steps into registers;
+//   2. the next tile's face records (neighbour offset, orientation row, flux
+//      factors of face g of element c at lane (c, g)) go to the other record
+//      buffer, and its face 0 is gathered;
+//   3. per 16-row tile of LIFT: the products over the four faces, then the
+//      result goes through the wave's 16 x 17 buffer so that each lane
+//      updates and stores rows of 16 consecutive values. The rhs, u, u0
+//      values of row tile tt + 1 are loaded before the products of row
+//      tile tt (those of row tile 0 during the jumps).
+// Wait counts are in order, so every prefetch is consumed after the loads
+// issued before it, never before.
 template <int N, int PREC>
-__global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel(SurfArgs a) {
+__global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_kernel(SurfArgs a) {
   using C = SCfg<N, PREC>;
-  constexpr int NP = C::NP, NFP = C::NFP, NF = C::NF, MTL = C::MTL, KSL = C::KSL, W = C::W;
-  constexpr int NPT = C::NPT, PARTS = C::PARTS, NPH = C::NPH, NVH = C::NVH;
-  constexpr bool STAGE = C::STAGE;
+  using JT = typename C::JT;
+  constexpr int NP = C::NP, NFP = C::NFP, MTL = C::MTL, KSF = C::KSF, W = C::W;
+  constexpr int NPT = C::NPT, PARTS = C::PARTS, NG = C::NG;
   constexpr dgk::FmaskTable<N> FM{};
   __shared__ __attribute__((aligned(16))) unsigned char sA[C::FRAG_BYTES];
-  __shared__ double sTall[W][C::BUF];
+  __shared__ __attribute__((aligned(16))) double sTall[W][C::BUF];
+  __shared__ FaceRec sRall[W][2][64];
   __shared__ uint8_t sFN[4 * 24 * NFP];
-  __shared__ uint8_t sFM[NF];
+  __shared__ uint8_t sFM[4 * NFP];
   {
     const uint4* src = (const uint4*)a.frag;
     uint4* dst = (uint4*)sA;
     for (int i = threadIdx.x; i < (int)(C::FRAG_BYTES / 16); i += 64 * W) dst[i] = src[i];
     for (int i = threadIdx.x; i < 4 * 24 * NFP; i += 64 * W) sFN[i] = a.face_nodes[i];
-    for (int i = threadIdx.x; i < NF; i += 64 * W) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
+    for (int i = threadIdx.x; i < 4 * NFP; i += 64 * W) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
   }
   __syncthreads();
-  const double dt = a.mode == 0 ? *a.dt : 0.0;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int c = lane & 15, g = lane >> 4;  // element of the tile, face
+  // wave index as a scalar, so that tile indices and records stay in SGPRs
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15, g = lane >> 4;  // element of the tile, face (or k group)
   double* sT = sTall[w];
+  JT* sJ = (JT*)sT;
   const int64_t K = a.K;
   double lmax = 0.0;
+  // Gather pair of instruction j: element pc[j] of the tile, point pq[j].
+  int pc[NG], pq[NG];
+  bool pv[NG];
+#pragma unroll
+  for (int j = 0; j < NG; ++j) {
+    const int p = lane + 64 * j;
+    pv[j] = p < 16 * NFP;
+    const int pp = pv[j] ? p : 16 * NFP - 1;
+    pc[j] = pp / NFP;
+    pq[j] = pp - pc[j] * NFP;
+  }
   // Tile order: with a grid that is a multiple of 8, block b runs on XCD
   // b % 8 (round-robin dispatch), and each XCD walks one contiguous eighth of
   // the tile list, so neighbour traces mostly hit that XCD's L2; otherwise a
@@ -143,152 +180,203 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W)) surface_update_kernel
     t1 = (int)((int64_t)a.ntiles * (x + 1) / 8);
     ts = (gridDim.x / 8) * W;
   }
-  for (int t = t0; t < t1; t += ts) {
-    const dgk::Tile tile = a.tiles[t];
+  // Tile records are read one tile ahead and made wave-uniform only when
+  // used, so waiting for one never drains the loads issued after it.
+  auto uniform = [](const dgk::Tile& v) -> dgk::Tile {
+    dgk::Tile u;
+    u.vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v.vbase >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)v.vbase));
+    u.gbase = __builtin_amdgcn_readfirstlane(v.gbase);
+    u.count = __builtin_amdgcn_readfirstlane(v.count);
+    return u;
+  };
+  // Face-record inputs of lane (c, g) for a tile (element clamped to the
+  // tile, so every load stays in it).
+  int32_t rnb = 0, rcode = 0;
+  double rga = 0.0, rgb = 0.0, rgc = 0.0;
+  auto rec_load = [&](const dgk::Tile& tl) {
+    const int ce = c < tl.count ? c : tl.count - 1;
+    const int64_t ge = tl.gbase + ce;
+    rnb = a.nbr[(int64_t)g * K + ge];
+    rcode = a.codes[ge];
+    rga = a.bgeo[ge];
+    rgb = a.bgeo[K + ge];
+    rgc = a.bgeo[2 * K + ge];
+  };
+  auto rec_store = [&](FaceRec* sR) {
+    FaceRec r;
+    r.nb = rnb;
+    r.fnrow = (g * 24 + ((rcode >> (5 * g)) & 31)) * NFP;
+    dgk::face_factors(g, rga, rgb, rgc, r.fa, r.fb);
+    sR[g * 16 + c] = r;
+    wave_sync();
+  };
+  // Own and neighbour traces of face f of a tile at the pairs of this lane:
+  // two faces in flight, face f in slot f % 2.
+  double uM[2][NG], uP[2][NG];
+  auto gather = [&](const dgk::Tile& tl, const FaceRec* sR, int f) {
+    const double* ut = a.u + tl.vbase;
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      const int ce = pc[j] < tl.count ? pc[j] : tl.count - 1;
+      const FaceRec& r = sR[f * 16 + pc[j]];
+      uP[f & 1][j] = a.u[r.nb + sFN[r.fnrow + pq[j]]];
+      uM[f & 1][j] = ut[ce * NP + sFM[f * NFP + pq[j]]];
+    }
+  };
+  // mode 1 (rhs + LIFT g) is the update with a0 = 0, a1 = dt = 1 and u = 0
+  const double dt = a.mode == 0 ? *a.dt : 1.0;
+  const double a0 = a.mode == 0 ? a.a0 : 0.0, a1 = a.mode == 0 ? a.a1 : 1.0;
+  dgk::Tile tile_n{}, tile_raw{};
+  int rb = 0;  // record buffer of the current tile
+  if (t0 < t1) {
+    tile_n = uniform(a.tiles[t0]);
+    rec_load(tile_n);
+    rec_store(sRall[w][0]);
+    gather(tile_n, sRall[w][0], 0);
+    if (t0 + ts < t1) tile_raw = a.tiles[t0 + ts];
+  }
+  for (int t = t0; t < t1; t += ts, rb ^= 1) {
+    const dgk::Tile tile = tile_n;
+    const bool more = t + ts < t1;
+    if (more) {
+      tile_n = uniform(tile_raw);
+      rec_load(tile_n);
+      if (t + 2 * ts < t1) tile_raw = a.tiles[t + 2 * ts];
+    }
     const int cnt = tile.count;
-    const int ce = c < cnt ? c : cnt - 1;  // clamped: every load stays in the tile
-    const int64_t ge = tile.gbase + ce;
-    const double* ut = a.u + tile.vbase;
-    const int code = (a.codes[ge] >> (5 * g)) & 31;
-    const int32_t nb = a.nbr[(int64_t)g * K + ge];
-    double fa, fb;
-    dgk::face_factors(g, a.bgeo[ge], a.bgeo[K + ge], a.bgeo[2 * K + ge], fa, fb);
-    // Traces of face g, all loads in flight together.
-    const uint8_t* fn = sFN + (g * 24 + code) * NFP;
-    const uint8_t* fm = sFM + g * NFP;
-    double uP[NFP], uM[NFP];
-    // Own traces through LDS, a quarter tile at a time: unit-stride loads of
-    // elements 4 h .. 4 h + 3, then the lanes of those elements read theirs.
-#pragma unroll 1
-    for (int h = 0; h < (STAGE ? 4 : 0); ++h) {
-      const int lim = min(cnt, 4 * h + 4) * NP - 4 * h * NP;
-      wave_sync();  // previous readers of the buffer are done
-#pragma unroll
-      for (int j = 0; j < NVH; ++j) {
-        const int v = lane + 64 * j;
-        if (v < lim) {
-          const int e = v / NP;
-          sT[e * NPH + (v - e * NP)] = ut[4 * h * NP + v];
-        }
-      }
-      wave_sync();
-      if ((ce >> 2) == h) {
-        const double* se = sT + (ce - 4 * h) * NPH;
-#pragma unroll
-        for (int q = 0; q < NFP; ++q) uM[q] = se[fm[q]];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NFP; ++q) {
-      uP[q] = a.u[nb + fn[q]];
-      if constexpr (!STAGE) uM[q] = ut[ce * NP + fm[q]];
-    }
-    // Fscale (n.f(u-) - n.f*), f(u) = u^2/2 (1,1,1), factored; zero past Nfp.
-    auto jump = [&](int q) -> double {
-      if (q >= NFP) return 0.0;
-      const double al = fmax(fabs(uM[q]), fabs(uP[q]));
-      return (uM[q] - uP[q]) * fma(fa, uM[q] + uP[q], -fb * al);
+    const FaceRec* sR = sRall[w][rb];
+    gather(tile, sR, 1);
+    // The tile's state values through buffer resources: loads past the
+    // tile read zero and stores past it are dropped, so partial tiles take
+    // no branches.
+    const int nbytes = cnt * NP * 8;
+    const __amdgpu_buffer_rsrc_t rsR = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.rhs + tile.vbase), 0, nbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsU = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.u + tile.vbase), 0, a.mode == 0 ? nbytes : 0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a0 != 0.0 ? a.u0 + tile.vbase : a.rhs), 0, a0 != 0.0 ? nbytes : 0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsO = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.out + tile.vbase), 0, nbytes, 0x00020000);
+    // Byte offset of the value lane updates for row tile tt, slot j (element
+    // 4 j + lane / 16, row 16 tt + lane % 16); past the tile for rows >= Np.
+    auto upd_off = [&](int tt, int j) -> int {
+      const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
+      return i < NP ? (e * NP + i) * 8 : 0x40000000;
     };
-    // B operand of each step in the product's precision: the jumps (k = 4)
-    // or their bf16 parts (k = 32), split as soon as each is formed.
-    double gj[is_bf(PREC) ? 1 : NPT];
-    bf16x8 bp[is_bf(PREC) ? KSL : 1][PARTS];
-    if constexpr (is_bf(PREC)) {
-#pragma unroll
-      for (int s = 0; s < KSL; ++s)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const double x = jump(8 * s + e);
-          if constexpr (PARTS == 2) {
-            // two parts carry ~16 bits: the FP32 residual is exact enough
-            const float xf = (float)x;
-            const __bf16 h = (__bf16)xf;
-            bp[s][0][e] = h;
-            bp[s][1][e] = (__bf16)(xf - (float)h);
-          } else {
-            const __bf16 h = (__bf16)(float)x;
-            const double r1 = x - (double)(float)h;
-            const __bf16 m = (__bf16)(float)r1;
-            bp[s][0][e] = h;
-            bp[s][1][e] = m;
-            bp[s][2][e] = (__bf16)(float)(r1 - (double)(float)m);
-          }
-        }
-    } else {
-#pragma unroll
-      for (int q = 0; q < NPT; ++q) gj[q] = jump(q);
-    }
-    const double* rt = a.rhs + tile.vbase;
-    const double* u0t = a.a0 != 0.0 ? a.u0 + tile.vbase : nullptr;
-    double* ot = a.out + tile.vbase;
-    // out from preloaded rhs, u, u0 values
-    auto finish = [&](int64_t idx, double lg, double rv, double uv, double u0v) {
-      const double l = rv + lg;
-      double o;
-      if (a.mode == 0) {
-        o = a.a1 * fma(dt, l, uv);
-        if (a.a0 != 0.0) o = fma(a.a0, u0v, o);
-        lmax = fmax(lmax, fabs(o));
-      } else {
-        o = l;
-      }
-      __builtin_nontemporal_store(o, ot + idx);
-    };
-    // LIFT g, row tile by row tile (not unrolled: the preloads of every row
-    // tile would otherwise be hoisted together).
-#pragma unroll 1
-    for (int tt = 0; tt < MTL; ++tt) {
-      // The rows this lane updates after the product (element 4 j + lane / 16,
-      // row 16 tt + lane % 16), loaded ahead of the product.
-      double pr[4], pu[4], p0[4];
-      int64_t pidx[4];
-      bool pok[4];
+    double pr[2][4], pu[2][4], p0[2][4];
+    auto preload = [&](int tt) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
-        pok[j] = e < cnt && i < NP;
-        pidx[j] = pok[j] ? (int64_t)e * NP + i : 0;
-        pr[j] = rt[pidx[j]];
-        pu[j] = a.mode == 0 ? ut[pidx[j]] : 0.0;
-        p0[j] = a.a0 != 0.0 ? __builtin_nontemporal_load(u0t + pidx[j]) : 0.0;
+        const int off = upd_off(tt, j);
+        pr[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsR, off, 0, 0));
+        pu[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsU, off, 0, 0));
+        p0[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs0, off, 0, 2));
       }
+    };
+    // B operands of each face's steps: lane (c, g) takes points 4 s + g
+    // (k = 4) or 32 s + 8 g + e (bf16) of element c.
+    typename std::conditional<PREC == kF64, double, float>::type b[4][is_bf(PREC) ? 1 : KSF];
+    bf16x8 bp[4][is_bf(PREC) ? KSF : 1][PARTS];
+    if constexpr (NPT > NFP) {
+      // columns past Nfp of the jump buffer are zero (the update buffer
+      // overwrote them)
+      wave_sync();
+      for (int i = lane; i < 16 * (NPT - NFP); i += 64) sJ[(NFP + i / 16) * 17 + (i & 15)] = (JT)0;
+    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      // Fscale (n.f(u-) - n.f*), f(u) = u^2/2 (1,1,1), factored.
+      if (f > 0 || NPT == NFP) wave_sync();  // previous readers of the buffer are done
+#pragma unroll
+      for (int j = 0; j < NG; ++j) {
+        const FaceRec& r = sR[f * 16 + pc[j]];
+        const double m = uM[f & 1][j], p = uP[f & 1][j];
+        const double x = (m - p) * fma(r.fa, m + p, -r.fb * fmax(fabs(m), fabs(p)));
+        if (pv[j]) sJ[pq[j] * 17 + pc[j]] = (JT)x;
+      }
+      if (f < 2) gather(tile, sR, f + 2);
+      else if (f == 2) preload(0);
+      wave_sync();
+      if constexpr (!is_bf(PREC)) {
+#pragma unroll
+        for (int s = 0; s < KSF; ++s) b[f][s] = sJ[(4 * s + g) * 17 + c];
+      } else {
+#pragma unroll
+        for (int s = 0; s < KSF; ++s)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const JT x = sJ[(32 * s + 8 * g + e) * 17 + c];
+            if constexpr (PARTS == 2) {
+              // two parts carry ~16 bits: the FP32 value is exact enough
+              const __bf16 h = (__bf16)x;
+              bp[f][s][0][e] = h;
+              bp[f][s][1][e] = (__bf16)(x - (float)h);
+            } else {
+              const __bf16 h = (__bf16)(float)x;
+              const double r1 = x - (double)(float)h;
+              const __bf16 m = (__bf16)(float)r1;
+              bp[f][s][0][e] = h;
+              bp[f][s][1][e] = m;
+              bp[f][s][2][e] = (__bf16)(float)(r1 - (double)(float)m);
+            }
+          }
+      }
+    }
+    // The next tile: face records, then its first face in flight.
+    if (more) {
+      FaceRec* sRn = sRall[w][rb ^ 1];
+      rec_store(sRn);
+      gather(tile_n, sRn, 0);
+    }
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt) {
+      if (tt + 1 < MTL) preload(tt + 1);
       AccT<PREC> acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = 0;
       __builtin_amdgcn_s_setprio(1);  // matrix phase first (wave priority)
 #pragma unroll
-      for (int s = 0; s < KSL; ++s) {
-        const size_t q = ((size_t)tt * KSL + s) * 64 + lane;
-        if constexpr (PREC == kF64) {
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(((const double*)sA)[q], gj[s], acc, 0, 0, 0);
-        } else if constexpr (PREC == kF32) {
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((const float*)sA)[q], (float)gj[s], acc, 0, 0,
-                                                     0);
-        } else {
-          const bf16x8* F = (const bf16x8*)sA;
-          if constexpr (PREC == kBF3) {
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[s][0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][1], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][0], acc, 0, 0, 0);
+      for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int s = 0; s < KSF; ++s) {
+          const size_t q = (((size_t)tt * 4 + f) * KSF + s) * 64 + lane;
+          if constexpr (PREC == kF64) {
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(((const double*)sA)[q], b[f][s], acc, 0, 0, 0);
+          } else if constexpr (PREC == kF32) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((const float*)sA)[q], b[f][s], acc, 0, 0, 0);
           } else {
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[2 * C::NFRAG + q], bp[s][0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][2], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[s][1], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[s][0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][1], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[s][0], acc, 0, 0, 0);
+            const bf16x8* F = (const bf16x8*)sA;
+            if constexpr (PREC == kBF3) {
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[f][s][0], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[f][s][1], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[f][s][0], acc, 0, 0, 0);
+            } else {
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[2 * C::NFRAG + q], bp[f][s][0], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[f][s][2], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[f][s][1], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[C::NFRAG + q], bp[f][s][0], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[f][s][1], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[q], bp[f][s][0], acc, 0, 0, 0);
+            }
           }
         }
-      }
       __builtin_amdgcn_s_setprio(0);
       // Through the wave's 16 x 17 buffer to rows of 16 consecutive values.
       wave_sync();  // previous readers of the buffer are done
 #pragma unroll
       for (int r = 0; r < 4; ++r) sT[c * 17 + acc_row<PREC>(g, r)] = (double)acc[r];
       wave_sync();
+      // out = a0 u0 + a1 (u + dt (rhs + LIFT g))
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (pok[j]) finish(pidx[j], sT[(4 * j + (lane >> 4)) * 17 + (lane & 15)], pr[j], pu[j], p0[j]);
+      for (int j = 0; j < 4; ++j) {
+        const double l = pr[tt & 1][j] + sT[(4 * j + (lane >> 4)) * 17 + (lane & 15)];
+        const double o = fma(a0, p0[tt & 1][j], a1 * fma(dt, l, pu[tt & 1][j]));
+        const bool ok = 4 * j + (lane >> 4) < cnt && 16 * tt + (lane & 15) < NP;
+        lmax = fmax(lmax, ok ? fabs(o) : 0.0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, o), rsO, upd_off(tt, j), 0, 2);
+      }
     }
   }
   if (a.maxabs) {
@@ -317,43 +405,46 @@ float bf16_to_float(uint16_t b) {
 }
 
 // LIFT fragments in the MFMA A-operand lane order of the precision, columns
-// taken face by face: lane (row, g) of step s holds LIFT[row][g Nfp + s]
-// (k = 4) or LIFT[row][g Nfp + 8 s + e], e = 0..7 (bf16, zero past Nfp).
+// taken face by face: lane (row, k) of step s of face f holds
+// LIFT[row][f Nfp + 4 s + k] (k = 4) or LIFT[row][f Nfp + 32 s + 8 k + e],
+// e = 0..7 (bf16); zero past Nfp.
 std::vector<unsigned char> lift_fragments(const std::vector<double>& LIFT, int N, int prec) {
   const int NP = dgk::np_of(N), NFP = dgk::nfp_of(N), NF = 4 * NFP;
   const int MTL = (NP + 15) / 16;
-  const int KSL = is_bf(prec) ? (NFP + 7) / 8 : NFP, PARTS = parts_of(prec);
-  const size_t nfrag = (size_t)MTL * KSL * 64;
-  auto val = [&](int i, int g, int q) {
-    return (i < NP && q < NFP) ? LIFT[(size_t)i * NF + g * NFP + q] : 0.0;
+  const int KB = is_bf(prec) ? 32 : 4, KSF = (NFP + KB - 1) / KB, PARTS = parts_of(prec);
+  const size_t nfrag = (size_t)MTL * 4 * KSF * 64;
+  auto val = [&](int i, int f, int q) {
+    return (i < NP && q < NFP) ? LIFT[(size_t)i * NF + f * NFP + q] : 0.0;
   };
   std::vector<unsigned char> buf;
   if (!is_bf(prec)) {
     buf.resize(nfrag * (prec == kF64 ? 8 : 4));
     for (int tt = 0; tt < MTL; ++tt)
-      for (int s = 0; s < KSL; ++s)
-        for (int l = 0; l < 64; ++l) {
-          const double v = val(16 * tt + (l & 15), l >> 4, s);
-          const size_t q = ((size_t)tt * KSL + s) * 64 + l;
-          if (prec == kF64) ((double*)buf.data())[q] = v;
-          else ((float*)buf.data())[q] = (float)v;
-        }
+      for (int f = 0; f < 4; ++f)
+        for (int s = 0; s < KSF; ++s)
+          for (int l = 0; l < 64; ++l) {
+            const double v = val(16 * tt + (l & 15), f, 4 * s + (l >> 4));
+            const size_t q = (((size_t)tt * 4 + f) * KSF + s) * 64 + l;
+            if (prec == kF64) ((double*)buf.data())[q] = v;
+            else ((float*)buf.data())[q] = (float)v;
+          }
   } else {
     buf.resize(nfrag * 16 * PARTS);
     uint16_t* P0 = (uint16_t*)buf.data();
     for (int tt = 0; tt < MTL; ++tt)
-      for (int s = 0; s < KSL; ++s)
-        for (int l = 0; l < 64; ++l)
-          for (int e = 0; e < 8; ++e) {
-            const double v = val(16 * tt + (l & 15), l >> 4, 8 * s + e);
-            const size_t q = (((size_t)tt * KSL + s) * 64 + l) * 8 + e;
-            const uint16_t h = bf16_bits((float)v);
-            const double r1 = v - (double)bf16_to_float(h);
-            const uint16_t m = bf16_bits((float)r1);
-            P0[q] = h;
-            P0[nfrag * 8 + q] = m;
-            if (PARTS == 3) P0[2 * nfrag * 8 + q] = bf16_bits((float)(r1 - (double)bf16_to_float(m)));
-          }
+      for (int f = 0; f < 4; ++f)
+        for (int s = 0; s < KSF; ++s)
+          for (int l = 0; l < 64; ++l)
+            for (int e = 0; e < 8; ++e) {
+              const double v = val(16 * tt + (l & 15), f, 32 * s + 8 * (l >> 4) + e);
+              const size_t q = ((((size_t)tt * 4 + f) * KSF + s) * 64 + l) * 8 + e;
+              const uint16_t h = bf16_bits((float)v);
+              const double r1 = v - (double)bf16_to_float(h);
+              const uint16_t m = bf16_bits((float)r1);
+              P0[q] = h;
+              P0[nfrag * 8 + q] = m;
+              if (PARTS == 3) P0[2 * nfrag * 8 + q] = bf16_bits((float)(r1 - (double)bf16_to_float(m)));
+            }
   }
   return buf;
 }
@@ -380,13 +471,18 @@ int surface_launch(dgk_dgmesh m, const SurfArgs& a) {
   return DGK_OK;
 }
 
+template <int N, int PREC>
+int surface_w(dgk_dgmesh m, const SurfArgs* a) {
+  return a ? surface_launch<N, PREC>(m, *a) : surface_blocks<N, PREC>(m);
+}
+
 template <int N>
 int surface_n(dgk_dgmesh m, const SurfArgs* a) {
   switch (m->lift_prec) {
-    case kF64: return a ? surface_launch<N, kF64>(m, *a) : surface_blocks<N, kF64>(m);
-    case kF32: return a ? surface_launch<N, kF32>(m, *a) : surface_blocks<N, kF32>(m);
-    case kBF3: return a ? surface_launch<N, kBF3>(m, *a) : surface_blocks<N, kBF3>(m);
-    default: return a ? surface_launch<N, kBF6>(m, *a) : surface_blocks<N, kBF6>(m);
+    case kF64: return surface_w<N, kF64>(m, a);
+    case kF32: return surface_w<N, kF32>(m, a);
+    case kBF3: return surface_w<N, kBF3>(m, a);
+    default: return surface_w<N, kBF6>(m, a);
   }
 }
 

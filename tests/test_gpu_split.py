@@ -182,3 +182,4 @@ def test_split_graph_replay(device):
         b.step(2, graph=True)
         device.sync()
         assert np.array_equal(a.state(), b.state()), f"replay {rep}"
+
