@@ -48,6 +48,7 @@ using dgk::Tile;
 typedef const __attribute__((address_space(4))) double* cdouble_ptr;
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kH = 64;  // hidden width of the sensor network
 
@@ -284,13 +285,16 @@ struct StageAv {
 //   exactly the B layout of this product, so q feeds it from registers),
 //   and the face traces sig[f][fp] = eps n_f . q at own face points (q goes
 //   through LDS one component at a time).
-// av_surf_kernel: lane (c, f) gathers face f's own and neighbour traces of u
-//   and sig (all loads in flight together), forms
+// av_surf_kernel: lanes along the (element, point) pairs of one face at a
+//   time gather the own and neighbour traces of u and sig (two faces in
+//   flight; the next tile's first face during this tile's products) and form
 //     -Fscale ((sig- + sig+)/2 + tau (u- - u+))  [+ LF jump (BURG)]
 //   (sig+ is the neighbour's own eps n'.q with n' = -n, so the central flux
 //   n.(eps q)- - n.(eps q)+ is sig- + sig+; a boundary face, its own
-//   neighbour, has zero jump), LIFT on matrix cores per 16-row tile, and the
-//   RK update through a 16 x 17 LDS buffer with row-contiguous accesses.
+//   neighbour, has zero jump) into LDS, from which each lane reads its B
+//   operands; then LIFT on matrix cores per 16-row tile (columns face by
+//   face) and the RK update through a 16 x 17 LDS buffer with row-contiguous
+//   accesses.
 // ---------------------------------------------------------------------------
 template <int N, int WPB, bool BURG>
 __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
@@ -479,18 +483,30 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
   }
 }
 
+// Per-wave face record of the surface kernel: value offset of the
+// neighbour's node 0, row of the orientation tables, rows of the own and
+// neighbour face traces in sig, the boundary flag, Fscale, the penalty and
+// the Burgers flux factors.
+struct AvFaceRec {
+  int32_t nb, fnrow, mrow, prow, bnd, pad;
+  double fs, tau, fa, fb;
+};
+
 template <int N, int WPB, int MODE, bool BURG>
-__global__ void __launch_bounds__(64 * WPB) av_surf_kernel(AvArgs a, StageAv o) {
+__global__ void __launch_bounds__(64 * WPB, 2) av_surf_kernel(AvArgs a, StageAv o) {
   using SH = AvShape<N>;
-  constexpr int NP = SH::NP, NFP = SH::NFP, NF = SH::NF, MTL = SH::MTL, KSL = SH::KSL;
+  constexpr int NP = SH::NP, NFP = SH::NFP, MTL = SH::MTL, KSF = SH::SPF, NPT = 4 * KSF;
+  constexpr int NG = (16 * NFP + 63) / 64;  // gather instructions per face
   constexpr int NT = 64 * WPB;
   constexpr FmaskTable<N> FM{};
-  __shared__ __attribute__((aligned(16))) double sA[MTL * KSL * 64];
-  __shared__ double sBuf[WPB][16 * 17];
+  constexpr int BUF = NPT * 17 > 16 * 17 ? NPT * 17 : 16 * 17;
+  __shared__ __attribute__((aligned(16))) double sA[MTL * 4 * KSF * 64];
+  __shared__ double sBuf[WPB][BUF];
+  __shared__ AvFaceRec sRall[WPB][2][64];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFP[4 * 24 * NFP];  // neighbour's face point of each (face, code, point)
-  __shared__ uint8_t sFM[NF];
-  for (int i = threadIdx.x; i < NF; i += NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
+  __shared__ uint8_t sFM[4 * NFP];
+  for (int i = threadIdx.x; i < 4 * NFP; i += NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
   for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) {
     const int nn = a.face_nodes[i];
     sFN[i] = (uint8_t)nn;
@@ -500,18 +516,32 @@ __global__ void __launch_bounds__(64 * WPB) av_surf_kernel(AvArgs a, StageAv o) 
       if (FM.v[f2][q] == nn) fp2 = q;
     sFP[i] = (uint8_t)fp2;
   }
-  for (int q = threadIdx.x; q < MTL * KSL * 64; q += NT) {
-    const int l = q & 63, ts = q >> 6, cl = l & 15, gl = l >> 4;
-    const int tt = ts / KSL, s = ts % KSL, row = 16 * tt + cl;
-    sA[q] = row < NP ? a.LIFT[row * NF + gl * NFP + s] : 0.0;
+  // LIFT fragments, columns face by face: lane (row, k) of step s of face f
+  // holds LIFT[row][f Nfp + 4 s + k] (zero past Nfp).
+  for (int q = threadIdx.x; q < MTL * 4 * KSF * 64; q += NT) {
+    const int l = q & 63, ts = q >> 6, cl = l & 15, kl = l >> 4;
+    const int tt = ts / (4 * KSF), f = (ts / KSF) % 4, s = ts % KSF;
+    const int row = 16 * tt + cl, fp = 4 * s + kl;
+    sA[q] = row < NP && fp < NFP ? a.LIFT[row * 4 * NFP + f * NFP + fp] : 0.0;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int c = lane & 15, g = lane >> 4;  // element of the tile, face
+  // wave index as a scalar, so that tile indices and records stay in SGPRs
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15, g = lane >> 4;  // element of the tile, face (or k group)
   double* sB = sBuf[w];
   const int64_t K = a.K;
-  const double dt = MODE == 2 ? 0.0 : *o.dt;
   double lmax = 0.0;
+  // Gather pair of instruction j: element pc[j] of the tile, point pq[j].
+  int pc[NG], pq[NG];
+  bool pv[NG];
+#pragma unroll
+  for (int j = 0; j < NG; ++j) {
+    const int p = lane + 64 * j;
+    pv[j] = p < 16 * NFP;
+    const int pp = pv[j] ? p : 16 * NFP - 1;
+    pc[j] = pp / NFP;
+    pq[j] = pp - pc[j] * NFP;
+  }
   // XCD-contiguous tile ranges (see the split surface kernel).
   int t0 = blockIdx.x * WPB + w, t1 = a.ntiles, ts = gridDim.x * WPB;
   if (gridDim.x % 8 == 0) {
@@ -520,82 +550,179 @@ __global__ void __launch_bounds__(64 * WPB) av_surf_kernel(AvArgs a, StageAv o) 
     t1 = (int)((int64_t)a.ntiles * (x + 1) / 8);
     ts = (gridDim.x / 8) * WPB;
   }
-  for (int t = t0; t < t1; t += ts) {
-    const Tile tl = a.tiles[t];
-    const int cnt = tl.count;
-    const int ce = c < cnt ? c : cnt - 1;
-    const int64_t ge = (int64_t)tl.gbase + ce;
+  // Tile records are read one tile ahead and made wave-uniform only when
+  // used, so waiting for one never drains the loads issued after it.
+  auto uniform = [](const Tile& v) -> Tile {
+    Tile u;
+    u.vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v.vbase >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)v.vbase));
+    u.gbase = __builtin_amdgcn_readfirstlane(v.gbase);
+    u.count = __builtin_amdgcn_readfirstlane(v.count);
+    return u;
+  };
+  // Face-record inputs of lane (c, g) for a tile (element clamped to the tile).
+  int32_t rnb = 0, rne = 0, rcode = 0, rge = 0;
+  double ree = 0.0, rep = 0.0, rfs = 0.0, rga = 0.0, rgb = 0.0, rgc = 0.0;
+  auto rec_load = [&](const Tile& tl) {
+    const int ce = c < tl.count ? c : tl.count - 1;
+    const int64_t ge = tl.gbase + ce;
+    rge = (int32_t)ge;
+    rnb = a.nbr[(int64_t)g * K + ge];
+    rne = a.nbe[(int64_t)g * K + ge];
+    rcode = a.codes[ge];
+    ree = a.eps_in[ge];
+    rfs = a.geo[(13 + 4 * g) * K + ge];
+    if (BURG) {
+      rga = o.bgeo[ge];
+      rgb = o.bgeo[K + ge];
+      rgc = o.bgeo[2 * K + ge];
+    }
+  };
+  auto rec_store = [&](AvFaceRec* sR) {
+    const int code = (rcode >> (5 * g)) & 31, f2 = code / 6;
+    AvFaceRec r;
+    r.nb = rnb;
+    r.fnrow = (g * 24 + code) * NFP;
+    r.mrow = rge * 4 + g;
+    r.prow = rne * 4 + f2;
+    r.bnd = rne == rge && f2 == g;  // boundary face: its own neighbour
+    r.pad = 0;
+    r.fs = rfs;
+    r.tau = a.tau0 * fmax(ree, rep);
+    r.fa = 0.0;
+    r.fb = 0.0;
+    if (BURG) dgk::face_factors(g, rga, rgb, rgc, r.fa, r.fb);
+    sR[g * 16 + c] = r;
+    wave_fence();
+  };
+  // Own and neighbour traces of u and sig of face f at the pairs of this
+  // lane: two faces in flight, face f in slot f % 2.
+  double uM[2][NG], uP[2][NG], sM[2][NG], sP[2][NG];
+  auto gather = [&](const Tile& tl, const AvFaceRec* sR, int f) {
     const double* ut = a.u + tl.vbase;
-    const int code = (a.codes[ge] >> (5 * g)) & 31, f2 = code / 6;
-    const int32_t nb = a.nbr[(int64_t)g * K + ge];
-    const int32_t ne = a.nbe[(int64_t)g * K + ge];
-    const bool bnd = ne == ge && f2 == g;  // boundary face: its own neighbour
-    const double ee = a.eps_in[ge], ep = a.eps_in[ne];
-    const double tau = a.tau0 * fmax(ee, ep);
-    const double fs = a.geo[(13 + 4 * g) * K + ge];
-    double fa = 0.0, fb = 0.0;
-    if (BURG) dgk::face_factors(g, o.bgeo[ge], o.bgeo[K + ge], o.bgeo[2 * K + ge], fa, fb);
-    const uint8_t* fn = sFN + (g * 24 + code) * NFP;
-    const uint8_t* fq = sFP + (g * 24 + code) * NFP;
-    const uint8_t* fm = sFM + g * NFP;
-    const double* sgM = a.sig + (ge * 4 + g) * NFP;
-    const double* sgP = a.sig + ((int64_t)ne * 4 + f2) * NFP;
-    double uM[NFP], uP[NFP], sM[NFP], sP[NFP];
 #pragma unroll
-    for (int q = 0; q < NFP; ++q) {
-      uM[q] = ut[ce * NP + fm[q]];
-      uP[q] = a.u[nb + fn[q]];
-      sM[q] = sgM[q];
-      sP[q] = sgP[fq[q]];
+    for (int j = 0; j < NG; ++j) {
+      const int ce = pc[j] < tl.count ? pc[j] : tl.count - 1;
+      const AvFaceRec& r = sR[f * 16 + pc[j]];
+      uP[f & 1][j] = a.u[r.nb + sFN[r.fnrow + pq[j]]];
+      uM[f & 1][j] = ut[ce * NP + sFM[f * NFP + pq[j]]];
+      sM[f & 1][j] = a.sig[(int64_t)r.mrow * NFP + pq[j]];
+      sP[f & 1][j] = a.sig[(int64_t)r.prow * NFP + sFP[r.fnrow + pq[j]]];
     }
-    double bj[NFP];
-#pragma unroll
-    for (int q = 0; q < NFP; ++q) {
-      const double sn = bnd ? -sM[q] : sP[q];
-      double b = -fs * (0.5 * (sM[q] + sn) + tau * (uM[q] - uP[q]));
-      if (BURG)
-        b = fma(uM[q] - uP[q], fma(fa, uM[q] + uP[q], -fb * fmax(fabs(uM[q]), fabs(uP[q]))), b);
-      bj[q] = b;
+  };
+  // MODE 2 (R) is the update with a0 = 0, a1 = dt = 1 and u = 0
+  const double dt = MODE == 2 ? 1.0 : *o.dt;
+  const double a0 = MODE == 0 ? o.a0 : 0.0, a1 = MODE == 2 ? 1.0 : o.a1;
+  Tile tile_n{}, tile_raw{};
+  int rb = 0;  // record buffer of the current tile
+  if (t0 < t1) {
+    tile_n = uniform(a.tiles[t0]);
+    rec_load(tile_n);
+    rep = a.eps_in[rne];
+    rec_store(sRall[w][0]);
+    gather(tile_n, sRall[w][0], 0);
+    if (t0 + ts < t1) tile_raw = a.tiles[t0 + ts];
+  }
+  for (int t = t0; t < t1; t += ts, rb ^= 1) {
+    const Tile tile = tile_n;
+    const bool more = t + ts < t1;
+    if (more) {
+      tile_n = uniform(tile_raw);
+      rec_load(tile_n);
+      if (t + 2 * ts < t1) tile_raw = a.tiles[t + 2 * ts];
     }
-    // LIFT per 16-row tile; the rows this lane updates are loaded first.
-#pragma unroll 1
-    for (int tt = 0; tt < MTL; ++tt) {
-      double pv[4], pu[4], p0[4];
-      int64_t pidx[4];
-      bool pok[4];
+    const int cnt = tile.count;
+    const AvFaceRec* sR = sRall[w][rb];
+    gather(tile, sR, 1);
+    // The tile's state values through buffer resources: loads past the
+    // tile read zero and stores past it are dropped, so partial tiles take
+    // no branches.
+    const int nbytes = cnt * NP * 8;
+    const __amdgpu_buffer_rsrc_t rsV = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.vol + tile.vbase), 0, nbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsU = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.u + tile.vbase), 0, MODE == 2 ? 0 : nbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(MODE == 0 ? o.u0 + tile.vbase : a.vol), 0, MODE == 0 ? nbytes : 0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsO = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(o.out + tile.vbase), 0, nbytes, 0x00020000);
+    // Byte offset of the value lane updates for row tile tt, slot j (element
+    // 4 j + lane / 16, row 16 tt + lane % 16); past the tile for rows >= Np.
+    auto upd_off = [&](int tt, int j) -> int {
+      const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
+      return i < NP ? (e * NP + i) * 8 : 0x40000000;
+    };
+    double pv_[2][4], pu[2][4], p0[2][4];
+    auto preload = [&](int tt) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
-        pok[j] = e < cnt && i < NP;
-        pidx[j] = tl.vbase + (pok[j] ? (int64_t)e * NP + i : 0);
-        pv[j] = a.vol[pidx[j]];
-        pu[j] = MODE == 2 ? 0.0 : a.u[pidx[j]];
-        p0[j] = MODE == 0 ? __builtin_nontemporal_load(o.u0 + pidx[j]) : 0.0;
+        const int off = upd_off(tt, j);
+        pv_[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsV, off, 0, 0));
+        pu[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsU, off, 0, 0));
+        p0[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs0, off, 0, 2));
       }
+    };
+    // B operands of each face's steps: lane (c, g) takes point 4 s + g of
+    // element c.
+    double b[4][KSF];
+    if constexpr (NPT > NFP) {
+      // columns past Nfp of the jump buffer are zero (the update buffer
+      // overwrote them)
+      wave_fence();
+      for (int i = lane; i < 16 * (NPT - NFP); i += 64) sB[(NFP + i / 16) * 17 + (i & 15)] = 0.0;
+    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      if (f > 0 || NPT == NFP) wave_fence();  // previous readers of the buffer are done
+      // -Fscale ((sig- + sig+)/2 + tau (u- - u+))  [+ Fscale (n.f(u-) - n.f*)]
+#pragma unroll
+      for (int j = 0; j < NG; ++j) {
+        const AvFaceRec& r = sR[f * 16 + pc[j]];
+        const double m = uM[f & 1][j], p = uP[f & 1][j];
+        const double sm = sM[f & 1][j], sn = r.bnd ? -sm : sP[f & 1][j];
+        double x = -r.fs * (0.5 * (sm + sn) + r.tau * (m - p));
+        if (BURG) x = fma(m - p, fma(r.fa, m + p, -r.fb * fmax(fabs(m), fabs(p))), x);
+        if (pv[j]) sB[pq[j] * 17 + pc[j]] = x;
+      }
+      // the next tile's neighbour eps (its index arrived with the records)
+      if (f == 0 && more) rep = a.eps_in[rne];
+      if (f < 2) gather(tile, sR, f + 2);
+      else if (f == 2) preload(0);
+      wave_fence();
+#pragma unroll
+      for (int s = 0; s < KSF; ++s) b[f][s] = sB[(4 * s + g) * 17 + c];
+    }
+    // The next tile: face records, then its first face in flight.
+    if (more) {
+      AvFaceRec* sRn = sRall[w][rb ^ 1];
+      rec_store(sRn);
+      gather(tile_n, sRn, 0);
+    }
+#pragma unroll
+    for (int tt = 0; tt < MTL; ++tt) {
+      if (tt + 1 < MTL) preload(tt + 1);
       double4v acc = (double4v){0.0, 0.0, 0.0, 0.0};
       __builtin_amdgcn_s_setprio(1);  // matrix phase first (wave priority)
 #pragma unroll
-      for (int s = 0; s < KSL; ++s)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[(tt * KSL + s) * 64 + lane], bj[s], acc, 0,
-                                                   0, 0);
+      for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int s = 0; s < KSF; ++s)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[((tt * 4 + f) * KSF + s) * 64 + lane], b[f][s],
+                                                     acc, 0, 0, 0);
       __builtin_amdgcn_s_setprio(0);
+      // Through the wave's 16 x 17 buffer to rows of 16 consecutive values.
       wave_fence();  // previous readers of the buffer are done
 #pragma unroll
       for (int r = 0; r < 4; ++r) sB[c * 17 + g + 4 * r] = acc[r];
       wave_fence();
+      // out = a0 u0 + a1 (u + dt (vol + LIFT g))
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (!pok[j]) continue;
-        const double r = pv[j] + sB[(4 * j + (lane >> 4)) * 17 + (lane & 15)];
-        double v;
-        if (MODE == 2) {
-          v = r;
-        } else {
-          v = o.a1 * fma(dt, r, pu[j]);
-          if (MODE == 0) v = fma(o.a0, p0[j], v);
-          lmax = fmax(lmax, fabs(v));
-        }
-        o.out[pidx[j]] = v;
+        const double l = pv_[tt & 1][j] + sB[(4 * j + (lane >> 4)) * 17 + (lane & 15)];
+        const double v = fma(a0, p0[tt & 1][j], a1 * fma(dt, l, pu[tt & 1][j]));
+        const bool ok = 4 * j + (lane >> 4) < cnt && 16 * tt + (lane & 15) < NP;
+        lmax = fmax(lmax, ok ? fabs(v) : 0.0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rsO, upd_off(tt, j), 0, 0);
       }
     }
   }
@@ -1073,11 +1200,10 @@ int groups_grid(dgk_dgmesh m, int per_cu) {
 // orders use 4 waves and two blocks per CU.
 template <int N>
 constexpr int av_wpb() { return N >= 4 ? 8 : 4; }
-// Waves per block of the surface kernel. Blocks of four waves, three per CU
-// (three waves per SIMD, the register limit), ran the P=4 stage 8-9 % faster
-// than one block of twelve, although each block then holds its own copy of
-// the LIFT fragments in LDS.
-constexpr int kSurfW = 4;
+// Waves per block of the surface kernel: one block of eight per CU, two per
+// SIMD (the kernel keeps its own loads in flight a face and a row tile
+// ahead, which takes the registers of two waves per SIMD).
+constexpr int kSurfW = 8;
 
 int av_blocks(dgk_dgmesh m) {
   const int wpb = m->N >= 4 ? 8 : 4;
