@@ -127,6 +127,16 @@ __device__ __forceinline__ Tile group_tile(const Tile* tiles, int ntiles, int gr
   return t;
 }
 
+// A tile record loaded by every lane, in scalar registers (it is wave-uniform).
+__device__ __forceinline__ Tile tile_uniform(const Tile& v) {
+  Tile u;
+  u.vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v.vbase >> 32)) << 32) |
+                      (uint32_t)__builtin_amdgcn_readfirstlane((int)v.vbase));
+  u.gbase = __builtin_amdgcn_readfirstlane(v.gbase);
+  u.count = __builtin_amdgcn_readfirstlane(v.count);
+  return u;
+}
+
 // Stage the nodal values of the wave's four tiles of src into LDS rows of S
 // doubles (row 16 q + e for element e of tile q), unit-stride loads.
 template <int NP, int S>
@@ -161,69 +171,6 @@ __device__ __forceinline__ void unstage4(double* __restrict__ dst, const Tile& m
 }
 
 // ---------------------------------------------------------------------------
-// Persson-Peraire sensor, FP64, one element per lane.
-// ---------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(64) pp_sensor_kernel(AvArgs a) {
-  constexpr int NP = Tet<N>::Np, S = NP + 1;
-  constexpr ModeDeg<N> MD{};
-  __shared__ double sU[64 * S];
-  const int lane = threadIdx.x;
-  double emax = 0.0;
-  for (int grp = blockIdx.x; 4 * grp < a.ntiles; grp += gridDim.x) {
-    const Tile my = group_tile(a.tiles, a.ntiles, grp, lane);
-    wave_fence();
-    stage4<NP, S>(a.u, my, lane, sU);
-    wave_fence();
-    if ((lane & 15) < my.count) {
-      const double* su = sU + lane * S;
-      // zero offset that depends on the tile, so the operator loads stay in the loop
-      const int64_t z = (int64_t)(__builtin_amdgcn_readfirstlane(grp) >> 30);
-      const cdouble_ptr Vi = (cdouble_ptr)(a.Vinv + z);
-      double uk[NP];
-      double umax = 0.0;
-#pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        uk[j] = su[j];
-        umax = fmax(umax, fabs(uk[j]));
-      }
-      double uh[NP];
-      double tot = 0.0, top = 0.0;
-#pragma unroll
-      for (int i = 0; i < NP; ++i) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < NP; ++j) s = fma(Vi[i * NP + j], uk[j], s);
-        uh[i] = s;
-        tot = fma(s, s, tot);
-        if (MD.d[i] == N) top = fma(s, s, top);
-      }
-      const double Sv = tot > 0.0 ? top / tot : 0.0;
-      const double fr = pp_fraction(Sv, a.s0, a.kappa);
-      const int64_t g = (int64_t)my.gbase + (lane & 15);
-      const double vol = (4.0 / 3.0) * a.geo[9 * a.K + g];
-      const double h = cbrt(6.0 * vol);
-      const double e0 = a.ceps * h * sqrt(3.0) * umax / N;
-      const double e = e0 * fr;
-      a.eps[g] = e;
-      emax = fmax(emax, e);
-      if (a.frac) a.frac[g] = fr;
-      if (a.frac32) a.frac32[g] = (float)fr;
-      if (a.feats) {
-        const double inv = tot > 0.0 ? 1.0 / sqrt(tot) : 0.0;
-#pragma unroll
-        for (int i = 0; i < NP; ++i) a.feats[g * NP + i] = (float)(uh[i] * inv);
-      }
-    }
-  }
-  if (a.maxeps) {
-    emax = wmax(emax);
-    if (lane == 0 && emax > 0.0)
-      atomicMax((unsigned long long*)a.maxeps, (unsigned long long)__double_as_longlong(emax));
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Matrix-core kernels of the viscous stage (FP64, v_mfma_f64_16x16x4_f64).
 // Each wave takes 16-element tiles (the MFMA N dimension); the operator
 // fragments sit in LDS for the block. FP64 MFMA layout (checked on gfx950):
@@ -239,6 +186,133 @@ __global__ void __launch_bounds__(64) pp_sensor_kernel(AvArgs a) {
 // [S_r; S_s; S_t]; the surface term one LIFT product over face-per-lane k.
 // ---------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Persson-Peraire sensor on FP64 matrix cores. Per 16-element tile each wave
+// forms the modal coefficients uh = V^-1 u with v_mfma_f64_16x16x4_f64 (the
+// tile's elements are the N dimension; the V^-1 fragments sit in LDS for the
+// block). Lane (c, g) then holds modes 16 tt + g + 4 r of element c, so the
+// mode energies and max|u| of an element reduce over its four lanes. The
+// nodal values and volume of the next tile are loaded during the product.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256) pp_sensor_kernel(AvArgs a) {
+  constexpr int NP = Tet<N>::Np, MT = (NP + 15) / 16, KS = (NP + 3) / 4;
+  constexpr ModeDeg<N> MD{};
+  __shared__ __attribute__((aligned(16))) double sV[MT * KS * 64];
+  for (int q = threadIdx.x; q < MT * KS * 64; q += 256) {
+    const int l = q & 63, ts = q >> 6, tt = ts / KS, s = ts % KS;
+    const int row = 16 * tt + (l & 15), k = 4 * s + (l >> 4);
+    sV[q] = row < NP && k < NP ? a.Vinv[row * NP + k] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15, g = lane >> 4;
+  // modes of top degree among the rows this lane holds (bit 4 tt + r)
+  uint32_t topm = 0;
+#pragma unroll
+  for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * tt + g + 4 * r;
+      if (row < NP && MD.d[row] == N) topm |= 1u << (4 * tt + r);
+    }
+  // The nodal values of tile d with unit-stride loads (staged in LDS for the
+  // B operands) and the volume factor of element c, a tile ahead.
+  constexpr int PV = (16 * NP + 63) / 64, S = NP + 1;
+  __shared__ double sUall[4][16 * S];
+  double* sU = sUall[w];
+  double pre[PV], jn = 0.0;
+  auto issue = [&](const Tile& d) {
+#pragma unroll
+    for (int j = 0; j < PV; ++j) pre[j] = a.u[d.vbase + min(lane + 64 * j, d.count * NP - 1)];
+    jn = a.geo[9 * a.K + d.gbase + (c < d.count ? c : d.count - 1)];
+  };
+  const int t0 = blockIdx.x * 4 + w, ts = gridDim.x * 4, t1 = a.ntiles;
+  double emax = 0.0;
+  Tile tile_n{}, tile_raw{};
+  if (t0 < t1) {
+    tile_n = tile_uniform(a.tiles[t0]);
+    issue(tile_n);
+    if (t0 + ts < t1) tile_raw = a.tiles[t0 + ts];
+  }
+  for (int t = t0; t < t1; t += ts) {
+    const Tile tile = tile_n;
+    const double jv = jn;
+    wave_fence();  // previous readers of the tile buffer are done
+#pragma unroll
+    for (int j = 0; j < PV; ++j) {
+      const int i = lane + 64 * j;
+      if (i < 16 * NP) {
+        const int e = i / NP, nn = i - e * NP;
+        sU[e * S + nn] = i < tile.count * NP ? pre[j] : 0.0;
+      }
+    }
+    if (t + ts < t1) {
+      tile_n = tile_uniform(tile_raw);
+      issue(tile_n);
+      if (t + 2 * ts < t1) tile_raw = a.tiles[t + 2 * ts];
+    }
+    wave_fence();
+    // B operand of step s: node 4 s + g of element c (zero past Np)
+    double b[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[s] = 4 * s + g < NP ? sU[c * S + 4 * s + g] : 0.0;
+    double4v acc[MT];
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) acc[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int tt = 0; tt < MT; ++tt)
+        acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sV[(tt * KS + s) * 64 + lane], b[s], acc[tt],
+                                                       0, 0, 0);
+    double umax = 0.0, tot = 0.0, top = 0.0;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) umax = fmax(umax, fabs(b[s]));
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double e2 = acc[tt][r] * acc[tt][r];  // rows past Np are zero
+        tot += e2;
+        if (topm >> (4 * tt + r) & 1u) top += e2;
+      }
+#pragma unroll
+    for (int o = 16; o <= 32; o <<= 1) {
+      umax = fmax(umax, __shfl_xor(umax, o, 64));
+      tot += __shfl_xor(tot, o, 64);
+      top += __shfl_xor(top, o, 64);
+    }
+    const bool own = c < tile.count;
+    const int64_t gi = (int64_t)tile.gbase + c;
+    if (own && g == 0) {
+      const double Sv = tot > 0.0 ? top / tot : 0.0;
+      const double fr = pp_fraction(Sv, a.s0, a.kappa);
+      const double h = cbrt(6.0 * (4.0 / 3.0) * jv);
+      const double e = a.ceps * h * sqrt(3.0) * umax / N * fr;
+      a.eps[gi] = e;
+      emax = fmax(emax, e);
+      if (a.frac) a.frac[gi] = fr;
+      if (a.frac32) a.frac32[gi] = (float)fr;
+    }
+    if (a.feats && own) {
+      const double inv = tot > 0.0 ? 1.0 / sqrt(tot) : 0.0;
+#pragma unroll
+      for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * tt + g + 4 * r;
+          if (row < NP) a.feats[gi * NP + row] = (float)(acc[tt][r] * inv);
+        }
+    }
+  }
+  if (a.maxeps) {
+    emax = wmax(emax);
+    if (lane == 0 && emax > 0.0)
+      atomicMax((unsigned long long*)a.maxeps, (unsigned long long)__double_as_longlong(emax));
+  }
+}
 
 template <int N>
 struct AvShape {
@@ -552,14 +626,6 @@ __global__ void __launch_bounds__(64 * WPB, 2) av_surf_kernel(AvArgs a, StageAv 
   }
   // Tile records are read one tile ahead and made wave-uniform only when
   // used, so waiting for one never drains the loads issued after it.
-  auto uniform = [](const Tile& v) -> Tile {
-    Tile u;
-    u.vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v.vbase >> 32)) << 32) |
-                        (uint32_t)__builtin_amdgcn_readfirstlane((int)v.vbase));
-    u.gbase = __builtin_amdgcn_readfirstlane(v.gbase);
-    u.count = __builtin_amdgcn_readfirstlane(v.count);
-    return u;
-  };
   // Face-record inputs of lane (c, g) for a tile (element clamped to the tile).
   int32_t rnb = 0, rne = 0, rcode = 0, rge = 0;
   double ree = 0.0, rep = 0.0, rfs = 0.0, rga = 0.0, rgb = 0.0, rgc = 0.0;
@@ -616,7 +682,7 @@ __global__ void __launch_bounds__(64 * WPB, 2) av_surf_kernel(AvArgs a, StageAv 
   Tile tile_n{}, tile_raw{};
   int rb = 0;  // record buffer of the current tile
   if (t0 < t1) {
-    tile_n = uniform(a.tiles[t0]);
+    tile_n = tile_uniform(a.tiles[t0]);
     rec_load(tile_n);
     rep = a.eps_in[rne];
     rec_store(sRall[w][0]);
@@ -627,7 +693,7 @@ __global__ void __launch_bounds__(64 * WPB, 2) av_surf_kernel(AvArgs a, StageAv 
     const Tile tile = tile_n;
     const bool more = t + ts < t1;
     if (more) {
-      tile_n = uniform(tile_raw);
+      tile_n = tile_uniform(tile_raw);
       rec_load(tile_n);
       if (t + 2 * ts < t1) tile_raw = a.tiles[t + 2 * ts];
     }
@@ -899,7 +965,8 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
   // V^-1 fragments (BF16 A operand of 16x16x32, hi and lo parts), shared by
   // the block: entry [part][mt][s][lane] holds A[16 mt + row][32 s + 8 g + j].
   __shared__ bf16x8 sV[2][MTM][KSM][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // wave index as a scalar, so that tile indices and records stay in SGPRs
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = lane & 15, g = lane >> 4;
   for (int q = threadIdx.x; q < MTM * KSM * 64; q += 256) {
     const int l = q & 63, ms = q >> 6, mt = ms / KSM, s = ms % KSM;
@@ -920,18 +987,24 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
   mlp_load(P, L, lane, R);
   __syncthreads();
   double emax = 0.0;
-  const int nw = gridDim.x * 4;
-  int t = blockIdx.x * 4 + w;
-  // the wave's first tile, then each next tile's values in flight during the current one
-  Tile tl;
-  double pre[PV];
-  if (t < a.ntiles) {
-    tl = a.tiles[t];
+  // The wave's tiles: each next tile's values and volume factors are in
+  // flight during the current one, its record a tile earlier.
+  const int t0 = blockIdx.x * 4 + w, ts = gridDim.x * 4, t1 = a.ntiles;
+  Tile tl{}, tile_raw{};
+  double pre[PV], jn = 0.0;
+  auto issue = [&](const Tile& d) {
 #pragma unroll
-    for (int j = 0; j < PV; ++j) pre[j] = a.u[tl.vbase + min(lane + 64 * j, tl.count * NP - 1)];
+    for (int j = 0; j < PV; ++j) pre[j] = a.u[d.vbase + min(lane + 64 * j, d.count * NP - 1)];
+    jn = a.geo[9 * a.K + d.gbase + (c < d.count ? c : d.count - 1)];
+  };
+  if (t0 < t1) {
+    tl = tile_uniform(a.tiles[t0]);
+    issue(tl);
+    if (t0 + ts < t1) tile_raw = a.tiles[t0 + ts];
   }
-  for (; t < a.ntiles; t += nw) {
+  for (int t = t0; t < t1; t += ts) {
     const Tile cur = tl;
+    const double jv = jn;
     wave_fence();
 #pragma unroll
     for (int j = 0; j < PV; ++j) {
@@ -941,10 +1014,10 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
         sU[e * S + nn] = pre[j];
       }
     }
-    if (t + nw < a.ntiles) {
-      tl = a.tiles[t + nw];
-#pragma unroll
-      for (int j = 0; j < PV; ++j) pre[j] = a.u[tl.vbase + min(lane + 64 * j, tl.count * NP - 1)];
+    if (t + ts < t1) {
+      tl = tile_uniform(tile_raw);
+      issue(tl);
+      if (t + 2 * ts < t1) tile_raw = a.tiles[t + 2 * ts];
     }
     wave_fence();
     const int cc = c < cur.count ? c : 0;
@@ -993,8 +1066,7 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
     const float y = mlp_forward_tile(R, X);
     if (g == 0 && c < cur.count) {
       const int64_t ge = (int64_t)cur.gbase + c;
-      const double vol = (4.0 / 3.0) * a.geo[9 * a.K + ge];
-      const double h = cbrt(6.0 * vol);
+      const double h = cbrt(6.0 * (4.0 / 3.0) * jv);
       const double e = a.ceps * h * sqrt(3.0) * umax / N * (double)y;
       a.eps[ge] = e;
       if (a.frac) a.frac[ge] = (double)y;
@@ -1331,7 +1403,8 @@ extern "C" int dgk_av_sensor_pp(dgk_av a, const double* u_dev, double* maxeps_de
   r.frac = frac_dev;
   r.frac32 = frac32_dev;
   r.feats = feats_dev;
-  const dim3 grid(groups_grid(m, 8)), block(64);
+  // four waves (tiles) per block, two blocks per CU
+  const dim3 grid(std::max(1, std::min((m->ntiles64 + 3) / 4, 2 * m->dev->cu_count))), block(256);
   DGK_AV_LAUNCH(pp_sensor_kernel, grid, block, 0, r);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
