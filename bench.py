@@ -480,6 +480,7 @@ def bench_c5(a, dev) -> dict:
     Persson-Peraire runs of 32 seeded fields on a 16^3 x 6 mesh; then the
     Persson-Peraire and learned-viscosity runs are timed on the full mesh and
     judged against the exact entropy solution on a node sample."""
+    from dgkernel_amd import _lib
     from dgkernel_amd import av as AV
     from dgkernel_amd import dg
     from dgkernel_amd import mesh as M
@@ -507,7 +508,35 @@ def bench_c5(a, dev) -> dict:
                        "generate_s": t_gen, "steps": a.train_steps, "batch": a.train_batch,
                        "train_s": t_train, "samples_per_s": a.train_steps * a.train_batch / t_train,
                        "loss_history": hist}
-    del X, T
+    # The gradient over the whole training set (one dgk_mlp_grad call on the
+    # BF16 matrix cores): rows/s and the algorithmic rate against the dense
+    # BF16 peak. Flops per row: forward 2 (64 nin + 64 x 64), backward
+    # W2^T dz2, dz2^T h1 (2 x 64 x 64 each) and dz1^T x (2 x 64 nin).
+    nin = stt.ref.Np
+    gbuf = dev.zeros(net.nparams, np.float32)
+    lbuf = dev.zeros(4, np.float32)
+
+    def full_grad():
+        _lib.call("dgk_mlp_grad", net.h, X.ptr, T.ptr, nrows, gbuf.ptr, lbuf.ptr)
+
+    _warm(dev, full_grad)
+    reps = 10
+    dev.sync()
+    tm = dev.timer()
+    tm.start()
+    for _ in range(reps):
+        full_grad()
+    tm.stop()
+    g_ms = tm.ms() / reps
+    flops_row = 4 * 64 * nin + 6 * 64 * 64
+    out["training"]["grad_full"] = {
+        "rows": nrows, "ms": g_ms, "rows_per_s": nrows / (g_ms * 1e-3),
+        "tflops": flops_row * nrows / (g_ms * 1e-3) / 1e12,
+        "frac_bf16_peak": flops_row * nrows / (g_ms * 1e-3) / 2.5e15,
+        "flops_per_row": flops_row}
+    log(f"gradient over {nrows} rows: {g_ms:.3f} ms, {nrows / (g_ms * 1e-3):.3e} rows/s, "
+        f"{out['training']['grad_full']['tflops']:.1f} TFLOP/s")
+    del X, T, gbuf, lbuf
     # 2. the full mesh
     n = a.c5_n
     m = M.cube_mesh(n, n, a.nz, periodic=True)

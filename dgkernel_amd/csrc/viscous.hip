@@ -1080,20 +1080,12 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
   }
 }
 
-// ---------------------------------------------------------------------------
-// Training: FP32, one sample per lane, 64 samples per block. Forward and
-// backward of the mean squared error of y against the target; the block's
-// parameter-gradient partial sums (over its samples) are added to grad with
-// float atomics. Samples: index i of the batch takes data row
-// hash(seed, i) % ndata, or row i when seed == 0 (whole-set gradients).
-// ---------------------------------------------------------------------------
+// Sample index i of a training batch takes data row hash(seed, i) % ndata.
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
 }
-
-typedef const __attribute__((address_space(4))) float* cfloat_ptr;
 
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
@@ -1101,100 +1093,287 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-__global__ void __launch_bounds__(64) mlp_train_kernel(const float* __restrict__ Pg, int nin,
-                                                       const float* __restrict__ x,
-                                                       const float* __restrict__ tgt, int64_t ndata,
-                                                       int batch, uint64_t seed,
-                                                       float* __restrict__ grad,
-                                                       float* __restrict__ loss) {
-  const MlpLayout L = mlp_layout(nin);
-  // Parameters are read with wave-uniform addresses (scalar loads).
-  const cfloat_ptr P = (cfloat_ptr)Pg;
-  extern __shared__ float sm[];
-  float* sX = sm;                 // [64][nin]
-  float* sH1 = sX + 64 * nin;     // [64][64]
-  float* sD1 = sH1 + 64 * kH;
-  float* sD2 = sD1 + 64 * kH;
-  const int t = threadIdx.x;
-  const int64_t bi = (int64_t)blockIdx.x * 64 + t;
-  const bool valid = bi < batch;
-  const int64_t row = !valid ? 0
-                      : (seed ? (int64_t)(mix64(seed + 0x9e3779b97f4a7c15ULL * (uint64_t)(bi + 1)) %
-                                          (uint64_t)ndata)
-                              : bi);
-  for (int j = 0; j < nin; ++j) sX[t * nin + j] = valid ? x[row * nin + j] : 0.f;
-  float h1[kH];
-  for (int i = 0; i < kH; ++i) {
-    float z = P[L.ob1 + i];
-    for (int j = 0; j < nin; ++j) z = fmaf(P[L.oW1 + i * nin + j], sX[t * nin + j], z);
-    h1[i] = tanhf(z);
-    sH1[t * kH + i] = h1[i];
+// ---------------------------------------------------------------------------
+// Gradient of the sensor network on BF16 matrix cores (FP32 accumulation).
+// Each wave takes 32-row chunks (two 16-row tiles). The forward pass is the
+// inference path (x, W1, W2 and h1 rounded to BF16; tanh and the sigmoid in
+// FP32). Backward, with y - t of each row:
+//   dy = 2 (y - t) / B y (1 - y),   dz2 = dy w3 (1 - h2^2),
+//   dH1 = W2^T dz2 on matrix cores (dz2 and W2 in BF16; W2^T's k order is
+//         permuted like W2's, so dz2's result layout is the B operand),
+//   dz1 = dH1 (1 - h1^2),
+//   dW2 = dz2^T h1 and dW1 = dz1^T x on matrix cores with the chunk's 32 rows
+//         as the k dimension (dz2, h1, dz1 and x transposed through LDS in
+//         BF16),
+//   b1, b2, w3, b3 and the loss summed in FP32 on the vector units.
+// Each wave writes its sums to its own slot of the partial buffer, and
+// mlp_reduce_kernel adds the slots in a fixed order: no atomics, so a run
+// is reproducible.
+// ---------------------------------------------------------------------------
+constexpr int kGradWaves = 4;        // waves per block of the gradient kernel
+constexpr int kGradMaxWaves = 1024;  // partial slots
+
+struct GradArgs {
+  const float* P;   // FP32 master parameters
+  int nin;
+  const float* x;   // [ndata][nin]
+  const float* t;   // [ndata]
+  int64_t ndata;
+  int64_t rows;     // rows of this gradient (the batch)
+  uint64_t seed;    // 0: rows 0 .. rows-1 in order; else drawn with replacement
+  float* part;      // [waves][stride]: the parameters' sums, then the loss
+  int stride;
+};
+
+__global__ void __launch_bounds__(64 * kGradWaves, 1) mlp_grad_kernel(GradArgs a) {
+  constexpr int W = kGradWaves, RS = 40;  // transpose row stride (BF16): 80 B
+  const MlpLayout L = mlp_layout(a.nin);
+  const int nin = a.nin;
+  // A-operand fragments (16x16x32 BF16): W1 and W2 in the permuted k order
+  // (forward), W2^T in the permuted k order (backward).
+  __shared__ bf16x8 sA1[4][2][64], sA2[4][2][64], sAT[4][2][64];
+  __shared__ float sB1[kH], sB2[kH], sW3[kH];
+  // Per wave: [unit or feature][row of the chunk] in BF16.
+  __shared__ __attribute__((aligned(16))) __bf16 sT2[W][kH * RS], sTh[W][kH * RS],
+      sT1[W][kH * RS], sTx[W][kH * RS];
+  for (int q = threadIdx.x; q < 4 * 2 * 64; q += 64 * W) {
+    const int l = q & 63, ms = q >> 6, mt = ms >> 1, s = ms & 1, row = l & 15, g = l >> 4;
+    bf16x8 f1, f2, ft;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = perm_k(s, g, j), u = 16 * mt + row;
+      f1[j] = (__bf16)(k < nin ? a.P[L.oW1 + u * nin + k] : 0.f);
+      f2[j] = (__bf16)a.P[L.oW2 + u * kH + k];
+      ft[j] = (__bf16)a.P[L.oW2 + k * kH + u];
+    }
+    sA1[mt][s][l] = f1;
+    sA2[mt][s][l] = f2;
+    sAT[mt][s][l] = ft;
   }
-  float h2[kH];
-  float acc = P[L.ob3];
-  for (int i = 0; i < kH; ++i) {
-    float z = P[L.ob2 + i];
-    for (int j = 0; j < kH; ++j) z = fmaf(P[L.oW2 + i * kH + j], h1[j], z);
-    h2[i] = tanhf(z);
-    acc = fmaf(P[L.ow3 + i], h2[i], acc);
-  }
-  const float y = 1.f / (1.f + expf(-acc));
-  const float err = valid ? y - tgt[row] : 0.f;
-  const float dy = 2.f * err / (float)batch * y * (1.f - y);
-  const float lsum = wave_sum(err * err / (float)batch);
-  if (t == 0) atomicAdd(loss, lsum);
-  // d/dw3 and d/db3: sums over the block's samples by wave reduction.
-  float gw3 = 0.f;
-  for (int i = 0; i < kH; ++i) {
-    const float v = wave_sum(dy * h2[i]);
-    if (t == i) gw3 = v;
-  }
-  const float gb3 = wave_sum(dy);
-  for (int i = 0; i < kH; ++i) sD2[t * kH + i] = dy * P[L.ow3 + i] * (1.f - h2[i] * h2[i]);
-  for (int j = 0; j < kH; ++j) {
-    float s = 0.f;
-    for (int i = 0; i < kH; ++i) s = fmaf(P[L.oW2 + i * kH + j], sD2[t * kH + i], s);
-    sD1[t * kH + j] = s * (1.f - h1[j] * h1[j]);
+  for (int q = threadIdx.x; q < kH; q += 64 * W) {
+    sB1[q] = a.P[L.ob1 + q];
+    sB2[q] = a.P[L.ob2 + q];
+    sW3[q] = a.P[L.ow3 + q];
   }
   __syncthreads();
-  // Partial sums over this block's samples.
-  for (int p = t; p < L.ow3; p += 64) {
-    float s = 0.f;
-    if (p < L.ob1) {
-      const int i = p / nin, j = p - i * nin;
-      for (int q = 0; q < 64; ++q) s = fmaf(sD1[q * kH + i], sX[q * nin + j], s);
-    } else if (p < L.oW2) {
-      const int i = p - L.ob1;
-      for (int q = 0; q < 64; ++q) s += sD1[q * kH + i];
-    } else if (p < L.ob2) {
-      const int i = (p - L.oW2) / kH, j = (p - L.oW2) % kH;
-      for (int q = 0; q < 64; ++q) s = fmaf(sD2[q * kH + i], sH1[q * kH + j], s);
-    } else {
-      const int i = p - L.ob2;
-      for (int q = 0; q < 64; ++q) s += sD2[q * kH + i];
+  const float b3 = a.P[L.ob3];
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15, g = lane >> 4;
+  __bf16* T2 = sT2[w];
+  __bf16* Th = sTh[w];
+  __bf16* T1 = sT1[w];
+  __bf16* Tx = sTx[w];
+  const int nft = (nin + 15) / 16;  // feature tiles of dW1
+  const float inv_b = 1.f / (float)a.rows;
+  floatx4 gW2[4][4], gW1[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) gW2[i][j] = gW1[i][j] = (floatx4){0.f, 0.f, 0.f, 0.f};
+  float gb1[4][4], gb2[4][4], gw3[4][4], gb3 = 0.f, lsum = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) gb1[i][r] = gb2[i][r] = gw3[i][r] = 0.f;
+  const int64_t nchunk = (a.rows + 31) / 32;
+  const int wave = blockIdx.x * W + w, nwave = gridDim.x * W;
+  for (int64_t ch = wave; ch < nchunk; ch += nwave) {
+    wave_fence();  // previous readers of the transposes are done
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+      const int64_t rr = ch * 32 + 16 * tt + c;  // batch position of this lane's row
+      const bool valid = rr < a.rows;
+      const int64_t idx = !valid ? 0
+                          : (a.seed ? (int64_t)(mix64(a.seed + 0x9e3779b97f4a7c15ULL *
+                                                                   (uint64_t)(rr + 1)) %
+                                                (uint64_t)a.ndata)
+                                    : rr);
+      const float* xr = a.x + idx * nin;
+      bf16x8 X[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = perm_k(s, g, j);
+          X[s][j] = (__bf16)(k < nin ? xr[k] : 0.f);
+          Tx[k * RS + 16 * tt + c] = X[s][j];
+        }
+      const float tv = a.t[idx];
+      // forward
+      floatx4 z[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        z[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sA1[mt][s][lane], X[s], z[mt], 0, 0, 0);
+      }
+      float h1[4][4];
+      bf16x8 H[2];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int u = 16 * mt + 4 * g + r;
+          h1[mt][r] = tanh_fast(z[mt][r] + sB1[u]);
+          const __bf16 hb = (__bf16)h1[mt][r];
+          H[mt >> 1][4 * (mt & 1) + r] = hb;
+          Th[u * RS + 16 * tt + c] = hb;
+        }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        z[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sA2[mt][s][lane], H[s], z[mt], 0, 0, 0);
+      }
+      float h2[4][4], part = 0.f;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int u = 16 * mt + 4 * g + r;
+          h2[mt][r] = tanh_fast(z[mt][r] + sB2[u]);
+          part = fmaf(sW3[u], h2[mt][r], part);
+        }
+      part += __shfl_xor(part, 16, 64);
+      part += __shfl_xor(part, 32, 64);
+      const float y = __builtin_amdgcn_rcpf(1.f + __expf(-(part + b3)));
+      const float err = valid ? y - tv : 0.f;
+      const float dy = 2.f * err * inv_b * y * (1.f - y);
+      if (g == 0) {
+        lsum = fmaf(err * err, inv_b, lsum);
+        gb3 += dy;
+      }
+      // backward
+      bf16x8 D2[2];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int u = 16 * mt + 4 * g + r;
+          const float d = dy * sW3[u] * (1.f - h2[mt][r] * h2[mt][r]);
+          gb2[mt][r] += d;
+          gw3[mt][r] = fmaf(dy, h2[mt][r], gw3[mt][r]);
+          const __bf16 db = (__bf16)d;
+          D2[mt >> 1][4 * (mt & 1) + r] = db;
+          T2[u * RS + 16 * tt + c] = db;
+        }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        z[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sAT[mt][s][lane], D2[s], z[mt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int u = 16 * mt + 4 * g + r;
+          const float d = z[mt][r] * (1.f - h1[mt][r] * h1[mt][r]);
+          gb1[mt][r] += d;
+          T1[u * RS + 16 * tt + c] = (__bf16)d;
+        }
     }
-    atomicAdd(grad + p, s);
+    wave_fence();
+    // weight gradients: the chunk's 32 rows are the k dimension; lane (m, k)
+    // reads rows 8 k .. 8 k + 7 of unit (or feature) 16 tile + m
+    bf16x8 at2[4], at1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      at2[mt] = *(const bf16x8*)(T2 + (16 * mt + c) * RS + 8 * g);
+      at1[mt] = *(const bf16x8*)(T1 + (16 * mt + c) * RS + 8 * g);
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      const bf16x8 bh = *(const bf16x8*)(Th + (16 * nt + c) * RS + 8 * g);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+        gW2[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at2[mt], bh, gW2[mt][nt], 0, 0, 0);
+      if (nt < nft) {
+        const bf16x8 bx = *(const bf16x8*)(Tx + (16 * nt + c) * RS + 8 * g);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+          gW1[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at1[mt], bx, gW1[mt][nt], 0, 0, 0);
+      }
+    }
   }
-  atomicAdd(grad + L.ow3 + t, gw3);
-  if (t == 0) atomicAdd(grad + L.ob3, gb3);
+  // The wave's sums to its slot: D lanes (n, g) hold rows 16 mt + 4 g + r,
+  // column 16 nt + n; the vector sums reduce over the 16 rows of a lane
+  // group.
+  float* out = a.part + (int64_t)wave * a.stride;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int u = 16 * mt + 4 * g + r;
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        out[L.oW2 + u * kH + 16 * nt + c] = gW2[mt][nt][r];
+        const int f = 16 * nt + c;
+        if (nt < nft && f < nin) out[L.oW1 + u * nin + f] = gW1[mt][nt][r];
+      }
+      float s1 = gb1[mt][r], s2 = gb2[mt][r], s3 = gw3[mt][r];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+        s3 += __shfl_xor(s3, o, 64);
+      }
+      if (c == 0) {
+        out[L.ob1 + u] = s1;
+        out[L.ob2 + u] = s2;
+        out[L.ow3 + u] = s3;
+      }
+    }
+  gb3 = wave_sum(gb3);
+  lsum = wave_sum(lsum);
+  if (lane == 0) {
+    out[L.ob3] = gb3;
+    out[L.total] = lsum;
+  }
 }
 
-// Adam (Kingma & Ba 2015) on the FP32 master parameters; clears grad.
-__global__ void adam_kernel(float* __restrict__ P, float* __restrict__ g, float* __restrict__ m,
-                            float* __restrict__ v, int n, float lr, float b1, float b2, float eps,
-                            float c1, float c2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  g[i] = 0.f;
+// Sum of the nw partial slots in slot order. mode 0: grad[p] = sum (and the
+// loss into *loss); mode 1: Adam step on parameter p (beta1, beta2, eps,
+// bias corrections c1, c2), the loss added to *loss.
+__global__ void __launch_bounds__(256) mlp_reduce_kernel(const float* __restrict__ part, int nw,
+                                                         int stride, int total, int mode,
+                                                         float* __restrict__ grad,
+                                                         float* __restrict__ P,
+                                                         float* __restrict__ m,
+                                                         float* __restrict__ v, float lr, float b1,
+                                                         float b2, float eps, float c1, float c2,
+                                                         float* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > total) return;
+  // eight slots' loads in flight, added in slot order
+  float s = 0.f;
+  int k = 0;
+  for (; k + 8 <= nw; k += 8) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = part[(int64_t)(k + i) * stride + p];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += v[i];
+  }
+  for (; k < nw; ++k) s += part[(int64_t)k * stride + p];
+  if (p == total) {
+    if (loss) *loss = mode == 0 ? s : *loss + s;
+    return;
+  }
+  if (mode == 0) {
+    grad[p] = s;
+    return;
+  }
+  const float mi = b1 * m[p] + (1.f - b1) * s;
+  const float vi = b2 * v[p] + (1.f - b2) * s * s;
+  m[p] = mi;
+  v[p] = vi;
+  P[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-size_t train_smem(int nin) { return ((size_t)64 * nin + 3 * 64 * kH) * sizeof(float); }
 
 }  // namespace
 
@@ -1216,10 +1395,10 @@ struct dgk_mlp_s {
   int nin = 0, total = 0;
   long step = 0;
   float* P = nullptr;
-  float* g = nullptr;
   float* m = nullptr;
   float* v = nullptr;
   float* loss = nullptr;
+  float* part = nullptr;   // per-wave partial sums of the gradient kernel
 };
 
 namespace {
@@ -1476,11 +1655,10 @@ extern "C" int dgk_mlp_create(dgk_device dev, int nin, const float* params_host,
   const size_t bytes = (size_t)n->total * 4;
   int rc = DGK_OK;
   rc |= dgk_malloc(dev, bytes, params_host, (void**)&n->P);
-  rc |= dgk_malloc(dev, bytes, nullptr, (void**)&n->g);
   rc |= dgk_malloc(dev, bytes, nullptr, (void**)&n->m);
   rc |= dgk_malloc(dev, bytes, nullptr, (void**)&n->v);
   rc |= dgk_malloc(dev, 16, nullptr, (void**)&n->loss);
-  if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->g, 0, bytes);
+  rc |= dgk_malloc(dev, (size_t)kGradMaxWaves * (n->total + 1) * 4, nullptr, (void**)&n->part);
   if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->m, 0, bytes);
   if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->v, 0, bytes);
   if (rc == DGK_OK) rc = dgk_fill_bytes(dev, n->loss, 0, 16);
@@ -1495,10 +1673,10 @@ extern "C" int dgk_mlp_create(dgk_device dev, int nin, const float* params_host,
 extern "C" int dgk_mlp_destroy(dgk_mlp n) {
   if (!n) return DGK_OK;
   dgk_free(n->dev, n->P);
-  dgk_free(n->dev, n->g);
   dgk_free(n->dev, n->m);
   dgk_free(n->dev, n->v);
   dgk_free(n->dev, n->loss);
+  dgk_free(n->dev, n->part);
   delete n;
   return DGK_OK;
 }
@@ -1521,7 +1699,6 @@ extern "C" int dgk_mlp_set(dgk_mlp n, const float* params_host) {
   int rc = dgk_copy_h2d(n->dev, n->P, params_host, bytes, 0);
   if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->m, 0, bytes);
   if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->v, 0, bytes);
-  if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->g, 0, bytes);
   return rc;
 }
 
@@ -1540,19 +1717,44 @@ extern "C" int dgk_mlp_forward(dgk_mlp n, const float* x_dev, int64_t count, flo
 
 // Gradient of the mean squared error over all `count` rows into grad_dev
 // (FP32 [num_params]); loss_dev (may be NULL) receives the loss.
+namespace {
+
+// The gradient kernel over `rows` rows (drawn from the ndata rows when seed
+// is nonzero) into the per-wave slots; returns the number of slots used.
+int mlp_grad_launch(dgk_mlp n, const float* x, const float* t, int64_t ndata, int64_t rows,
+                    uint64_t seed) {
+  const int64_t chunks = (rows + 31) / 32;
+  const int waves = (int)std::min<int64_t>(chunks, kGradMaxWaves);
+  const int blocks = (waves + kGradWaves - 1) / kGradWaves;
+  GradArgs g;
+  g.P = n->P;
+  g.nin = n->nin;
+  g.x = x;
+  g.t = t;
+  g.ndata = ndata;
+  g.rows = rows;
+  g.seed = seed;
+  g.part = n->part;
+  g.stride = n->total + 1;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(64 * kGradWaves), 0, n->dev->stream, g);
+  return blocks * kGradWaves;
+}
+
+}  // namespace
+
+// Gradient of the mean squared error over all `count` rows into grad_dev
+// (FP32 [num_params]); loss_dev (may be NULL) receives the loss.
 extern "C" int dgk_mlp_grad(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t count,
                             float* grad_dev, float* loss_dev) {
   dgk::clear_error();
   DGK_REQUIRE(n && x_dev && t_dev && grad_dev, "dgk_mlp_grad: NULL argument");
   DGK_REQUIRE(count >= 1 && count <= (1 << 30), "dgk_mlp_grad: count %lld", (long long)count);
-  int rc = dgk_fill_bytes(n->dev, grad_dev, 0, (size_t)n->total * 4);
-  if (rc == DGK_OK) rc = dgk_fill_bytes(n->dev, n->loss, 0, 4);
-  if (rc != DGK_OK) return rc;
-  const int blocks = (int)((count + 63) / 64);
-  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(64), train_smem(n->nin), n->dev->stream,
-                     n->P, n->nin, x_dev, t_dev, count, (int)count, (uint64_t)0, grad_dev, n->loss);
+  const int nw = mlp_grad_launch(n, x_dev, t_dev, count, count, 0);
   DGK_LAUNCH_CHECK();
-  if (loss_dev) return dgk_copy_d2d(n->dev, loss_dev, n->loss, 4);
+  hipLaunchKernelGGL(mlp_reduce_kernel, dim3((n->total + 1 + 255) / 256), dim3(256), 0,
+                     n->dev->stream, n->part, nw, n->total + 1, n->total, 0, grad_dev, nullptr,
+                     nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, loss_dev ? loss_dev : n->loss);
+  DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
@@ -1566,15 +1768,13 @@ extern "C" int dgk_mlp_train_step(dgk_mlp n, const float* x_dev, const float* t_
   DGK_REQUIRE(ndata >= 1 && batch >= 1, "dgk_mlp_train_step: empty data or batch");
   n->step += 1;
   const uint64_t s = (seed | 1ULL) * 0x2545F4914F6CDD1DULL + (uint64_t)n->step;
-  const int blocks = (batch + 63) / 64;
-  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(64), train_smem(n->nin), n->dev->stream,
-                     n->P, n->nin, x_dev, t_dev, ndata, batch, s ? s : 1ULL, n->g,
-                     loss_acc_dev ? loss_acc_dev : n->loss);
+  const int nw = mlp_grad_launch(n, x_dev, t_dev, ndata, batch, s ? s : 1ULL);
   DGK_LAUNCH_CHECK();
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - powf(b1, (float)n->step), c2 = 1.f - powf(b2, (float)n->step);
-  hipLaunchKernelGGL(adam_kernel, dim3((n->total + 255) / 256), dim3(256), 0, n->dev->stream, n->P,
-                     n->g, n->m, n->v, n->total, lr, b1, b2, 1e-8f, c1, c2);
+  hipLaunchKernelGGL(mlp_reduce_kernel, dim3((n->total + 1 + 255) / 256), dim3(256), 0,
+                     n->dev->stream, n->part, nw, n->total + 1, n->total, 1, nullptr, n->P, n->m,
+                     n->v, lr, b1, b2, 1e-8f, c1, c2, loss_acc_dev ? loss_acc_dev : n->loss);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }

@@ -421,13 +421,18 @@ int dgk_mlp_get(dgk_mlp n, float* params_host);
 int dgk_mlp_set(dgk_mlp n, const float* params_host);
 /* y[i] = net(x[i]) for x [count][nin] (FP32, device) on BF16 matrix cores. */
 int dgk_mlp_forward(dgk_mlp n, const float* x_dev, int64_t count, float* y_dev);
-/* FP32 gradient of mean_i (net(x_i) - t_i)^2 over all rows into grad_dev
- * [num_params]; the loss into *loss_dev if not NULL. */
+/* Gradient of mean_i (net(x_i) - t_i)^2 over all rows into grad_dev
+ * [num_params] (FP32), the loss into *loss_dev if not NULL. Computed on BF16
+ * matrix cores with FP32 accumulation: the forward pass rounds as
+ * dgk_mlp_forward, the backward products take BF16 operands (dz2 and W2;
+ * dz2, h1, dz1 and x), the bias sums are FP32. Per-wave partial sums are
+ * added in a fixed order, so repeated calls give identical results. */
 int dgk_mlp_grad(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t count,
                  float* grad_dev, float* loss_dev);
 /* One Adam step (beta 0.9, 0.999) with learning rate lr on a batch drawn
- * with replacement from the ndata rows by a hash of (seed, step count); the
- * batch loss is added to *loss_acc_dev if not NULL. */
+ * with replacement from the ndata rows by a hash of (seed, step count), the
+ * gradient as dgk_mlp_grad's; the batch loss is added to *loss_acc_dev if
+ * not NULL. Two launches, no allocation. */
 int dgk_mlp_train_step(dgk_mlp n, const float* x_dev, const float* t_dev, int64_t ndata,
                        int batch, uint64_t seed, float lr, float* loss_acc_dev);
 

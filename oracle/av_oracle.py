@@ -5,9 +5,10 @@ CPU leg of bench.py as the checker. The product package never imports it.
 
 * ``mlp_forward`` / ``mlp_grad``: the sensor network nin -> 64 -> 64 -> 1
   (tanh, tanh, sigmoid) and the gradient of the mean squared error, written
-  from the chain rule. ``mlp_forward(..., bf16=True)`` rounds what the
-  device's matrix-core path rounds (inputs, W1, W2 and the first hidden
-  layer, round-to-nearest-even to bfloat16).
+  from the chain rule. ``bf16_path=True`` rounds what the device's
+  matrix-core path rounds (round-to-nearest-even to bfloat16): inputs, W1,
+  W2 and the first hidden layer in the forward pass, and the operands of the
+  backward products in the gradient.
 * ``burgers_exact``: the entropy solution of u_t + div(u^2/2 (1,1,1)) = 0 on
   the periodic unit cube. On every line x0 + tau (1,1,1) the equation is the
   1D Burgers equation v_t + v v_tau = 0, so past shock formation
@@ -59,20 +60,33 @@ def mlp_forward(P: np.ndarray, x: np.ndarray, nin: int, bf16_path: bool = False)
     return 1.0 / (1.0 + np.exp(-(h2 @ w["w3"] + w["b3"][0])))
 
 
-def mlp_grad(P: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int):
-    """(gradient of mean (y - t)^2 as a flat array, loss), float64."""
+def mlp_grad(P: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, bf16_path: bool = False,
+             rnd=None):
+    """(gradient of mean (y - t)^2 as a flat array, loss), float64.
+
+    ``bf16_path`` restates the device's matrix-core gradient: the forward
+    rounds x, W1, W2 and h1 to bfloat16 (as ``mlp_forward``), the backward
+    rounds dz2 and W2 (operands of W2^T dz2) and dz2, h1, dz1, x (operands of
+    the weight-gradient products) to bfloat16, and takes the derivative of a
+    rounding as 1 (tanh' from the unrounded tanh); the bias, w3 and b3
+    gradients sum the unrounded values. ``rnd`` replaces the rounding (the
+    identity gives the exact gradient: a check of the rounding points)."""
     w = unpack(P, nin)
     x = np.asarray(x, dtype=np.float64)
     t = np.asarray(t, dtype=np.float64)
+    q = rnd if rnd is not None else (bf16 if bf16_path else (lambda v: np.asarray(v, np.float64)))
     B = x.shape[0]
-    h1 = np.tanh(x @ w["W1"].T + w["b1"])
-    h2 = np.tanh(h1 @ w["W2"].T + w["b2"])
+    xq, W1q, W2q = q(x), q(w["W1"]), q(w["W2"])
+    h1 = np.tanh(xq @ W1q.T + w["b1"])
+    h1q = q(h1)
+    h2 = np.tanh(h1q @ W2q.T + w["b2"])
     y = 1.0 / (1.0 + np.exp(-(h2 @ w["w3"] + w["b3"][0])))
     err = y - t
     dy = 2.0 * err / B * y * (1.0 - y)
     dz2 = np.outer(dy, w["w3"]) * (1.0 - h2 * h2)
-    dz1 = (dz2 @ w["W2"]) * (1.0 - h1 * h1)
-    g = {"W1": dz1.T @ x, "b1": dz1.sum(0), "W2": dz2.T @ h1, "b2": dz2.sum(0),
+    dz2q = q(dz2)
+    dz1 = (dz2q @ W2q) * (1.0 - h1 * h1)
+    g = {"W1": q(dz1).T @ xq, "b1": dz1.sum(0), "W2": dz2q.T @ h1q, "b2": dz2.sum(0),
          "w3": h2.T @ dy, "b3": np.array([dy.sum()])}
     flat = np.zeros(sum(int(np.prod(s)) for _, s in layout(nin).values()))
     for k, (o, shp) in layout(nin).items():

@@ -146,6 +146,36 @@ def test_mlp_gradient_finite_differences():
         assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd)
 
 
+def test_mlp_gradient_bf16_path():
+    """The BF16 gradient path: with its rounding replaced by the identity it is
+    the exact (finite-difference-pinned) gradient; with bfloat16 rounding it
+    stays within the rounding's reach of the exact gradient (a few 1e-3
+    relative), while a dropped tanh derivative in the backward pass lands
+    far outside it."""
+    nin = 35
+    rng = np.random.default_rng(7)
+    P = rng.normal(0, 0.25, A.layout(nin)["b3"][0] + 1)
+    x = rng.uniform(-1, 1, (200, nin))
+    t = rng.uniform(0, 1, 200)
+    g, loss = A.mlp_grad(P, x, t, nin)
+    gi, li = A.mlp_grad(P, x, t, nin, rnd=lambda v: np.asarray(v, np.float64))
+    assert np.array_equal(g, gi) and loss == li
+    gb, lb = A.mlp_grad(P, x, t, nin, bf16_path=True)
+    e = np.linalg.norm(gb - g) / np.linalg.norm(g)
+    assert 1e-4 < e < 2e-2
+    assert abs(lb - loss) <= 2e-2 * loss
+    # planted mistake: tanh' of the first layer omitted from the backward pass
+    w = A.unpack(P, nin)
+    h1 = np.tanh(x @ w["W1"].T + w["b1"])
+    h2 = np.tanh(h1 @ w["W2"].T + w["b2"])
+    y = 1.0 / (1.0 + np.exp(-(h2 @ w["w3"] + w["b3"][0])))
+    dz2 = np.outer(2.0 * (y - t) / len(t) * y * (1 - y), w["w3"]) * (1 - h2 * h2)
+    bad = (dz2 @ w["W2"]).T @ x
+    o = A.layout(nin)["W1"][0]
+    good = g[o:o + 64 * nin].reshape(64, nin)
+    assert np.linalg.norm(bad - good) / np.linalg.norm(good) > 10 * e
+
+
 def test_viscous_run_bounded_and_closer_to_entropy_solution():
     N, n, t_end = 3, 5, 0.3
     m = M.cube_mesh(n, n, n, periodic=True)

@@ -1,7 +1,7 @@
 """Shock capturing on the GPU against the oracle: Persson-Peraire sensor,
 BR1 viscous operator, the viscous SSP-RK3 stepper, the sensor network's
-forward pass (BF16 matrix cores) and gradients (FP32), and the learned
-sensor trained on the GPU from Persson-Peraire runs."""
+forward pass and gradients (BF16 matrix cores), and the learned sensor
+trained on the GPU from Persson-Peraire runs."""
 import math
 
 import numpy as np
@@ -126,21 +126,34 @@ def test_mlp_forward_vs_numpy(device, nin, n):
     assert np.max(np.abs(got - full)) <= 2e-2
 
 
-@pytest.mark.parametrize("nin,n", [(35, 500), (12, 64), (35, 1)])
+@pytest.mark.parametrize("nin,n", [(35, 500), (12, 64), (35, 1), (35, 70001)])
 def test_mlp_gradient_vs_numpy(device, nin, n):
+    """The gradient kernel (BF16 matrix cores, FP32 accumulation) against the
+    oracle's restatement of the same rounding points, and against the exact
+    gradient within the rounding's reach; 70001 rows give each wave several
+    32-row chunks and a partial last one. Two calls agree bit for bit (the
+    per-wave sums are added in a fixed order)."""
     P = _random_params(nin, 5)
     x = M.uniform(6, 1, n * nin, -1, 1).astype(np.float32).reshape(n, nin)
     t = M.uniform(6, 2, n, 0, 1).astype(np.float32)
     net = AV.SensorNet(device, nin, P)
-    g, loss = net.grad(device.upload(x), device.upload(t), n)
+    xd, td = device.upload(x), device.upload(t)
+    g, loss = net.grad(xd, td, n)
+    g2, loss2 = net.grad(xd, td, n)
+    assert np.array_equal(g, g2) and loss == loss2
+    gq, lq = A.mlp_grad(P, x, t, nin, bf16_path=True)
     gw, lw = A.mlp_grad(P, x, t, nin)
     # Every gradient entry carries the factor y - t, so FP32 rounding of y
     # (~1e-7 y) is amplified by y / |y - t| (large when a single sample's
     # output is close to its target); batch means average this out.
-    y = A.mlp_forward(P, x, nin)
+    y = A.mlp_forward(P, x, nin, bf16_path=True)
     amp = float(np.max(np.abs(y) / np.maximum(np.abs(y - t), 1e-12)))
-    assert rel_l2(g, gw) <= 2e-5 + 2e-7 * amp
-    assert abs(loss - lw) <= 1e-5 * lw + 6e-7 * amp * lw  # (y - t)^2: twice the amplification
+    eq, ew = rel_l2(g, gq), rel_l2(g, gw)
+    print(f"nin={nin} n={n}: rel L2 vs BF16-path oracle {eq:.3e}, vs exact {ew:.3e}, amp {amp:.1f}")
+    assert eq <= 1e-4 + 2e-8 * amp
+    if n >= 64:  # a single sample with y close to t amplifies the BF16 forward error
+        assert ew <= 3e-2
+    assert abs(loss - lq) <= 1e-5 * lq + 6e-7 * amp * lq  # (y - t)^2: twice the amplification
 
 
 def test_mlp_sensor_matches_network_on_modal_features(device):
