@@ -408,13 +408,14 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
   const double* sAv = sA;
   const double* sAl = sA + MTG * KSG * 64;
   const double* sAd = sA + SH::NA_G;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // wave index as a scalar, so that tile indices and records stay in SGPRs
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = lane & 15, g = lane >> 4;
   double* sU = sUall[w];
   const int64_t K = a.K;
   const int nw = gridDim.x * WPB;
   for (int t = blockIdx.x * WPB + w; t < a.ntiles; t += nw) {
-    const Tile tl = a.tiles[t];
+    const Tile tl = tile_uniform(a.tiles[t]);
     const int cnt = tl.count;
     wave_fence();
     for (int i = lane; i < cnt * NP; i += 64) {
@@ -429,6 +430,35 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
     int32_t nb[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) nb[f] = a.nbr[f * K + ge];
+    // Every other load of the tile is issued here, before the products that
+    // cover its latency: the neighbour traces of the gradient's face jumps
+    // (they need the neighbour offsets), the metrics, the face normals and
+    // Fscale, eps and the folded metrics.
+    double uPv[4][SPF];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int code = (packed >> (5 * f)) & 31;
+#pragma unroll
+      for (int sp = 0; sp < SPF; ++sp) {
+        const int fpc = min(4 * sp + g, NFP - 1);
+        uPv[f][sp] = a.u[nb[f] + sFN[(f * 24 + code) * NFP + fpc]];
+      }
+    }
+    double mt[9], nrm[4][3], hfs[4];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + ge];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      hfs[f] = 0.5 * a.geo[(13 + 4 * f) * K + ge];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) nrm[f][d] = a.geo[(10 + 4 * f + d) * K + ge];
+    }
+    const double ee = a.eps_in[ge];
+    double gh[3] = {0.0, 0.0, 0.0};
+    if (BURG) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gh[d] = 0.5 * bgeo[d * K + ge];
+    }
     // Gradient: [Dr; Ds; Dt] u and LIFT_f (u+ - u-) per face.
     double4v accV[MTG];
 #pragma unroll
@@ -445,34 +475,26 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
     double4v accF[4][MTL];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-      const int code = (packed >> (5 * f)) & 31;
 #pragma unroll
       for (int tt = 0; tt < MTL; ++tt) accF[f][tt] = (double4v){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int sp = 0; sp < SPF; ++sp) {
         const int fp = 4 * sp + g;
-        const int fpc = min(fp, NFP - 1);
-        const double uP = a.u[nb[f] + sFN[(f * 24 + code) * NFP + fpc]];
-        const double uM = su[face_node<N>(f, 4 * sp, g)];
-        const double b = fp < NFP ? uP - uM : 0.0;
+        const double uM = su[sFM[f * NFP + min(fp, NFP - 1)]];
+        const double b = fp < NFP ? uPv[f][sp] - uM : 0.0;
 #pragma unroll
         for (int tt = 0; tt < MTL; ++tt)
           accF[f][tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
               sAl[(tt * 4 * SPF + f * SPF + sp) * 64 + lane], b, accF[f][tt], 0, 0, 0);
       }
     }
-    double mt[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) mt[q] = a.geo[q * K + ge];
     double qv[3][QN];
     {
       double cf[4][3];
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        const double hfs = 0.5 * a.geo[(13 + 4 * f) * K + ge];
+      for (int f = 0; f < 4; ++f)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cf[f][d] = hfs * a.geo[(10 + 4 * f + d) * K + ge];
-      }
+        for (int d = 0; d < 3; ++d) cf[f][d] = hfs[f] * nrm[f][d];
 #pragma unroll
       for (int q = 0; q < QN; ++q) {
         const double ur = accV[(4 * q) / 16][((4 * q) % 16) / 4];
@@ -492,16 +514,10 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
       }
     }
     // Volume: [Dr | Ds | Dt] [S_r; S_s; S_t], B from registers.
-    const double ee = a.eps_in[ge];
     double4v accD[MTL];
 #pragma unroll
     for (int tt = 0; tt < MTL; ++tt) accD[tt] = (double4v){0.0, 0.0, 0.0, 0.0};
     {
-      double gh[3] = {0.0, 0.0, 0.0};
-      if (BURG) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gh[d] = 0.5 * bgeo[d * K + ge];
-      }
 #pragma unroll
       for (int s = 0; s < KSD; ++s) {
         const int blk = (4 * s) / NPR;
@@ -531,7 +547,7 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
         if (i < NP) sU[c * S + i] = qv[d][q];
       }
       wave_fence();
-      const double nd = a.geo[(10 + 4 * g + d) * K + ge];
+      const double nd = g == 0 ? nrm[0][d] : g == 1 ? nrm[1][d] : g == 2 ? nrm[2][d] : nrm[3][d];
 #pragma unroll
       for (int fp = 0; fp < NFP; ++fp) sg[fp] = fma(nd, su[sFM[g * NFP + fp]], sg[fp]);
     }
