@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(64 * WPB) burgers_stage_mfma_kernel(StageParam
   __shared__ __attribute__((aligned(16))) double sUall[WPB][E * S];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[NF];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = lane & 15, gq = lane >> 4;
   for (int i = threadIdx.x; i < 4 * 24 * NFP; i += NT) sFN[i] = p.face_nodes[i];
   for (int i = threadIdx.x; i < NF; i += NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
