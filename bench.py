@@ -65,16 +65,30 @@ def stage_kernel_name(N: int, solver) -> str:
 
 def read_traffic(name: str, key: str = "traffic_bytes_per_launch"):
     """HBM bytes per launch recorded from rocprofv3 --pmc runs (profiles/<name>)."""
+    v = read_profile_key(name, key)
+    try:
+        return None if v is None else float(v.split()[0])
+    except ValueError:
+        return None
+
+
+def read_profile_key(name: str, key: str):
+    """The text after `key:` on its own line of profiles/<name>, or None."""
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
         if line.strip().startswith(key + ":"):
-            try:
-                return float(line.split(":", 1)[1].strip().split()[0])
-            except ValueError:
-                return None
+            return line.split(":", 1)[1].strip()
     return None
+
+
+def stage_source_hash() -> str:
+    """SHA-256 (first 16 hex digits) of the fused stage kernel's source: a
+    profile records it beside the traffic it measured, so a line can say
+    whether that figure belongs to the kernel it timed."""
+    import hashlib
+    return hashlib.sha256((ROOT / "dgkernel_amd" / "csrc" / "burgers.hip").read_bytes()).hexdigest()[:16]
 
 
 def build_c3(n: int, N: int, nparts: int, seed: int, morton: bool = False):
@@ -749,9 +763,14 @@ def main(argv=None) -> int:
     bpe = stage_bytes_per_elem(Np)
     achieved = bpe * K / (stage_ms.mean() * 1e-3) / 1e9
     flops = stage_flops_per_elem(Np, Nfp) * K / (stage_ms.mean() * 1e-3) / 1e12
-    traffic = read_traffic(f"burgers_stage_p{a.order}_{a.n}cube_pmc.md",
-                           "traffic_bytes_per_launch" if morton
+    prof = f"burgers_stage_p{a.order}_{a.n}cube_pmc.md"
+    traffic = read_traffic(prof, "traffic_bytes_per_launch" if morton
                            else "traffic_bytes_per_launch_generator_order")
+    # The traffic is not counted in this run (hardware counters need their own
+    # rocprofv3 --pmc runs); it comes from the profile, which records the
+    # hash of the stage source it was measured on.
+    src = stage_source_hash()
+    traffic_src = read_profile_key(prof, "stage_source_sha256")
 
     kname = stage_kernel_name(a.order, solver)
     del solver
@@ -791,6 +810,9 @@ def main(argv=None) -> int:
                    "element_order": "morton" if morton else "generator"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": {"profile": f"profiles/{prof}",
+                                        "measured_on_this_source": traffic_src == src,
+                                        "stage_source_sha256": src},
                      "kernel": kname,
                      "bytes_per_elem": bpe,
                      "stage_ms_mean": float(stage_ms.mean()),
