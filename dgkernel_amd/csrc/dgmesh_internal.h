@@ -89,7 +89,9 @@ struct dgk_dgmesh_s {
   int prec = 0;                          // DGK_PREC_* of the D contractions
   int lift_prec = 0;                     // DGK_PREC_* of the LIFT contraction
   dgk_deriv_plan plan = nullptr;         // built on first use for (N, prec)
-  double* rhs = nullptr;                 // state-sized FP64 volume term
+  void* rhs = nullptr;                   // state-sized volume term (FP64, or FP32
+                                         // where the plan's epilogue is FP32)
+  int rhs32 = 0;                         // rhs holds FP32 values
   void* lift_frag = nullptr;             // LIFT fragments for lift_prec
   int split_blocks = 0;                  // persistent grid of the surface kernel
 };
@@ -113,9 +115,11 @@ void split_release(dgk_dgmesh m);
 
 // Volume term of the Burgers RHS through a derivative plan (mixed.hip):
 // out = -(div f(u)), f(u) = u^2/2 (1,1,1), for the elements of the tiles
-// (16 elements each, runner layout), FP64 output, geo = the 26-row
-// geometry with stride gs.
+// (16 elements each, runner layout), geo = the 26-row geometry with stride
+// gs. The output is FP32 where deriv_plan_divergence_f32 says so (plans
+// whose epilogue computes in FP32), else FP64.
 int deriv_plan_divergence(dgk_deriv_plan plan, const Tile* tiles, int ntiles, const double* u,
-                          const double* geo, int64_t gs, double* out);
+                          const double* geo, int64_t gs, void* out);
+bool deriv_plan_divergence_f32(dgk_deriv_plan plan);
 
 }  // namespace dgk

@@ -26,8 +26,8 @@
 //      in LDS;
 //   4. the epilogue applies the chain rule and writes the outputs with
 //      contiguous stores: in FP64 for the FP64 plan and the BF16x6
-//      divergence, in FP32 for the other plans below FP64 (FP32 outputs; the
-//      divergence of the split Burgers stage is stored as FP64).
+//      divergence, in FP32 for the other plans below FP64 (FP32 outputs,
+//      the divergence of the split Burgers stage included).
 //
 //   F64:     v_mfma_f64_16x16x4_f64.
 //   F32:     v_mfma_f32_16x16x4_f32 on FP32-rounded operator and w.
@@ -182,7 +182,8 @@ struct AffineArgs {
 
 // DIV = false: du/dx, du/dy, du/dz into ox, oy, oz (FP64 or FP32).
 // DIV = true: the Burgers volume term -div f(u), f(u) = u^2/2 (1,1,1), into
-// ox (FP64): the staged values are f, and the epilogue folds the metrics to
+// ox (FP32 for the plans with an FP32 epilogue, FP32 and BF16x3; else FP64):
+// the staged values are f, and the epilogue folds the metrics to
 // -( (rx+ry+rz) Dr f + (sx+sy+sz) Ds f + (tx+ty+tz) Dt f ).
 template <int N, int PREC, bool DIV>
 __global__ void __launch_bounds__((ACfg<N, PREC>::NT))
@@ -602,9 +603,11 @@ deriv_affine_kernel(AffineArgs a) {
         const float ur = (float)re[i] + m2.y, us = (float)re[NP + i] + m2.z,
                     ut = (float)re[2 * NP + i] + m2.w;
         if constexpr (DIV) {
+          // an FP32 result: stored as FP32, which halves the volume term's
+          // traffic here and in the surface kernel that reads it
           const float ga = m0.x + m0.y + m0.z, gb = m0.w + m1.x + m1.y, gc = m1.z + m1.w + m2.x;
-          __builtin_nontemporal_store(-(double)fmaf(gc, ut, fmaf(gb, us, ga * ur)),
-                                      (double*)a.ox + tr.vbase + v);
+          __builtin_nontemporal_store(-fmaf(gc, ut, fmaf(gb, us, ga * ur)),
+                                      (float*)a.ox + tr.vbase + v);
         } else {
           __builtin_nontemporal_store(fmaf(m1.z, ut, fmaf(m0.w, us, m0.x * ur)),
                                       (float*)a.ox + tr.vbase + v);
@@ -1025,8 +1028,12 @@ extern "C" int dgk_phys_deriv_affine(dgk_deriv_plan p, int64_t K, const double* 
   return dispatch(p, &a, 1);
 }
 
+bool dgk::deriv_plan_divergence_f32(dgk_deriv_plan p) {
+  return p->prec == kF32 || p->prec == kBF3;
+}
+
 int dgk::deriv_plan_divergence(dgk_deriv_plan p, const dgk::Tile* tiles, int ntiles, const double* u,
-                               const double* geo, int64_t gs, double* out) {
+                               const double* geo, int64_t gs, void* out) {
   if (ntiles == 0) return DGK_OK;
   AffineArgs a;
   a.K = 0;

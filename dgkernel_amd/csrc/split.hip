@@ -1,7 +1,8 @@
 // Split Burgers stage for orders above 4 (where the fused stage's operators
 // no longer fit a CU) and for contractions below FP64:
 //   1. volume term  rhs = -div f(u)           deriv_affine_kernel<.., DIV>
-//      ([Dr; Ds; Dt] f on matrix cores, metrics folded in FP64);
+//      ([Dr; Ds; Dt] f on matrix cores, metrics folded in FP64, or in FP32
+//      for the FP32 and BF16x3 plans, whose rhs is then stored as FP32);
 //   2. surface term and RK update, this file: per element the Lax-Friedrichs
 //      flux jump g on the 4 Nfp face points (FP64, neighbour trace gathered
 //      through the orientation-code table), LIFT g on matrix cores, then
@@ -90,7 +91,7 @@ struct SurfArgs {
   const int32_t* codes;       // [K] four 5-bit orientation codes
   const double* u;
   const double* u0;
-  const double* rhs;
+  const void* rhs;            // volume term, FP64 or (R32) FP32
   double* out;
   double a0, a1;
   const double* dt;
@@ -131,7 +132,7 @@ __device__ __forceinline__ void wave_sync() {
 //      tile tt (those of row tile 0 during the jumps).
 // Wait counts are in order, so every prefetch is consumed after the loads
 // issued before it, never before.
-template <int N, int PREC>
+template <int N, int PREC, bool R32>
 __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_kernel(SurfArgs a) {
   using C = SCfg<N, PREC>;
   using JT = typename C::JT;
@@ -197,7 +198,9 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_ker
   auto rec_load = [&](const dgk::Tile& tl) {
     const int ce = c < tl.count ? c : tl.count - 1;
     const int64_t ge = tl.gbase + ce;
-    rnb = a.nbr[(int64_t)g * K + ge];
+    int gg = g;
+    asm volatile("" : "+v"(gg));  // (the row base is derived here, not kept live)
+    rnb = a.nbr[(int64_t)gg * K + ge];
     rcode = a.codes[ge];
     rga = a.bgeo[ge];
     rgb = a.bgeo[K + ge];
@@ -214,14 +217,19 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_ker
   // Own and neighbour traces of face f of a tile at the pairs of this lane:
   // two faces in flight, face f in slot f % 2.
   double uM[2][NG], uP[2][NG];
+  // (The pair indices pass through an empty asm, so that the compiler
+  // derives the LDS addresses from them here instead of keeping one address
+  // register per face and pair live across the tile loop.)
   auto gather = [&](const dgk::Tile& tl, const FaceRec* sR, int f) {
     const double* ut = a.u + tl.vbase;
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
-      const int ce = pc[j] < tl.count ? pc[j] : tl.count - 1;
-      const FaceRec& r = sR[f * 16 + pc[j]];
-      uP[f & 1][j] = a.u[r.nb + sFN[r.fnrow + pq[j]]];
-      uM[f & 1][j] = ut[ce * NP + sFM[f * NFP + pq[j]]];
+      int jc = pc[j], jq = pq[j];
+      asm volatile("" : "+v"(jc), "+v"(jq));
+      const int ce = jc < tl.count ? jc : tl.count - 1;
+      const FaceRec& r = sR[f * 16 + jc];
+      uP[f & 1][j] = a.u[r.nb + sFN[r.fnrow + jq]];
+      uM[f & 1][j] = ut[ce * NP + sFM[f * NFP + jq]];
     }
   };
   // mode 1 (rhs + LIFT g) is the update with a0 = 0, a1 = dt = 1 and u = 0
@@ -252,11 +260,12 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_ker
     // no branches.
     const int nbytes = cnt * NP * 8;
     const __amdgpu_buffer_rsrc_t rsR = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(a.rhs + tile.vbase), 0, nbytes, 0x00020000);
+        (void*)((const char*)a.rhs + tile.vbase * (R32 ? 4 : 8)), 0, R32 ? nbytes / 2 : nbytes,
+        0x00020000);
     const __amdgpu_buffer_rsrc_t rsU = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(a.u + tile.vbase), 0, a.mode == 0 ? nbytes : 0, 0x00020000);
     const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(a0 != 0.0 ? a.u0 + tile.vbase : a.rhs), 0, a0 != 0.0 ? nbytes : 0, 0x00020000);
+        (void*)(a0 != 0.0 ? a.u0 + tile.vbase : a.u), 0, a0 != 0.0 ? nbytes : 0, 0x00020000);
     const __amdgpu_buffer_rsrc_t rsO = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(a.out + tile.vbase), 0, nbytes, 0x00020000);
     // Byte offset of the value lane updates for row tile tt, slot j (element
@@ -265,12 +274,17 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_ker
       const int e = 4 * j + (lane >> 4), i = 16 * tt + (lane & 15);
       return i < NP ? (e * NP + i) * 8 : 0x40000000;
     };
-    double pr[2][4], pu[2][4], p0[2][4];
+    typename std::conditional<R32, float, double>::type pr[2][4];
+    double pu[2][4], p0[2][4];
     auto preload = [&](int tt) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int off = upd_off(tt, j);
-        pr[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsR, off, 0, 0));
+        if constexpr (R32)
+          pr[tt & 1][j] =
+              __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsR, off >> 1, 0, 0));
+        else
+          pr[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsR, off, 0, 0));
         pu[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsU, off, 0, 0));
         p0[tt & 1][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs0, off, 0, 2));
       }
@@ -371,7 +385,7 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_ker
       // out = a0 u0 + a1 (u + dt (rhs + LIFT g))
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double l = pr[tt & 1][j] + sT[(4 * j + (lane >> 4)) * 17 + (lane & 15)];
+        const double l = (double)pr[tt & 1][j] + sT[(4 * j + (lane >> 4)) * 17 + (lane & 15)];
         const double o = fma(a0, p0[tt & 1][j], a1 * fma(dt, l, pu[tt & 1][j]));
         const bool ok = 4 * j + (lane >> 4) < cnt && 16 * tt + (lane & 15) < NP;
         lmax = fmax(lmax, ok ? fabs(o) : 0.0);
@@ -449,31 +463,32 @@ std::vector<unsigned char> lift_fragments(const std::vector<double>& LIFT, int N
   return buf;
 }
 
-template <int N, int PREC>
+template <int N, int PREC, bool R32>
 int surface_blocks(dgk_dgmesh m) {
   int per_cu = 0;
-  DGK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, surface_update_kernel<N, PREC>,
+  DGK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, surface_update_kernel<N, PREC, R32>,
                                                        64 * SCfg<N, PREC>::W, 0));
   if (per_cu < 1) per_cu = 1;
   m->split_blocks = m->dev->cu_count * per_cu;
   return DGK_OK;
 }
 
-template <int N, int PREC>
+template <int N, int PREC, bool R32>
 int surface_launch(dgk_dgmesh m, const SurfArgs& a) {
   const int W = SCfg<N, PREC>::W;
   int grid = (a.ntiles + W - 1) / W;
   if (grid > m->split_blocks) grid = m->split_blocks;
   if (grid < 1) return DGK_OK;
-  hipLaunchKernelGGL((surface_update_kernel<N, PREC>), dim3(grid), dim3(64 * W), 0, m->dev->stream,
-                     a);
+  hipLaunchKernelGGL((surface_update_kernel<N, PREC, R32>), dim3(grid), dim3(64 * W), 0,
+                     m->dev->stream, a);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
 
 template <int N, int PREC>
 int surface_w(dgk_dgmesh m, const SurfArgs* a) {
-  return a ? surface_launch<N, PREC>(m, *a) : surface_blocks<N, PREC>(m);
+  if (m->rhs32) return a ? surface_launch<N, PREC, true>(m, *a) : surface_blocks<N, PREC, true>(m);
+  return a ? surface_launch<N, PREC, false>(m, *a) : surface_blocks<N, PREC, false>(m);
 }
 
 template <int N>
@@ -508,6 +523,7 @@ int dgk::split_prepare(dgk_dgmesh m) {
   int rc = dgk_deriv_plan_create(m->dev, m->N, m->Dr_h.data(), m->Ds_h.data(), m->Dt_h.data(),
                                  m->r_h.data(), m->s_h.data(), m->t_h.data(), m->prec, &m->plan);
   if (rc != DGK_OK) return rc;
+  m->rhs32 = dgk::deriv_plan_divergence_f32(m->plan) ? 1 : 0;
   const size_t state_bytes = (size_t)(m->state_off.empty() ? 0 : m->state_off.back()) * 8 + 16;
   rc = dgk_malloc(m->dev, state_bytes, nullptr, (void**)&m->rhs);
   if (rc != DGK_OK) return rc;
