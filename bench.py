@@ -351,6 +351,8 @@ def bench_c4_stage(a, dev) -> dict:
     for name in precs:
         solver = dg.BurgersSolver(dev, st, cfl=0.3, hmin=hmin)
         hmin = solver.hmin
+        if a.tile_order >= 0:
+            solver.mesh.set_tile_order(bool(a.tile_order))
         solver.mesh.set_precision(STAGE_PRECS[name])
         solver.set_state(u0)
         _warm(dev, lambda: solver.step(1), min_calls=1)  # outside the timed run, restarted below
@@ -627,6 +629,9 @@ def main(argv=None) -> int:
     ap.add_argument("--variant", type=int, default=-1,
                     help="stage kernel (0 auto, 1 VALU, 2 MFMA pipelined, 3 split, 4 MFMA)")
     ap.add_argument("--xcd", type=int, default=-1, help="1: XCD-swept tile order")
+    ap.add_argument("--tile-order", type=int, default=-1,
+                    help="1: tiles of all partitions in the Morton order of their centroids, "
+                         "0: partition by partition (library default)")
     ap.add_argument("--morton", dest="morton", action="store_true", default=None,
                     help="renumber elements in Morton order within each partition "
                          "(default for the P=3 stage, off for --mode c4)")
@@ -696,12 +701,16 @@ def main(argv=None) -> int:
         solver.mesh.set_variant(a.variant)
     if a.xcd >= 0:
         solver.mesh.set_order(bool(a.xcd))
+    if a.tile_order >= 0:
+        solver.mesh.set_tile_order(bool(a.tile_order))
     if a.tile or a.bpc:
         solver.mesh.set_tile(a.tile or 64, a.bpc)
-    if a.variant >= 0 or a.tile or a.bpc or a.xcd >= 0:
+    if a.variant >= 0 or a.tile or a.bpc or a.xcd >= 0 or a.tile_order >= 0:
         e = dict(solver.tuned or {})
         e.update({k: v for k, v in (("variant", a.variant), ("tile", a.tile), ("bpc", a.bpc),
                                      ("xcd", a.xcd)) if v not in (-1, 0) or k == "variant" and v == 0})
+        if a.tile_order >= 0:
+            e["spatial"] = a.tile_order
         solver.tuned = e
     solver.set_state(u0)
     del u0
@@ -773,6 +782,7 @@ def main(argv=None) -> int:
     traffic_src = read_profile_key(prof, "stage_source_sha256")
 
     kname = stage_kernel_name(a.order, solver)
+    walk = "morton across partitions" if (solver.tuned or {}).get("spatial") else "partition order"
     del solver
     p6 = None
     if not a.no_p6:
@@ -807,7 +817,8 @@ def main(argv=None) -> int:
                                   else ""),
                    "tets": K, "order": a.order, "Np": Np, "partitions": st.npart,
                    "kpart_max": int(st.Kp.max()), "stages_timed": 3 * a.steps,
-                   "element_order": "morton" if morton else "generator"},
+                   "element_order": "morton" if morton else "generator",
+                   "tile_walk": walk},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": {"profile": f"profiles/{prof}",

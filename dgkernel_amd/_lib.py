@@ -117,6 +117,7 @@ _SIGS = {
     "dgk_burgers_set_variant": (_i32, [_vp, _i32]),
     "dgk_burgers_set_precision": (_i32, [_vp, _i32]),
     "dgk_burgers_set_order": (_i32, [_vp, _i32]),
+    "dgk_dgmesh_set_tile_order": (_i32, [_vp, _i32]),
     "dgk_av_create": (_i32, [_vp, _pf64, C.POINTER(AVParams), C.POINTER(_vp)]),
     "dgk_av_destroy": (_i32, [_vp]),
     "dgk_av_eps_dev": (_i32, [_vp, C.POINTER(_vp)]),

@@ -7,15 +7,17 @@ warm-up launches first) and records the fastest in a JSON table,
 tuned entry for its order; BurgersSolver applies it when the table exists.
 
 Usage: python -m dgkernel_amd.autotune [--orders 1,2,3,4] [--tets 600000]
-                                        [--reps 12] [--out PATH] [--morton] [--merge]
+                                        [--reps 12] [--parts 8] [--out PATH] [--morton]
+                                        [--merge]
 
 Candidates (variant numbers of dgk_burgers_set_variant):
   1 VALU stage (one element per lane; 64/128/256 elements per block),
   2 pipelined FP64-MFMA stage, 4 MFMA stage without the load pipeline,
   5 deep-pipeline MFMA stage and 6 register-operator MFMA stage (N <= 3),
   each MFMA kernel with 2 or 4 waves per block (32/64 elements), 1-4
-  blocks per CU, and grid-stride or XCD-swept tile order. Orders 5 and 6
-  run the split stage only (one candidate).
+  blocks per CU, and three tile walks: grid stride, XCD-swept in partition
+  order, XCD-swept along the Morton curve of the tile centroids of all
+  partitions. Orders 5 and 6 run the split stage only (one candidate).
 """
 from __future__ import annotations
 
@@ -32,17 +34,21 @@ TABLE = Path(__file__).resolve().parent / "tuned_stage.json"
 
 def candidates(N: int, quick: bool = False) -> list[dict]:
     if N > 4:
-        return [{"variant": 0, "tile": 0, "bpc": 0, "xcd": 0}]
+        return [{"variant": 0, "tile": 0, "bpc": 0, "xcd": 0, "spatial": 0}]
     out = []
     mfma = [2, 4] + ([5, 6] if N <= 3 else [])
+    # tile walks: grid stride, XCD-swept partition order, XCD-swept spatial
+    # (Morton) order of the tiles of all partitions
+    walks = ((0, 0),) if quick else ((0, 0), (1, 0), (1, 1))
     for v in mfma:
         for tile in (32, 64):
             for bpc in ((0,) if quick else (1, 2, 3, 4)):
-                for xcd in ((0,) if quick else (0, 1)):
-                    out.append({"variant": v, "tile": tile, "bpc": bpc, "xcd": xcd})
+                for xcd, spatial in walks:
+                    out.append({"variant": v, "tile": tile, "bpc": bpc, "xcd": xcd,
+                                "spatial": spatial})
     for tile in (64, 128, 256):
         for bpc in ((0,) if quick else (0, 2)):
-            out.append({"variant": 1, "tile": tile, "bpc": bpc, "xcd": 0})
+            out.append({"variant": 1, "tile": tile, "bpc": bpc, "xcd": 0, "spatial": 0})
     return out
 
 
@@ -51,6 +57,7 @@ def configure(mesh, cfg: dict) -> None:
     family the tile shape refers to)."""
     mesh.set_variant(int(cfg["variant"]))
     mesh.set_order(bool(cfg.get("xcd", 0)))
+    mesh.set_tile_order(bool(cfg.get("spatial", 0)))
     if cfg.get("tile", 0) or cfg.get("bpc", 0):
         mesh.set_tile(int(cfg.get("tile", 0)), int(cfg.get("bpc", 0)))
 
@@ -94,20 +101,24 @@ def time_stage(solver, reps: int) -> float:
 
 
 def tune(dev, orders, tets: int = 600_000, reps: int = 12, quick: bool = False, log=print,
-         morton: bool = False) -> dict:
+         morton: bool = False, parts: int = 8) -> dict:
     from . import dg
     from . import mesh as M
     table = {"device": dev.name, "tets_target": tets, "reps": reps, "entries": {}, "measured": {}}
     for N in orders:
         n = max(2, int(round((tets / 6) ** (1 / 3))))
         m = M.cube_mesh(n, n, n, periodic=True)
-        Kp = M.slab_partition(m.K, 8, 0.4, seed=1)
+        Kp = M.slab_partition(m.K, parts, 0.4, seed=1)
         if morton:
             m = M.morton_within_partitions(m, Kp)
         st = dg.make_setup(m, N, Kp=Kp)
         x, y, z = st.node_coords()
         u0 = 0.5 + 0.4 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
         solver = dg.BurgersSolver(dev, st, cfl=0.3, tuned=False)
+        # clocks up before the first candidate: at least ~100 ms of stages
+        solver.set_state(u0)
+        warm = time_stage(solver, 2)
+        solver.step(max(1, int(100.0 / max(3 * warm, 1e-3))))
         res = []
         for cfg in candidates(N, quick):
             configure(solver.mesh, cfg)
@@ -116,7 +127,7 @@ def tune(dev, orders, tets: int = 600_000, reps: int = 12, quick: bool = False, 
             res.append(dict(cfg, ms=ms))
             log(f"N={N} K={m.K} {cfg}: {ms:.4f} ms/stage")
         best = min(res, key=lambda r: r["ms"])
-        table["entries"][str(N)] = dict(best, K=m.K, morton=morton,
+        table["entries"][str(N)] = dict(best, K=m.K, morton=morton, parts=parts,
                                         dof_stage_per_s=m.K * st.ref.Np / (best["ms"] * 1e-3))
         table["measured"][str(N)] = res
         del solver
@@ -132,6 +143,8 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default=str(TABLE))
     ap.add_argument("--morton", action="store_true",
                     help="tune on a mesh renumbered in Morton order within partitions")
+    ap.add_argument("--parts", type=int, default=8,
+                    help="slab partitions of the tuning mesh (+-40%% imbalance)")
     ap.add_argument("--merge", action="store_true",
                     help="update only the tuned orders of an existing table at --out")
     a = ap.parse_args(argv)
@@ -139,7 +152,8 @@ def main(argv=None) -> int:
     dev = _lib.Device()
     orders = [int(v) for v in a.orders.split(",") if v]
     table = tune(dev, orders, a.tets, a.reps, a.quick,
-                 log=lambda s: print(s, file=sys.stderr, flush=True), morton=a.morton)
+                 log=lambda s: print(s, file=sys.stderr, flush=True), morton=a.morton,
+                 parts=a.parts)
     old = load_table(Path(a.out)) if a.merge else None
     if old:
         for key in ("entries", "measured"):

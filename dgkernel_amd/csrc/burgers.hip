@@ -24,6 +24,7 @@
 //  * the RK combination is written back through LDS with unit stride;
 //  * one persistent launch walks a flat list of tiles covering every
 //    partition, so uneven partition sizes cost no padding.
+#include <algorithm>
 #include <cmath>
 
 #include "dgmesh_internal.h"
@@ -1113,6 +1114,14 @@ int build_tile_list(dgk_dgmesh m, int per_tile, Tile** out, int* nout, int* nful
     }
     g += m->Kp[p];
   }
+  if (m->tile_order == 1 && !m->ekey.empty()) {
+    // Full tiles of all partitions along the Morton curve of the centroid of
+    // each tile's middle element: tiles that run at the same time on one XCD
+    // then cover a compact region, across partition boundaries.
+    std::stable_sort(full.begin(), full.end(), [&](const Tile& x, const Tile& y) {
+      return m->ekey[x.gbase + x.count / 2] < m->ekey[y.gbase + y.count / 2];
+    });
+  }
   if (nfull) *nfull = (int)full.size();
   full.insert(full.end(), part.begin(), part.end());
   if (*out) dgk_free(m->dev, *out);
@@ -1327,6 +1336,38 @@ extern "C" int dgk_dgmesh_create(dgk_device dev, const dgk_dgmesh_desc* d, dgk_d
     }
     codes[g] = packed;
   }
+  // Morton keys of the element centroids (10 bits per axis over the mesh's
+  // bounding box), for the spatial tile order.
+  {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    std::vector<float> cen((size_t)3 * K);
+    for (int64_t g = 0; g < K; ++g)
+      for (int a = 0; a < 3; ++a) {
+        const double* v = d->vtx + g * 12;
+        const double c = 0.25 * (v[a] + v[3 + a] + v[6 + a] + v[9 + a]);
+        cen[(size_t)3 * g + a] = (float)c;
+        lo[a] = std::min(lo[a], c);
+        hi[a] = std::max(hi[a], c);
+      }
+    auto spread = [](uint32_t x) {  // bit i of x to bit 3 i
+      x &= 0x3ff;
+      x = (x | (x << 16)) & 0x30000ff;
+      x = (x | (x << 8)) & 0x300f00f;
+      x = (x | (x << 4)) & 0x30c30c3;
+      x = (x | (x << 2)) & 0x9249249;
+      return x;
+    };
+    m->ekey.resize((size_t)K);
+    for (int64_t g = 0; g < K; ++g) {
+      uint32_t q[3];
+      for (int a = 0; a < 3; ++a) {
+        const double span = hi[a] - lo[a];
+        const double t = span > 0 ? (cen[(size_t)3 * g + a] - lo[a]) / span : 0.0;
+        q[a] = (uint32_t)std::min(1023.0, std::max(0.0, t * 1023.0));
+      }
+      m->ekey[(size_t)g] = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
+    }
+  }
   std::vector<uint8_t> fn((size_t)4 * 24 * Nfp);
   for (size_t i = 0; i < fn.size(); ++i) {
     int v = d->face_nodes[i];
@@ -1499,6 +1540,15 @@ extern "C" int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked) {
   DGK_REQUIRE(m, "dgk_burgers_set_order: NULL mesh");
   m->chunked = xcd_chunked ? 1 : 0;
   return DGK_OK;
+}
+
+extern "C" int dgk_dgmesh_set_tile_order(dgk_dgmesh m, int order) {
+  dgk::clear_error();
+  DGK_REQUIRE(m, "dgk_dgmesh_set_tile_order: NULL mesh");
+  DGK_REQUIRE(order == 0 || order == 1, "dgk_dgmesh_set_tile_order: order %d", order);
+  if (order == m->tile_order) return DGK_OK;
+  m->tile_order = order;
+  return build_tiles(m);
 }
 
 extern "C" int dgk_burgers_set_variant(dgk_dgmesh m, int variant) {

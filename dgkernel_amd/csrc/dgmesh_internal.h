@@ -81,6 +81,8 @@ struct dgk_dgmesh_s {
   int tbm = 64;               // MFMA block: 16 elements per wave, 1/2/4 waves per block
   int tb = 64, blocks_per_cu = 0, chunked = 0;
   int variant = 0;            // 0 auto, 1 VALU, 2 MFMA
+  int tile_order = 0;         // 0 partition order, 1 Morton order of tile centroids
+  std::vector<uint32_t> ekey; // per element: 30-bit Morton key of its centroid
   unsigned long long* scratch = nullptr;  // reduction slot for synchronous helpers
   // Host copies of the reference operators (row-major) and nodes.
   std::vector<double> Dr_h, Ds_h, Dt_h, LIFT_h, r_h, s_h, t_h;

@@ -169,6 +169,11 @@ class DGMesh:
     def set_order(self, xcd_chunked: bool) -> None:
         _lib.call("dgk_burgers_set_order", self.h, 1 if xcd_chunked else 0)
 
+    def set_tile_order(self, spatial: bool) -> None:
+        """Walk the 16-element tiles partition by partition (False) or, across
+        partitions, along the Morton curve of their centroids (True)."""
+        _lib.call("dgk_dgmesh_set_tile_order", self.h, 1 if spatial else 0)
+
     def trace_plus(self, u: _lib.DeviceBuffer, out: _lib.DeviceBuffer) -> None:
         _lib.call("dgk_dgmesh_trace_plus", self.h, u.ptr, out.ptr)
 

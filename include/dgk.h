@@ -336,6 +336,14 @@ int dgk_burgers_set_variant(dgk_dgmesh m, int variant);
  * tiles (blocks renumbered around the round-robin XCD dispatch), 0 (default)
  * walks the tile list with a grid stride. */
 int dgk_burgers_set_order(dgk_dgmesh m, int xcd_chunked);
+/* Order of the 16-element tiles that every persistent launch on the mesh
+ * walks (stage, split stage, viscosity, sensors): 0 (default) partition
+ * by partition, 1 the full tiles of all partitions along the Morton curve
+ * of their centroids (partial tiles last). A scheduling choice only:
+ * results are identical. No counterpart in the reference, whose kernels
+ * loop over partitions (runner/builder/builder.go:172-277). Call outside
+ * graph capture (it reallocates the tile list). */
+int dgk_dgmesh_set_tile_order(dgk_dgmesh m, int order);
 /* Precision of the Dr/Ds/Dt and LIFT contractions in the stage (DGK_PREC_*,
  * default DGK_PREC_F64); OR-ing DGK_PREC_LIFT_F64 keeps the LIFT product in
  * FP64 while Dr/Ds/Dt run in the given precision (DGK_PREC_I8X6 always

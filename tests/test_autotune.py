@@ -27,6 +27,9 @@ class _FakeMesh:
     def set_order(self, x):
         self.calls.append(("order", x))
 
+    def set_tile_order(self, x):
+        self.calls.append(("walk", x))
+
     def set_tile(self, t, b):
         self.calls.append(("tile", t, b))
 
@@ -34,7 +37,7 @@ class _FakeMesh:
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_candidates_cover_the_kernel_families(N):
     cs = autotune.candidates(N)
-    assert cs and all(set(c) == {"variant", "tile", "bpc", "xcd"} for c in cs)
+    assert cs and all(set(c) == {"variant", "tile", "bpc", "xcd", "spatial"} for c in cs)
     vs = {c["variant"] for c in cs}
     if N <= 3:
         assert vs == {1, 2, 4, 5, 6}
@@ -47,12 +50,13 @@ def test_candidates_cover_the_kernel_families(N):
 
 def test_configure_sets_variant_before_tile_and_apply_uses_table():
     m = _FakeMesh(3)
-    autotune.configure(m, {"variant": 5, "tile": 32, "bpc": 2, "xcd": 1})
-    assert m.calls == [("variant", 5), ("order", True), ("tile", 32, 2)]
+    autotune.configure(m, {"variant": 5, "tile": 32, "bpc": 2, "xcd": 1, "spatial": 1})
+    assert m.calls == [("variant", 5), ("order", True), ("walk", True), ("tile", 32, 2)]
+    # entries written before the tile walk was tuned keep the partition order
     table = {"entries": {"3": {"variant": 2, "tile": 64, "bpc": 0, "xcd": 0, "ms": 1.0}}}
     m2 = _FakeMesh(3)
     assert autotune.apply(m2, table)["variant"] == 2
-    assert m2.calls == [("variant", 2), ("order", False), ("tile", 64, 0)]
+    assert m2.calls == [("variant", 2), ("order", False), ("walk", False), ("tile", 64, 0)]
     m3 = _FakeMesh(4)
     assert autotune.apply(m3, table) is None and m3.calls == []
     assert autotune.entry_for({"entries": {"3": {"variant": 2}}}, 3) is None  # incomplete entry

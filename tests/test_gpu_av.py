@@ -71,6 +71,30 @@ def test_viscous_rhs_vs_oracle(device, N, n, parts):
     assert np.max(np.abs(tot - want_t)) <= tol * np.max(np.abs(want_t))
 
 
+@pytest.mark.parametrize("N", [2, 4])
+def test_viscous_operator_and_sensor_in_spatial_tile_order(device, N):
+    """The viscous operator and the sensor walk the same tile list as the
+    stage: in the Morton order of the tile centroids they still match the
+    oracle."""
+    m = M.cube_mesh(4, 4, 3, periodic=True, jitter=0.06, seed=40 + N)
+    st = dg.make_setup(m, N, Kp=[100, 1, m.K - 101])
+    o = O.Setup.from_mesh(N, m)
+    p = AV.AVParams.for_order(N, kappa=1.5)
+    u = smooth_u0(st, amp=0.8, noise=1e-2)
+    eps = M.uniform(N, 5, m.K, 0.0, 0.02)
+    want = o.av_rhs(u, eps, 1.0) + o.burgers_rhs(u)
+    s = AV.AVSolver(device, st, p, "pp")
+    s.mesh.set_tile_order(True)
+    s.set_eps(eps)
+    got = s.rhs_total(u)
+    tol = 1e-11 + 50 * _spread(o)
+    assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want))
+    eps_pp = o.pp_sensor(u, p.s0, p.kappa, p.ceps)[0]
+    s.u.copy_from(st.to_pooled(u))
+    s.sense()
+    assert np.max(np.abs(s.eps() - eps_pp)) <= 1e-10 * max(eps_pp.max(), 1e-30)
+
+
 @pytest.mark.parametrize("N,parts", [(2, [162]), (4, [60, 102])])
 def test_viscous_rhs_with_boundary_faces_vs_oracle(device, N, parts):
     """Non-periodic cube: boundary faces are their own neighbours (zero jump

@@ -407,6 +407,27 @@ def test_tile_shapes_agree(device, variant, tb, bpc, chunk):
     assert np.array_equal(ref.state(), s.state())
 
 
+@pytest.mark.parametrize("N,variant,prec", [(3, 1, 0), (3, 2, 0), (3, 5, 0), (3, 6, 0), (4, 2, 0),
+                                            (5, 3, 0), (5, 3, 2), (5, 3, 1 | 16)])
+def test_spatial_tile_order_agrees(device, N, variant, prec):
+    """Walking the tiles along the Morton curve of their centroids, across
+    partitions, gives the same state bit for bit as the partition order."""
+    m = M.cube_mesh(7, 6, 5, periodic=True)
+    st = dg.make_setup(m, N, Kp=M.slab_partition(m.K, 4, 0.4, seed=N))
+    u0 = smooth_u0(st)
+    out = []
+    for spatial in (False, True):
+        s = dg.BurgersSolver(device, st)
+        s.mesh.set_variant(variant)
+        s.mesh.set_precision(prec)
+        s.mesh.set_order(True)
+        s.mesh.set_tile_order(spatial)
+        s.set_state(u0)
+        s.step(3)
+        out.append(s.state())
+    assert np.array_equal(out[0], out[1])
+
+
 def test_mfma_and_valu_stage_agree(device):
     m = M.cube_mesh(6, 6, 6, periodic=True)
     st = dg.make_setup(m, 3, Kp=M.slab_partition(m.K, 5, 0.4, seed=3))
