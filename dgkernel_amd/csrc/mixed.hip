@@ -29,7 +29,8 @@
 //      divergence, in FP32 for the other plans below FP64 (FP32 outputs,
 //      the divergence of the split Burgers stage included).
 //
-//   F64:     v_mfma_f64_16x16x4_f64.
+//   F64:     v_mfma_f64_4x4x4_4b_f64 (four per 16x16x4 step, the operator as
+//            the B operand).
 //   F32:     v_mfma_f32_16x16x4_f32 on FP32-rounded operator and w.
 //   BF16X3:  v_mfma_f32_16x16x32_bf16 on two-part bf16 splits,
 //            hi*hi + hi*lo + lo*hi (Henry, Tang & Heinecke, ARITH 2019);
@@ -460,22 +461,53 @@ deriv_affine_kernel(AffineArgs a) {
     for (int j = 0; j < TPW; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[j][r] = 0;
-    if constexpr (PREC == kF64 || PREC == kF32) {
-      using BT = typename std::conditional<PREC == kF64, double, float>::type;
-      const BT* ub;
-      if constexpr (PREC == kF64) ub = sU[bu] + c * NPU;
-      else ub = sWf[bu] + c * NPF;
+    if constexpr (PREC == kF64) {
+      // FP64 on v_mfma_f64_4x4x4_4b, which issues about twice the FP64 rate
+      // of v_mfma_f64_16x16x4 on gfx950 (72 against 36-47 TFLOP/s in a
+      // 1-2 wave per SIMD loop). Its operand layout (checked on gfx950):
+      // block j of four is lanes 16 k + 4 j + i, with A[i][k] and B[k][i] at
+      // that lane and D[i][n] at lane 16 i + 4 j + n. The operator fragment
+      // (lane l: row 16 tt + l%16, column 4 s + l/16) is the B operand
+      // unchanged, block j taking operator rows 4 j..4 j+3; instruction q
+      // takes elements 4 q..4 q+3 as A (lane l: element 4 q + l%4, node
+      // 4 s + l/16), and leaves row 16 tt + l%16 of element 4 q + l/16.
+      double a4[TPW][4];
+#pragma unroll
+      for (int j = 0; j < TPW; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a4[j][q] = 0.0;
+      const double* ua = sU[bu] + (lane & 3) * NPU + (lane >> 4);
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const bool kin = 4 * s + (lane >> 4) < NP;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double a = kin ? ua[4 * q * NPU + 4 * s] : 0.0;
+#pragma unroll
+          for (int j = 0; j < TPW; ++j)
+            a4[j][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, fa[j][s].v, a4[j][q], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) {
+        const int tt = w * TPW + j;
+        if (tt < MT) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            sR[bu][(4 * q + (lane >> 4)) * ROWSP + 16 * tt + (lane & 15)] = a4[j][q];
+        }
+      }
+      return;
+    } else if constexpr (PREC == kF32) {
+      using BT = float;
+      const BT* ub = sWf[bu] + c * NPF;
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
         const int k = 4 * s + g;
         const BT b = k < NP ? ub[k] : (BT)0;
 #pragma unroll
-        for (int j = 0; j < TPW; ++j) {
-          if constexpr (PREC == kF64)
-            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[j][s].v, b, acc[j], 0, 0, 0);
-          else
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[j][s].v, b, acc[j], 0, 0, 0);
-        }
+        for (int j = 0; j < TPW; ++j)
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[j][s].v, b, acc[j], 0, 0, 0);
       }
     } else if constexpr (PREC == kI8) {
       // Per row tile: kDiag int32 accumulators, one per pair weight; B digits
@@ -572,17 +604,13 @@ deriv_affine_kernel(AffineArgs a) {
         }
       }
     }
-    // Results to LDS, [element][row]. FP64 D lanes hold rows g + 4r of a tile,
-    // FP32 D lanes hold rows 4g + r.
+    // Results to LDS, [element][row]; FP32 D lanes hold rows 4g + r.
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
       const int tt = w * TPW + j;
       if (tt < MT) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = PREC == kF64 ? g + 4 * r : 4 * g + r;
-          sR[bu][c * ROWSP + 16 * tt + row] = acc[j][r];
-        }
+        for (int r = 0; r < 4; ++r) sR[bu][c * ROWSP + 16 * tt + 4 * g + r] = acc[j][r];
       }
     }
   };
