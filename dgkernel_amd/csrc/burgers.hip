@@ -323,6 +323,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One k = 4 step of a 16-row operator tile whose only rows are its first
+// four (the last LIFT tile at N = 1, 3, 4) as one v_mfma_f64_4x4x4_4b
+// instead of a v_mfma_f64_16x16x4_f64, which on gfx950 issues about eight
+// times slower. The 4x4x4 operand layout (checked on gfx950): block j of the
+// four is lanes 16 k + 4 j + i, with A[i][k] and B[k][i] at that lane and
+// D[i][n] at lane 16 i + 4 j + n. With block j as elements 4 j..4 j+3, B is
+// the 16x16x4 B operand unchanged, lane l reads A row l%4, column l/16 from
+// the tile's 16x16x4 fragment table, and the result lands where register 0
+// of the 16x16x4 result would (row l/16, element l%16); registers 1-3, rows
+// past the operator, stay zero.
+__device__ __forceinline__ void mfma_f64_rows4(double4_t& acc, const double* frag, int lane,
+                                               double b) {
+  acc[0] = __builtin_amdgcn_mfma_f64_4x4x4f64(frag[16 * (lane >> 4) + (lane & 3)], b, acc[0], 0,
+                                              0, 0);
+}
+
 // Each wave works through its own sequence of 16-element tiles; the waves of a
 // block only share the operator fragments (written once, one block barrier).
 template <int N, int WPB>
@@ -916,9 +932,13 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
       const double al = fmax(fabs(uM), fabs(up[s]));
       const double gm = (uM - up[s]) * fma(fa, uM + up[s], -fb * al);
 #pragma unroll
-      for (int tt = 0; tt < MTL; ++tt)
-        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,
-                                                         accL[tt], 0, 0, 0);
+      for (int tt = 0; tt < MTL; ++tt) {
+        if (NP - 16 * tt <= 4)
+          mfma_f64_rows4(accL[tt], sAl + (tt * KSL + s) * 64, lane, gm);
+        else
+          accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,
+                                                           accL[tt], 0, 0, 0);
+      }
     }
     __builtin_amdgcn_s_setprio(0);
 #pragma unroll
