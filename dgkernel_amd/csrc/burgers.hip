@@ -483,9 +483,13 @@ __global__ void __launch_bounds__(64 * WPB) burgers_stage_mfma_kernel(StageParam
         gm = fsf * (0.5 * (fM - fP) * snf + 0.5 * al * fabs(snf) * (up - uM));
       }
 #pragma unroll
-      for (int tt = 0; tt < MTL; ++tt)
-        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,
-                                                         accL[tt], 0, 0, 0);
+      for (int tt = 0; tt < MTL; ++tt) {
+        if (NP - 16 * tt <= 4)
+          mfma_f64_rows4(accL[tt], sAl + (tt * KSL + s) * 64, lane, gm);
+        else
+          accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAl[(tt * KSL + s) * 64 + lane], gm,
+                                                           accL[tt], 0, 0, 0);
+      }
     }
     // Rows i = gq + 4q of this lane's element: Dr F at row i, Ds F at NPR + i,
     // Dt F at 2 NPR + i, LIFT G at i; tile and slot are compile-time.
@@ -631,10 +635,15 @@ __device__ __forceinline__ void pipe_tiles(const StageParams& p, const double* s
   const int c = lane & 15, gq = lane >> 4;  // element column, face of this lane
   const int64_t K = p.K;
   constexpr int NFR = REGA ? MTV * KSV + MTL * KSL : 1;
+  // LIFT fragments of a last tile with four rows or fewer feed a 4x4x4
+  // product (mfma_f64_rows4), which reads lane 16 (l/16) + l%4 of the table.
+  constexpr bool L4 = NP - 16 * (MTL - 1) <= 4;
   double ra[NFR];
   if (REGA) {
 #pragma unroll
-    for (int q = 0; q < NFR; ++q) ra[q] = sA[q * 64 + lane];
+    for (int q = 0; q < NFR; ++q)
+      ra[q] = sA[q * 64 + (L4 && q >= MTV * KSV + (MTL - 1) * KSL ? 16 * (lane >> 4) + (lane & 3)
+                                                                   : lane)];
   }
   const double* sAv = sA;
   const double* sAl = sA + MTV * KSV * 64;
@@ -731,10 +740,19 @@ __device__ __forceinline__ void pipe_tiles(const StageParams& p, const double* s
       // Fscale (n.f(u-) - n.f*) with f(u) = u^2/2 (1,1,1), factored
       const double gm = (uM - up) * fma(fa, uM + up, -fb * al);
 #pragma unroll
-      for (int tt = 0; tt < MTL; ++tt)
-        accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            REGA ? ra[(MTV * KSV + tt * KSL + s) % NFR] : sAl[(tt * KSL + s) * 64 + lane], gm,
-                                                         accL[tt], 0, 0, 0);
+      for (int tt = 0; tt < MTL; ++tt) {
+        if (L4 && tt == MTL - 1) {
+          if (REGA)
+            accL[tt][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(ra[(MTV * KSV + tt * KSL + s) % NFR],
+                                                             gm, accL[tt][0], 0, 0, 0);
+          else
+            mfma_f64_rows4(accL[tt], sAl + (tt * KSL + s) * 64, lane, gm);
+        } else {
+          accL[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              REGA ? ra[(MTV * KSV + tt * KSL + s) % NFR] : sAl[(tt * KSL + s) * 64 + lane], gm,
+              accL[tt], 0, 0, 0);
+        }
+      }
     }
     __builtin_amdgcn_s_setprio(0);
 #pragma unroll

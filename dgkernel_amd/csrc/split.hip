@@ -357,7 +357,21 @@ __global__ void __launch_bounds__((64 * SCfg<N, PREC>::W), 2) surface_update_ker
         for (int s = 0; s < KSF; ++s) {
           const size_t q = (((size_t)tt * 4 + f) * KSF + s) * 64 + lane;
           if constexpr (PREC == kF64) {
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(((const double*)sA)[q], b[f][s], acc, 0, 0, 0);
+            if (NP - 16 * tt <= 4) {
+              // a last row tile of four rows or fewer: one v_mfma_f64_4x4x4_4b
+              // (about eight times cheaper to issue on gfx950). Its block j is
+              // lanes 16 k + 4 j + i (A[i][k], B[k][i]; D[i][n] at lane
+              // 16 i + 4 j + n): B unchanged, A row l%4 from lane
+              // 16 (l/16) + l%4 of the table, result in register 0's place
+              int lp = lane;
+              asm volatile("" : "+v"(lp));  // (derived here, not held across the tile)
+              acc[0] = __builtin_amdgcn_mfma_f64_4x4x4f64(
+                  ((const double*)sA)[q - lane + 16 * (lp >> 4) + (lp & 3)], b[f][s], acc[0], 0, 0,
+                  0);
+            } else {
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(((const double*)sA)[q], b[f][s], acc, 0, 0,
+                                                         0);
+            }
           } else if constexpr (PREC == kF32) {
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(((const float*)sA)[q], b[f][s], acc, 0, 0, 0);
           } else {

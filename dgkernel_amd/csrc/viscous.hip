@@ -187,6 +187,24 @@ __device__ __forceinline__ void unstage4(double* __restrict__ dst, const Tile& m
 // ---------------------------------------------------------------------------
 typedef double double4v __attribute__((ext_vector_type(4)));
 
+// One k = 4 step of a 16-row operator tile, frag = the tile's 64-entry
+// fragment table in the 16x16x4 FP64 A-operand lane order. A tile whose only
+// rows are its first four (the last row tile at Np = 35, 20, 4, 84) runs as
+// one v_mfma_f64_4x4x4_4b, which issues about eight times faster than the
+// 16x16x4 on gfx950. Its block j is lanes 16 k + 4 j + i (A[i][k], B[k][i];
+// D[i][n] at lane 16 i + 4 j + n, checked on gfx950): B is the 16x16x4 B
+// operand unchanged, lane l reads A row l%4, column l/16 from lane
+// 16 (l/16) + l%4 of the table, and the result lands in register 0's place
+// (row l/16, element l%16); registers 1-3 stay zero.
+__device__ __forceinline__ void mfma_step(double4v& acc, const double* frag, int lane, double b,
+                                          bool four_rows) {
+  if (four_rows)
+    acc[0] = __builtin_amdgcn_mfma_f64_4x4x4f64(frag[16 * (lane >> 4) + (lane & 3)], b, acc[0], 0,
+                                                0, 0);
+  else
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[lane], b, acc, 0, 0, 0);
+}
+
 // ---------------------------------------------------------------------------
 // Persson-Peraire sensor on FP64 matrix cores. Per 16-element tile each wave
 // forms the modal coefficients uh = V^-1 u with v_mfma_f64_16x16x4_f64 (the
@@ -265,8 +283,7 @@ __global__ void __launch_bounds__(256) pp_sensor_kernel(AvArgs a) {
     for (int s = 0; s < KS; ++s)
 #pragma unroll
       for (int tt = 0; tt < MT; ++tt)
-        acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sV[(tt * KS + s) * 64 + lane], b[s], acc[tt],
-                                                       0, 0, 0);
+        mfma_step(acc[tt], sV + (tt * KS + s) * 64, lane, b[s], NP - 16 * tt <= 4);
     double umax = 0.0, tot = 0.0, top = 0.0;
 #pragma unroll
     for (int s = 0; s < KS; ++s) umax = fmax(umax, fabs(b[s]));
@@ -484,8 +501,8 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
         const double b = fp < NFP ? uPv[f][sp] - uM : 0.0;
 #pragma unroll
         for (int tt = 0; tt < MTL; ++tt)
-          accF[f][tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              sAl[(tt * 4 * SPF + f * SPF + sp) * 64 + lane], b, accF[f][tt], 0, 0, 0);
+          mfma_step(accF[f][tt], sAl + (tt * 4 * SPF + f * SPF + sp) * 64, lane, b,
+                    NP - 16 * tt <= 4);
       }
     }
     double qv[3][QN];
@@ -530,8 +547,7 @@ __global__ void __launch_bounds__(64 * WPB, WPB >= 8 ? 2 : 1)
         }
 #pragma unroll
         for (int tt = 0; tt < MTL; ++tt)
-          accD[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAd[(tt * KSD + s) * 64 + lane], b,
-                                                           accD[tt], 0, 0, 0);
+          mfma_step(accD[tt], sAd + (tt * KSD + s) * 64, lane, b, NP - 16 * tt <= 4);
       }
     }
     // Face traces sig = eps n_g . q at the points of face g of element c.
@@ -789,8 +805,7 @@ __global__ void __launch_bounds__(64 * WPB, 2) av_surf_kernel(AvArgs a, StageAv 
       for (int f = 0; f < 4; ++f)
 #pragma unroll
         for (int s = 0; s < KSF; ++s)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[((tt * 4 + f) * KSF + s) * 64 + lane], b[f][s],
-                                                     acc, 0, 0, 0);
+          mfma_step(acc, sA + ((tt * 4 + f) * KSF + s) * 64, lane, b[f][s], NP - 16 * tt <= 4);
       __builtin_amdgcn_s_setprio(0);
       // Through the wave's 16 x 17 buffer to rows of 16 consecutive values.
       wave_fence();  // previous readers of the buffer are done
