@@ -919,11 +919,12 @@ __device__ __forceinline__ float tanh_fast(float x) {
 
 // Forward pass of 16 inputs (columns); X[s] is lane (c, g)'s BF16 slice of
 // input c in k step s. Returns y for column c (all four lane groups).
+// The biases enter as the accumulators' initial values.
 __device__ __forceinline__ float mlp_forward_tile(const MlpRegs& R, const bf16x8 (&X)[2]) {
   floatx4 z[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    z[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    z[mt] = (floatx4){R.b1[mt][0], R.b1[mt][1], R.b1[mt][2], R.b1[mt][3]};
 #pragma unroll
     for (int s = 0; s < 2; ++s) z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.A1[mt][s], X[s], z[mt], 0, 0, 0);
   }
@@ -933,11 +934,11 @@ __device__ __forceinline__ float mlp_forward_tile(const MlpRegs& R, const bf16x8
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int mt = 2 * s + (j >> 2), r = j & 3;
-      H[s][j] = (__bf16)tanh_fast(z[mt][r] + R.b1[mt][r]);
+      H[s][j] = (__bf16)tanh_fast(z[mt][r]);
     }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    z[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    z[mt] = (floatx4){R.b2[mt][0], R.b2[mt][1], R.b2[mt][2], R.b2[mt][3]};
 #pragma unroll
     for (int s = 0; s < 2; ++s) z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.A2[mt][s], H[s], z[mt], 0, 0, 0);
   }
@@ -945,7 +946,7 @@ __device__ __forceinline__ float mlp_forward_tile(const MlpRegs& R, const bf16x8
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) part = fmaf(R.w3[mt][r], tanh_fast(z[mt][r] + R.b2[mt][r]), part);
+    for (int r = 0; r < 4; ++r) part = fmaf(R.w3[mt][r], tanh_fast(z[mt][r]), part);
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
   return __builtin_amdgcn_rcpf(1.f + __expf(-(part + R.b3)));
@@ -992,7 +993,7 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
   constexpr int MTM = (NP + 15) / 16, KSM = (NP + 31) / 32;
   constexpr int PV = (16 * NP + 63) / 64;  // tile values per lane
   static_assert(NP <= 48, "modal rows must fit three 16-row tiles");
-  __shared__ double sUall[4][16 * S];
+  __shared__ double sUall[4][16 * S + 1];  // + a slot for the stores past the tile
   // V^-1 fragments (BF16 A operand of 16x16x32, hi and lo parts), shared by
   // the block: entry [part][mt][s][lane] holds A[16 mt + row][32 s + 8 g + j].
   __shared__ bf16x8 sV[2][MTM][KSM][64];
@@ -1040,10 +1041,8 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
 #pragma unroll
     for (int j = 0; j < PV; ++j) {
       const int i = lane + 64 * j;
-      if (i < cur.count * NP) {
-        const int e = i / NP, nn = i - e * NP;
-        sU[e * S + nn] = pre[j];
-      }
+      const int e = i / NP, nn = i - e * NP;
+      sU[i < cur.count * NP ? e * S + nn : 16 * S] = pre[j];
     }
     if (t + ts < t1) {
       tl = tile_uniform(tile_raw);
@@ -1055,17 +1054,17 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
     floatx4 acc[MTM];
 #pragma unroll
     for (int mt = 0; mt < MTM; ++mt) acc[mt] = (floatx4){0.f, 0.f, 0.f, 0.f};
-    double umax = 0.0;
+    float umax = 0.f;
 #pragma unroll
     for (int s = 0; s < KSM; ++s) {
       bf16x8 bh, bl;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int k = 32 * s + 8 * g + j;
-        const double uv = k < NP ? sU[cc * S + k] : 0.0;
-        umax = fmax(umax, fabs(uv));
-        bh[j] = bf16_hi((float)uv);
-        bl[j] = bf16_lo((float)uv);
+        const float uv = k < NP ? (float)sU[cc * S + k] : 0.f;
+        umax = fmaxf(umax, fabsf(uv));
+        bh[j] = bf16_hi(uv);
+        bl[j] = bf16_lo(uv);
       }
 #pragma unroll
       for (int mt = 0; mt < MTM; ++mt) {
@@ -1083,8 +1082,8 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
       for (int r = 0; r < 4; ++r) tot = fmaf(acc[mt][r], acc[mt][r], tot);
     tot += __shfl_xor(tot, 16, 64);
     tot += __shfl_xor(tot, 32, 64);
-    umax = fmax(umax, __shfl_xor(umax, 16, 64));
-    umax = fmax(umax, __shfl_xor(umax, 32, 64));
+    umax = fmaxf(umax, __shfl_xor(umax, 16, 64));
+    umax = fmaxf(umax, __shfl_xor(umax, 32, 64));
     const float inv = tot > 0.f ? 1.f / sqrtf(tot) : 0.f;
     bf16x8 X[2];
 #pragma unroll
@@ -1097,8 +1096,10 @@ __global__ void __launch_bounds__(256) mlp_sensor_kernel(AvArgs a, const float* 
     const float y = mlp_forward_tile(R, X);
     if (g == 0 && c < cur.count) {
       const int64_t ge = (int64_t)cur.gbase + c;
-      const double h = cbrt(6.0 * (4.0 / 3.0) * jv);
-      const double e = a.ceps * h * sqrt(3.0) * umax / N * (double)y;
+      // h = (6 vol)^(1/3), vol = 4/3 J; FP32 (the network output it scales
+      // carries BF16 rounding)
+      const float h = 2.f * cbrtf((float)jv);
+      const double e = (double)((float)a.ceps * h * 1.7320508f * umax / N * y);
       a.eps[ge] = e;
       if (a.frac) a.frac[ge] = (double)y;
       emax = fmax(emax, e);
