@@ -100,68 +100,108 @@ struct NodalArgs {
   const void* t_off[3];
 };
 
+// A block's arrays based at its partition: every offset is read once per
+// block (the stores could alias the offset tables, so offsets read per node
+// would be read again after every store).
+template <typename MT, typename OT>
+struct NodalBase {
+  const MT* m[9];
+  OT* o[3];
+  double* t[3];
+};
+
 template <typename MT, typename OT, typename OFF>
-__device__ __forceinline__ void store_node(const NodalArgs& a, int part, int64_t li, double ar,
-                                           double as, double at) {
-  double m[9];
+__device__ __forceinline__ NodalBase<MT, OT> nodal_base(const NodalArgs& a, int part) {
+  NodalBase<MT, OT> b;
 #pragma unroll
-  for (int q = 0; q < 9; ++q)
-    m[q] = (double)load_as<MT>(a.m[q], (int64_t)((const OFF*)a.m_off[q])[part] + li);
-  ((OT*)a.o[0])[(int64_t)((const OFF*)a.o_off[0])[part] + li] = (OT)(m[0] * ar + m[3] * as + m[6] * at);
-  ((OT*)a.o[1])[(int64_t)((const OFF*)a.o_off[1])[part] + li] = (OT)(m[1] * ar + m[4] * as + m[7] * at);
-  ((OT*)a.o[2])[(int64_t)((const OFF*)a.o_off[2])[part] + li] = (OT)(m[2] * ar + m[5] * as + m[8] * at);
-  if (a.t[0]) {
-    a.t[0][(int64_t)((const OFF*)a.t_off[0])[part] + li] = ar;
-    a.t[1][(int64_t)((const OFF*)a.t_off[1])[part] + li] = as;
-    a.t[2][(int64_t)((const OFF*)a.t_off[2])[part] + li] = at;
+  for (int q = 0; q < 9; ++q) b.m[q] = (const MT*)a.m[q] + (int64_t)((const OFF*)a.m_off[q])[part];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    b.o[q] = (OT*)a.o[q] + (int64_t)((const OFF*)a.o_off[q])[part];
+    b.t[q] = a.t[0] ? a.t[q] + (int64_t)((const OFF*)a.t_off[q])[part] : nullptr;
+  }
+  return b;
+}
+
+template <typename MT, typename OT>
+__device__ __forceinline__ void load_metrics(const NodalBase<MT, OT>& b, int64_t li, double (&m)[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = (double)b.m[q][li];
+}
+
+// The metrics are loaded (load_metrics) before any output of the block's
+// thread is stored: the stores could alias the metric arrays, so loads
+// placed after a store would wait for it.
+template <typename MT, typename OT>
+__device__ __forceinline__ void store_node(const NodalBase<MT, OT>& b, int64_t li,
+                                           const double (&m)[9], double ar, double as, double at) {
+  b.o[0][li] = (OT)(m[0] * ar + m[3] * as + m[6] * at);
+  b.o[1][li] = (OT)(m[1] * ar + m[4] * as + m[7] * at);
+  b.o[2][li] = (OT)(m[2] * ar + m[5] * as + m[8] * at);
+  if (b.t[0]) {
+    b.t[0][li] = ar;
+    b.t[1][li] = as;
+    b.t[2][li] = at;
   }
 }
 
-// One block: G*EPG elements of one partition; threads = NP * G (padded to a
-// wave multiple), thread (node n, group g) handles node n of elements g,
-// g+G, ..., g+(EPG-1)G.
+// One block: tiles of G*EPG elements of one partition, tiles tx = blockIdx.x,
+// blockIdx.x + gridDim.x, ... below ntx (the flat list, or the tiles of
+// partition blockIdx.y), so Dr/Ds/Dt are staged in LDS once per block;
+// threads = NP * G (padded to a wave multiple), thread (node n, group g)
+// handles node n of elements g, g+G, ..., g+(EPG-1)G.
 template <int NP, int G, int EPG, typename MT, typename OT, typename OFF>
 __global__ void __launch_bounds__(256)
-deriv_nodal_kernel(PartTiles pt, NodalArgs a) {
+deriv_nodal_kernel(PartTiles pt, NodalArgs a, int ntx) {
   constexpr int E = G * EPG;
   __shared__ double sD[3 * NP * NP];
   __shared__ double sU[E * NP];
-  int part, first, count;
-  if (!find_tile(pt, E, part, first, count)) return;
   const int tid = threadIdx.x;
   for (int i = tid; i < 3 * NP * NP; i += blockDim.x) {
     int q = i / (NP * NP), r = i - q * NP * NP;
     int n = r / NP, j = r - n * NP;
     sD[i] = a.D[q][n * a.drs + j * a.dcs];
   }
-  const int64_t ub = (int64_t)((const OFF*)a.u_off)[part] + (int64_t)first * NP;
-  const int nval = count * NP;
-  for (int i = tid; i < nval; i += blockDim.x) sU[i] = a.u[ub + i];
-  __syncthreads();
   const int n = tid % NP, g = tid / NP;
-  if (g >= G) return;
-  double ar[EPG], as[EPG], at[EPG];
-#pragma unroll
-  for (int q = 0; q < EPG; ++q) ar[q] = as[q] = at[q] = 0.0;
   const double* dr = sD + n * NP;
   const double* ds = sD + NP * NP + n * NP;
   const double* dt = sD + 2 * NP * NP + n * NP;
+  for (int tx = blockIdx.x; tx < ntx; tx += gridDim.x) {
+    int part, first, count;
+    if (!find_tile_at(pt, tx, E, part, first, count)) continue;
+    const int64_t ub = (int64_t)((const OFF*)a.u_off)[part] + (int64_t)first * NP;
+    const NodalBase<MT, OT> base = nodal_base<MT, OT, OFF>(a, part);
+    const int nval = count * NP;
+    __syncthreads();  // the previous tile's readers of sU are done
+    for (int i = tid; i < nval; i += blockDim.x) sU[i] = a.u[ub + i];
+    __syncthreads();
+    if (g >= G) continue;
+    double ar[EPG], as[EPG], at[EPG];
+#pragma unroll
+    for (int q = 0; q < EPG; ++q) ar[q] = as[q] = at[q] = 0.0;
 #pragma unroll 5
-  for (int j = 0; j < NP; ++j) {
-    double x = dr[j], y = ds[j], z = dt[j];
+    for (int j = 0; j < NP; ++j) {
+      double x = dr[j], y = ds[j], z = dt[j];
+#pragma unroll
+      for (int q = 0; q < EPG; ++q) {
+        double uj = sU[(g + q * G) * NP + j];
+        ar[q] = fma(x, uj, ar[q]);
+        as[q] = fma(y, uj, as[q]);
+        at[q] = fma(z, uj, at[q]);
+      }
+    }
+    double mv[EPG][9];
 #pragma unroll
     for (int q = 0; q < EPG; ++q) {
-      double uj = sU[(g + q * G) * NP + j];
-      ar[q] = fma(x, uj, ar[q]);
-      as[q] = fma(y, uj, as[q]);
-      at[q] = fma(z, uj, at[q]);
+      const int e = g + q * G;
+      load_metrics(base, (int64_t)(first + (e < count ? e : 0)) * NP + n, mv[q]);
     }
-  }
 #pragma unroll
-  for (int q = 0; q < EPG; ++q) {
-    int e = g + q * G;
-    if (e >= count) continue;
-    store_node<MT, OT, OFF>(a, part, (int64_t)(first + e) * NP + n, ar[q], as[q], at[q]);
+    for (int q = 0; q < EPG; ++q) {
+      int e = g + q * G;
+      if (e >= count) continue;
+      store_node(base, (int64_t)(first + e) * NP + n, mv[q], ar[q], as[q], at[q]);
+    }
   }
 }
 
@@ -175,6 +215,7 @@ deriv_nodal_big_kernel(PartTiles pt, NodalArgs a) {
   if (!find_tile(pt, G, part, first, count)) return;
   const int tid = threadIdx.x;
   const int64_t ub = (int64_t)((const OFF*)a.u_off)[part] + (int64_t)first * NP;
+  const NodalBase<MT, OT> base = nodal_base<MT, OT, OFF>(a, part);
   for (int i = tid; i < count * NP; i += blockDim.x) sU[i] = a.u[ub + i];
   __syncthreads();
   for (int w = tid; w < count * NP; w += blockDim.x) {
@@ -187,7 +228,9 @@ deriv_nodal_big_kernel(PartTiles pt, NodalArgs a) {
       as = fma(a.D[1][di], uj, as);
       at = fma(a.D[2][di], uj, at);
     }
-    store_node<MT, OT, OFF>(a, part, (int64_t)first * NP + w, ar, as, at);
+    double mv[9];
+    load_metrics(base, (int64_t)first * NP + w, mv);
+    store_node(base, (int64_t)first * NP + w, mv, ar, as, at);
   }
 }
 
@@ -211,8 +254,13 @@ int launch_nodal(dgk_device dev, int npart, const int64_t* K, const void* kdev, 
     if (rc) return rc;
     if (grid.x == 0) return DGK_OK;
     int threads = ((NP * G + 63) / 64) * 64;
+    // a few blocks per CU walk the tiles (the operators are staged once per
+    // block); in the 2-D form, per partition
+    const int ntx = (int)grid.x;
+    const int cap = pt.mode2d ? std::max(1, 4 * dev->cu_count / (int)grid.y) : 4 * dev->cu_count;
+    if ((int)grid.x > cap) grid.x = cap;
     hipLaunchKernelGGL((deriv_nodal_kernel<NP, G, EPG, MT, OT, OFF>), grid, dim3(threads), 0,
-                       dev->stream, pt, a);
+                       dev->stream, pt, a, ntx);
   } else {
     constexpr int G = 16;
     int rc = dgk::build_part_tiles(dev, npart, K, G, kdev, ksize, pt, grid, force2d);

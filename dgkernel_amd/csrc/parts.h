@@ -25,6 +25,32 @@ struct PartTiles {
   int K[kMaxPartsByValue];          // elements per partition (flat mode)
 };
 
+// Tile tx of the flat list (flat mode), or tile tx of partition blockIdx.y
+// (2-D mode); false if it has no elements.
+__device__ __forceinline__ bool find_tile_at(const PartTiles& pt, int tx, int per_tile, int& part,
+                                             int& first, int& count) {
+  if (pt.mode2d) {
+    part = blockIdx.y;
+    first = tx * per_tile;
+    long long k = pt.ksize == 8 ? (long long)((const int64_t*)pt.kdev)[part]
+                                : (long long)((const int32_t*)pt.kdev)[part];
+    long long rem = k - first;
+    count = rem < per_tile ? (int)rem : per_tile;
+    return count > 0;
+  }
+  const int tile = tx;
+  int lo = 0, hi = pt.npart;  // last p with start[p] <= tile
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pt.start[mid] <= tile) lo = mid; else hi = mid;
+  }
+  part = lo;
+  first = (tile - pt.start[lo]) * per_tile;
+  int rem = pt.K[lo] - first;
+  count = rem < per_tile ? rem : per_tile;
+  return count > 0;
+}
+
 // Tile of this block; false if the block has no elements (2-D mode only).
 __device__ __forceinline__ bool find_tile(const PartTiles& pt, int per_tile, int& part, int& first,
                                           int& count) {
