@@ -339,6 +339,18 @@ __device__ __forceinline__ void mfma_f64_rows4(double4_t& acc, const double* fra
                                               0, 0);
 }
 
+// The same for a tile whose rows end inside it: one 4x4x4 per group of four
+// rows that holds operator rows (ng of them, a constant once the callers'
+// loops are unrolled); group r reads A row 4 r + l%4 from lane
+// 16 (l/16) + 4 r + l%4 and fills register r's place.
+__device__ __forceinline__ void mfma_f64_row_groups(double4_t& acc, const double* frag, int lane,
+                                                    double b, int ng) {
+  const double* f = frag + 16 * (lane >> 4) + (lane & 3);
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    if (r < ng) acc[r] = __builtin_amdgcn_mfma_f64_4x4x4f64(f[4 * r], b, acc[r], 0, 0, 0);
+}
+
 // Each wave works through its own sequence of 16-element tiles; the waves of a
 // block only share the operator fragments (written once, one block barrier).
 template <int N, int WPB>
@@ -926,9 +938,14 @@ __device__ __forceinline__ void deep_tiles(const StageParams& p, const double* s
         const double uk = su[k < NP ? k : NP - 1];
         const double Fk = k < NP ? uk * uk : 0.0;  // the operator carries -1/2
 #pragma unroll
-        for (int tt = 0; tt < MTV; ++tt)
-          accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAv[(tt * KSV + s) * 64 + lane], Fk,
-                                                           accV[tt], 0, 0, 0);
+        for (int tt = 0; tt < MTV; ++tt) {
+          if (3 * NPR - 16 * tt < 16)  // a last row tile with padding rows
+            mfma_f64_row_groups(accV[tt], sAv + (tt * KSV + s) * 64, lane, Fk,
+                                (3 * NPR - 16 * tt + 3) / 4);
+          else
+            accV[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sAv[(tt * KSV + s) * 64 + lane], Fk,
+                                                             accV[tt], 0, 0, 0);
+        }
       }
 #pragma unroll
       for (int q = 0; q < QN; ++q) {
