@@ -354,6 +354,8 @@ def bench_c4_stage(a, dev) -> dict:
         if a.tile_order >= 0:
             solver.mesh.set_tile_order(bool(a.tile_order))
         solver.mesh.set_precision(STAGE_PRECS[name])
+        if a.variant >= 0:
+            solver.mesh.set_variant(a.variant)
         solver.set_state(u0)
         _warm(dev, lambda: solver.step(1), min_calls=1)  # outside the timed run, restarted below
         solver.set_state(u0)

@@ -96,6 +96,9 @@ struct dgk_dgmesh_s {
   int rhs32 = 0;                         // rhs holds FP32 values
   void* lift_frag = nullptr;             // LIFT fragments for lift_prec
   int split_blocks = 0;                  // persistent grid of the surface kernel
+  // Fused BF16x3 stage at P = 5, 6 (fused_hi.hip)
+  void* fused_frag = nullptr;            // [-Dr | -Ds | -Dt | LIFT] BF16 hi/lo fragments
+  float* fused_lam = nullptr;            // [3][Np] barycentric weights of the nodes
 };
 
 namespace dgk {
@@ -114,6 +117,15 @@ int split_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, 
 // (N, prec) if not there yet (allocates: call outside graph capture).
 int split_prepare(dgk_dgmesh m);
 void split_release(dgk_dgmesh m);
+
+// Fused stage for P = 5, 6 with the D and LIFT contractions in BF16x3
+// (fused_hi.hip): volume term, flux, LIFT and RK update in one launch; used
+// by split_stage in place of its two launches unless variant 3 asks for them.
+bool fused_applies(dgk_dgmesh m);
+int fused_prepare(dgk_dgmesh m);
+void fused_release(dgk_dgmesh m);
+int fused_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
+                double a1, const double* dt, unsigned long long* maxabs, int mode);
 
 // Volume term of the Burgers RHS through a derivative plan (mixed.hip):
 // out = -(div f(u)), f(u) = u^2/2 (1,1,1), for the elements of the tiles

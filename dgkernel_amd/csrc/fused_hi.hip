@@ -1,0 +1,634 @@
+// Fused Burgers stage at P = 5, 6 in split BF16 (three products): the volume
+// term, the Lax-Friedrichs flux, LIFT and the RK update in one launch, in
+// place of the split stage's two launches (volume term through a derivative
+// plan into a state-sized buffer, then flux + LIFT + update), which read u
+// three times and the volume term twice. The stage is the one the other
+// stage kernels compute (strong-form LF + LIFT after Hesthaven & Warburton
+// 2008 §6, against the reference's volume-only RHS in
+// examples/3d_scalar_burgers_equation.go:295-340 and its LF + LIFT sketch in
+// documentation/DGKernel/example_application.md:296-353).
+//
+// Per 16-element tile the stage is four products sharing one B,
+//   rhs = ga (-Dr W) + gb (-Ds W) + gc (-Dt W) + LIFT G + corr,
+// W = [w] (Np x 16) and G = [g] (4 Nfp x 16) one column per element, with
+// ga = rx+ry+rz, gb = sx+sy+sz, gc = tx+ty+tz the folded metrics of the
+// affine element, w = f - Lf for f = u^2/2 minus its linear interpolant
+// through the vertex nodes (rounding then scales with |f - Lf|; D Lf is the
+// constant (a1, a2, a3)/2 of the interpolant's slopes and enters as the
+// per-element correction corr = -(ga a1 + gb a2 + gc a3)/2), and g the LF
+// jumps
+//   g = (u- - u+) (A (u- + u+) - B max(|u-|, |u+|)),
+// A = Fscale n.(1,1,1)/4, B = Fscale |n.(1,1,1)|/2 -- the same strong-form
+// flux as the fused P <= 4 stage (burgers.hip) and the split stage
+// (split.hip). Then out = a0 u0 + a1 (u + dt rhs). The metrics scale the
+// products' FP32 results in FP64, so B carries w once.
+//
+// Block: MT matrix waves (one per 16-row tile of [-Dr | -Ds | -Dt | LIFT],
+// its BF16 hi/lo fragments held in registers for the launch) and MW memory
+// waves. The block walks its tiles in phases separated by block barriers;
+// in phase n
+//   matrix waves: B of tile n from LDS -> v_mfma_f32_16x16x32_bf16 (A_lo B_hi
+//                 + A_hi B_lo + A_hi B_hi, computed as its transpose so that
+//                 a lane holds consecutive rows of one element) -> RK update
+//                 of their rows (u, u0, the element's metrics and the tile
+//                 record from LDS: no global loads);
+//   memory waves: the face records of tile n+3 (loaded in phase n-1) into
+//                 LDS; the loads of tile n+4's records and of tile n+2 (u,
+//                 u0 and the neighbour traces, gathered through the records
+//                 stored in phase n-1); tile n+1's u and u0 (loaded in phase
+//                 n-1) into LDS, w and the jumps (own traces from that LDS
+//                 tile) as BF16 hi/lo into B of tile n+1.
+// Memory wave q owns elements 4q..4q+3 of every tile, so it only reads LDS it
+// wrote itself between block barriers.
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#include "dgmesh_internal.h"
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+template <int N>
+struct FCfg {
+  static constexpr int NP = dgk::Tet<N>::Np, NFP = dgk::Tet<N>::Nfp;
+  static constexpr int MT = (NP + 15) / 16;   // row tiles of A = matrix waves
+  static constexpr int KV = (NP + 31) / 32;   // k = 32 steps per volume block
+  static constexpr int KL = (NFP + 31) / 32;  // k = 32 steps per face
+  static constexpr int KS = 3 * KV + 4 * KL;  // k steps of A
+  static constexpr int KB = KV + 4 * KL;      // k steps of B (w once, then the jumps)
+  static constexpr int MW = 4;                // memory waves
+  static constexpr int EPW = 16 / MW;         // elements per memory wave
+  static constexpr int NT = 64 * (MT + MW);
+  static constexpr int NPU = NP | 1;          // LDS row stride of u, doubles
+  static constexpr int UPL = (EPW * NP + 63) / 64;        // u values per lane
+  static constexpr int GPL = (EPW * 4 * NFP + 63) / 64;  // trace pairs per lane
+  static constexpr size_t NFRAG = (size_t)MT * KS * 64;   // fragments per part
+  // vertex nodes of the reference tet in the nodal order (checked on the host)
+  static constexpr int VTX[4] = {0, N, (N + 1) * (N + 2) / 2 - 1, NP - 1};
+};
+
+// Face record of (face f, element j) of a memory wave's elements: value
+// offset of the neighbour's node 0, row of the orientation table, flux
+// factors.
+struct FRec {
+  int32_t nb, fnrow;
+  float fa, fb;
+};
+
+struct FusedArgs {
+  const dgk::Tile* tiles;
+  int ntiles;
+  const bf16x8* frag;         // [part][MT][KS][64]
+  const uint8_t* face_nodes;  // [4][24][Nfp]
+  const double* bgeo;         // [19][K]; rows 0..2 the folded metrics
+  int64_t K;
+  const int32_t* nbr;         // [4][K]
+  const int32_t* codes;       // [K]
+  const float* lam;           // [3][Np] barycentric weights of the nodes
+  const double* u;
+  const double* u0;
+  double* out;
+  double a0, a1;
+  const double* dt;
+  unsigned long long* maxabs;
+  int mode;                   // 0: RK stage, 1: out = rhs
+};
+
+__device__ __forceinline__ uint16_t bf16_of(float x) {
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// (The tile list is a separate const __restrict__ argument: the compiler may
+// then read its wave-uniform entries through the scalar cache, whose wait
+// counter does not hold back on the vector loads in flight.)
+template <int N>
+__global__ void __launch_bounds__(FCfg<N>::NT)
+    fused_bf3_kernel(FusedArgs a, const dgk::Tile* __restrict__ tiles) {
+  using C = FCfg<N>;
+  constexpr int NP = C::NP, NFP = C::NFP, MT = C::MT, KV = C::KV, KL = C::KL, KS = C::KS;
+  constexpr int KB = C::KB, MW = C::MW, EPW = C::EPW, NPU = C::NPU, UPL = C::UPL, GPL = C::GPL;
+  constexpr dgk::FmaskTable<N> FM{};
+  // B of two tiles: [buffer][part][element][k] BF16 (hi, lo), k = 32 s + the
+  // step's column: w at the nodes (32 KV columns), then the jumps face by
+  // face (32 KL columns each). Rows are padded by 8 columns, so that the 16
+  // elements of a 16-byte read fall in distinct banks; the padding also takes
+  // the writes of lanes past the tile's values.
+  constexpr int KBR = 32 * KB + 8, DUMP = 32 * KB;
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][16][KBR];
+  // u and u0 of two tiles (+ a dump slot), and their tile records, staged by
+  // the memory waves for the matrix waves' update
+  __shared__ double sU[2][16 * NPU + 1], sU0[2][16 * NPU + 1];
+  __shared__ int64_t sTv[2];
+  __shared__ int sTc[2];
+  // per element of two tiles: ga, gb, gc and the linear part's correction
+  __shared__ __attribute__((aligned(16))) double sE[2][16][4];
+  // per memory wave: face records and folded metrics of its elements for
+  // four tiles (slot = tile % 4)
+  __shared__ FRec sRec[MW][4][4 * EPW];
+  __shared__ double sGm[MW][4][EPW][4];
+  __shared__ __attribute__((aligned(16))) float sLam[NP][4];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  __shared__ uint8_t sFM[4 * NFP];
+  {
+    uint4* z = (uint4*)&sB[0][0][0][0];
+    for (int i = threadIdx.x; i < (int)(sizeof(sB) / 16); i += C::NT) z[i] = uint4{0, 0, 0, 0};
+    for (int i = threadIdx.x; i < 4 * 24 * NFP; i += C::NT) sFN[i] = a.face_nodes[i];
+    for (int i = threadIdx.x; i < 4 * NFP; i += C::NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
+    for (int i = threadIdx.x; i < 4 * NP; i += C::NT)
+      sLam[i / 4][i % 4] = i % 4 < 3 ? a.lam[(i % 4) * NP + i / 4] : 0.f;
+  }
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15, g = lane >> 4;
+  const int64_t K = a.K;
+  // Tiles of this block: with a grid that is a multiple of 8 (block b on XCD
+  // b % 8), each XCD walks one contiguous eighth of the tile list.
+  int t0 = blockIdx.x, t1 = a.ntiles, ts = gridDim.x;
+  if (gridDim.x % 8 == 0) {
+    const int x = blockIdx.x % 8;
+    t0 = (int)((int64_t)a.ntiles * x / 8) + blockIdx.x / 8;
+    t1 = (int)((int64_t)a.ntiles * (x + 1) / 8);
+    ts = gridDim.x / 8;
+  }
+  const int nb = t0 < t1 ? (t1 - t0 + ts - 1) / ts : 0;
+  auto tile_of = [&](int n) -> dgk::Tile {
+    const dgk::Tile tr = tiles[t0 + n * ts];
+    dgk::Tile u;
+    u.vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tr.vbase >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)tr.vbase));
+    u.gbase = __builtin_amdgcn_readfirstlane(tr.gbase);
+    u.count = __builtin_amdgcn_readfirstlane(tr.count);
+    return u;
+  };
+  // Lane-derived values are rederived in each step (an empty asm hides the
+  // lane index) rather than held across the tile loop, where they spilled.
+  auto lane_now = [&]() {
+    int l = lane;
+    asm volatile("" : "+v"(l));
+    return l;
+  };
+  auto wsync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // Face records of memory wave q's elements for tile n go to slot n % 4 of
+  // sRec[q] (their folded metrics to sGm[q]), written by that wave three
+  // phases ahead from loads it issued a phase before: lane s < 4 EPW takes
+  // face s / EPW of element EPW q + s % EPW (clamped to the tile).
+  int32_t rnb = 0, rcode = 0;
+  double rga = 0.0, rgb = 0.0, rgc = 0.0;
+  auto rec_load = [&](int qq, int n) {
+    const int ln = lane_now(), rf = ln / EPW, rj = ln % EPW;
+    if (n >= nb || ln >= 4 * EPW) return;
+    const dgk::Tile tl = tile_of(n);
+    const int e = EPW * qq + rj < tl.count ? EPW * qq + rj : tl.count - 1;
+    const int64_t ge = tl.gbase + e;
+    rnb = a.nbr[(int64_t)rf * K + ge];
+    rcode = a.codes[ge];
+    rga = a.bgeo[ge];
+    rgb = a.bgeo[K + ge];
+    rgc = a.bgeo[2 * K + ge];
+  };
+  auto rec_store = [&](int qq, int n) {
+    if (n >= nb) return;
+    const int ln = lane_now(), rf = ln / EPW, rj = ln % EPW;
+    if (ln < 4 * EPW) {
+      FRec r;
+      r.nb = rnb;
+      r.fnrow = (rf * 24 + ((rcode >> (5 * rf)) & 31)) * NFP;
+      double fa, fb;
+      dgk::face_factors(rf, rga, rgb, rgc, fa, fb);
+      r.fa = (float)fa;
+      r.fb = (float)fb;
+      sRec[qq][n & 3][ln] = r;
+      if (rf == 0) {
+        sGm[qq][n & 3][rj][0] = rga;
+        sGm[qq][n & 3][rj][1] = rgb;
+        sGm[qq][n & 3][rj][2] = rgc;
+      }
+    }
+  };
+  __syncthreads();
+
+  if (w < MT) {
+    // ---------------- matrix wave: rows 16 w .. 16 w + 15 ----------------
+    bf16x8 fh[KS], fl[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      fh[s] = a.frag[((size_t)w * KS + s) * 64 + lane];
+      fl[s] = a.frag[C::NFRAG + ((size_t)w * KS + s) * 64 + lane];
+    }
+    const double dt = a.mode == 0 ? *a.dt : 1.0;
+    const double a0 = a.mode == 0 ? a.a0 : 0.0, a1 = a.mode == 0 ? a.a1 : 1.0;
+    // The fragments and dt are complete before the loop: without this wait
+    // the compiler counts them as pending at the loop head and holds the
+    // products behind the stores of the previous tile.
+    __builtin_amdgcn_s_waitcnt(0);
+    double lmax = 0.0;
+    __syncthreads();  // phase -1: the memory waves prepare tile 0
+    for (int n = 0; n < nb; ++n) {
+      const int bu = n & 1;
+      const int64_t tv = sTv[bu];
+      const int64_t vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tv >> 32)) << 32) |
+                                      (uint32_t)__builtin_amdgcn_readfirstlane((int)tv));
+      const int count = __builtin_amdgcn_readfirstlane(sTc[bu]);
+      // The products run transposed, B^T A^T: the tile's B as the first
+      // operand (its registers are B's as read) and the fragments as the
+      // second, so that lane l, register r holds row 16 w + l % 16 of element
+      // 4 (l / 16) + r and each store below writes 16 consecutive rows of an
+      // element. -Dr w, -Ds w, -Dt w and LIFT g accumulate apart; the
+      // element's metrics scale them in the update.
+      floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aS = aR, aT = aR, aL = aR;
+      __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+      for (int s = 0; s < KV; ++s) {
+        const bf16x8 bh = *(const bf16x8*)&sB[bu][0][c][32 * s + 8 * g];
+        const bf16x8 bl = *(const bf16x8*)&sB[bu][1][c][32 * s + 8 * g];
+        aR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[s], aR, 0, 0, 0);
+        aS = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[KV + s], aS, 0, 0, 0);
+        aT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[2 * KV + s], aT, 0, 0, 0);
+        aR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[s], aR, 0, 0, 0);
+        aS = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[KV + s], aS, 0, 0, 0);
+        aT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[2 * KV + s], aT, 0, 0, 0);
+        aR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[s], aR, 0, 0, 0);
+        aS = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[KV + s], aS, 0, 0, 0);
+        aT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[2 * KV + s], aT, 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4 * KL; ++t) {
+        const bf16x8 bh = *(const bf16x8*)&sB[bu][0][c][32 * (KV + t) + 8 * g];
+        const bf16x8 bl = *(const bf16x8*)&sB[bu][1][c][32 * (KV + t) + 8 * g];
+        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[3 * KV + t], aL, 0, 0, 0);
+        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[3 * KV + t], aL, 0, 0, 0);
+        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[3 * KV + t], aL, 0, 0, 0);
+      }
+      __builtin_amdgcn_s_setprio(0);
+      // Stores through a buffer resource sized to the tile: the ones past it
+      // (elements >= count, rows >= Np) are dropped, so the partial tile takes
+      // no branches.
+      const __amdgpu_buffer_rsrc_t rsO = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(a.out + vbase), 0, count * NP * 8, 0x00020000);
+      const int row = 16 * w + c;
+      const int rr = row < NP ? row : NP - 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int e = 4 * g + r;
+        const double* E = sE[bu][e];
+        const double rhs = fma(E[0], (double)aR[r],
+                               fma(E[1], (double)aS[r], fma(E[2], (double)aT[r], (double)aL[r] + E[3])));
+        const double uv = a.mode == 0 ? sU[bu][e * NPU + rr] : 0.0;
+        const double u0v = a0 != 0.0 ? sU0[bu][e * NPU + rr] : 0.0;
+        const double o = fma(a0, u0v, a1 * fma(dt, rhs, uv));
+        const bool ok = row < NP && e < count;
+        lmax = fmax(lmax, ok ? fabs(o) : 0.0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, o), rsO,
+                                              row < NP ? (e * NP + row) * 8 : 0x40000000, 0, 2);
+      }
+      __syncthreads();  // end of phase n
+    }
+    if (a.maxabs) {
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
+      if (lane == 0 && lmax > 0.0)
+        atomicMax(a.maxabs, (unsigned long long)__double_as_longlong(lmax));
+    }
+    return;
+  }
+
+  // ---------------- memory wave q: elements EPW q .. EPW q + EPW - 1 ----------------
+  const int q = w - MT;
+  const int e0 = EPW * q;
+  // Per lane and trace pair k (pair p = lane + 64 k: slot p / NFP = f EPW + j,
+  // point p % NFP), fixed for the launch: index of the own trace in a tile's
+  // u (12 bits), column of the jump in B (9 bits; the padding past the pairs),
+  // record slot (5 bits), element (2 bits).
+  uint32_t fw[GPL];
+#pragma unroll
+  for (int k = 0; k < GPL; ++k) {
+    const int p = lane + 64 * k;
+    const bool ok = EPW * 4 * NFP % 64 == 0 || p < EPW * 4 * NFP;
+    const int pp = ok ? p : EPW * 4 * NFP - 1;
+    const int sl = pp / NFP, qq = pp % NFP, f = sl / EPW, j = sl % EPW;
+    const int iu = (e0 + j) * NPU + sFM[f * NFP + qq];
+    const int col = ok ? 32 * (KV + f * KL) + qq : DUMP + (lane & 7);
+    fw[k] = (uint32_t)iu | ((uint32_t)col << 12) | ((uint32_t)sl << 21) | ((uint32_t)j << 26);
+  }
+  // u, u0 values (pair p = lane + 64 k: element e0 + p / NP, node p % NP) and
+  // neighbour traces (pair p as above), by slot (compile-time constants, so
+  // the arrays stay in registers)
+  double U[2][UPL], U0[2][UPL], P[2][GPL];
+  int64_t TV[2] = {0, 0};  // tile records by slot
+  int TC[2] = {1, 1};
+  const bool with_u0 = a.mode == 0 && a.a0 != 0.0;
+  // Loads of tile n (clamped to the block's last tile, so that every phase
+  // issues the same loads) into a slot.
+  auto issue = [&](int n, auto SL) {
+    constexpr int slot = decltype(SL)::value;
+    const int nn = n < nb ? n : nb - 1;
+    const dgk::Tile tl = tile_of(nn);
+    const FRec* R = sRec[q][nn & 3];
+    const int lane = lane_now();
+#pragma unroll
+    for (int k = 0; k < UPL; ++k) {
+      const int p = lane + 64 * k;
+      const int e = e0 + p / NP;
+      const int64_t i = (int64_t)(e < tl.count ? e : tl.count - 1) * NP + p % NP;
+      U[slot][k] = a.u[tl.vbase + i];
+    }
+    if (with_u0) {
+#pragma unroll
+      for (int k = 0; k < UPL; ++k) {
+        const int p = lane + 64 * k;
+        const int e = e0 + p / NP;
+        const int64_t i = (int64_t)(e < tl.count ? e : tl.count - 1) * NP + p % NP;
+        U0[slot][k] = a.u0[tl.vbase + i];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < UPL; ++k) U0[slot][k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) {
+      const int p = lane + 64 * k;
+      const int pp = p < EPW * 4 * NFP ? p : EPW * 4 * NFP - 1;
+      const FRec& r = R[pp / NFP];
+      P[slot][k] = a.u[r.nb + sFN[r.fnrow + pp % NFP]];
+    }
+    TV[slot] = tl.vbase;
+    TC[slot] = tl.count;
+  };
+  // Tile n's values (loaded into slot) -> LDS u, u0, the per-element
+  // metrics and correction, and B.
+  auto prepare = [&](int n, auto SL) {
+    constexpr int slot = decltype(SL)::value;
+    const int lane = lane_now();
+    const int bu = n & 1;
+    const FRec* R = sRec[q][n & 3];
+    double* su = sU[bu];
+#pragma unroll
+    for (int k = 0; k < UPL; ++k) {
+      const int p = lane + 64 * k;
+      const int o = p < EPW * NP ? (e0 + p / NP) * NPU + p % NP : 16 * NPU;
+      su[o] = U[slot][k];
+      sU0[bu][o] = U0[slot][k];
+    }
+    if (q == 0 && lane == 0) {
+      sTv[bu] = TV[slot];
+      sTc[bu] = TC[slot];
+    }
+    // f = u^2/2 at the vertex nodes of the wave's elements, read from the
+    // lanes that hold them: f(v0) and the slopes f(v1..3) - f(v0) of the
+    // linear interpolant, uniform per element
+    // (made wave-uniform again after the arithmetic, so that they are held
+    // in scalar registers)
+    double L0[EPW], L1[EPW], L2[EPW], L3[EPW];
+#pragma unroll
+    for (int j = 0; j < EPW; ++j) {
+      double fv[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int p = j * NP + C::VTX[v];
+        const double x = readlane_f64(U[slot][p / 64], p % 64);
+        fv[v] = 0.5 * x * x;
+      }
+      L0[j] = readlane_f64(fv[0], 0);
+      L1[j] = readlane_f64(fv[1] - fv[0], 0);
+      L2[j] = readlane_f64(fv[2] - fv[0], 0);
+      L3[j] = readlane_f64(fv[3] - fv[0], 0);
+    }
+    // per element (lane j < EPW): the metrics and the correction
+    // -(ga a1 + gb a2 + gc a3)/2 for the interpolant's slopes
+    if (lane < EPW) {
+      double s1 = L1[0], s2 = L2[0], s3 = L3[0];
+#pragma unroll
+      for (int j = 1; j < EPW; ++j)
+        if (lane == j) {
+          s1 = L1[j];
+          s2 = L2[j];
+          s3 = L3[j];
+        }
+      const double* G = sGm[q][n & 3][lane];
+      const double ga = G[0], gb = G[1], gc = G[2];
+      double* E = sE[bu][e0 + lane];
+      E[0] = ga;
+      E[1] = gb;
+      E[2] = gc;
+      E[3] = -0.5 * (ga * s1 + gb * s2 + gc * s3);
+    }
+    // volume columns: w = f - Lf at node i of element j
+#pragma unroll
+    for (int k = 0; k < UPL; ++k) {
+      const int p = lane + 64 * k;
+      const bool ok = EPW * NP % 64 == 0 || p < EPW * NP;
+      const int pc = ok ? p : EPW * NP - 1;
+      const int j = pc / NP, i = pc % NP;
+      // the lanes of step k hold elements jlo .. jhi
+      const int jlo = (64 * k) / NP;
+      const int jhi = (64 * k + 63) / NP < EPW - 1 ? (64 * k + 63) / NP : EPW - 1;
+      double f0 = L0[jlo], d1 = L1[jlo], d2 = L2[jlo], d3 = L3[jlo];
+#pragma unroll
+      for (int jj = jlo + 1; jj <= jhi; ++jj)
+        if (j >= jj) {
+          f0 = L0[jj];
+          d1 = L1[jj];
+          d2 = L2[jj];
+          d3 = L3[jj];
+        }
+      const float4 lm = *(const float4*)sLam[i];
+      const double x = U[slot][k];
+      const double wv = 0.5 * x * x - (f0 + d1 * (double)lm.x + d2 * (double)lm.y + d3 * (double)lm.z);
+      const float v = (float)wv;
+      const __bf16 h = (__bf16)v;
+      uint16_t* b = &sB[bu][0][e0 + j][ok ? i : DUMP + (lane & 7)];
+      b[0] = __builtin_bit_cast(uint16_t, h);
+      b[16 * KBR] = bf16_of(v - (float)h);  // part 1: 16 rows further
+    }
+    wsync();  // the wave's u of the tile is in LDS
+    // jump columns: the jump at point qq of face f of element j
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) {
+      const uint32_t fk = fw[k];
+      const int iu = fk & 4095, col = (fk >> 12) & 511, sl = (fk >> 21) & 31, j = fk >> 26;
+      const FRec& r = R[sl];
+      const double m = su[iu], pv = P[slot][k];
+      const double x =
+          (m - pv) * fma((double)r.fa, m + pv, -(double)r.fb * fmax(fabs(m), fabs(pv)));
+      const float v = (float)x;
+      const __bf16 h = (__bf16)v;
+      uint16_t* b = &sB[bu][0][e0 + j][col];
+      b[0] = __builtin_bit_cast(uint16_t, h);
+      b[16 * KBR] = bf16_of(v - (float)h);
+    }
+  };
+
+  // Prologue (phase -1 of the matrix waves): records of tiles 0..2 stored
+  // and tile 3's loaded, loads of tiles 0 and 1, tile 0 prepared.
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+#pragma unroll 1
+  for (int n = 0; n < 3; ++n) {
+    rec_load(q, n);
+    rec_store(q, n);
+  }
+  rec_load(q, 3);
+  wsync();
+  if (nb > 0) {
+    issue(0, S0{});
+    issue(1, S1{});
+    prepare(0, S0{});
+  }
+  __syncthreads();  // end of phase -1
+  // Phase n (the matrix waves on tile n): the records of tile n+3 (loaded a
+  // phase ago) stored, the loads of tile n+4's records and of tile n+2 (its
+  // records are in LDS since phase n-1) into slot n & 1, then tile n+1
+  // prepared from slot (n+1) & 1, loaded a phase ago: each load is consumed
+  // a phase after it is issued, and the ones it waits for are the oldest in
+  // flight.
+  auto phase = [&](int n, auto A1, auto A2) {
+    rec_store(q, n + 3);
+    rec_load(q, n + 4);
+    wsync();
+    issue(n + 2, A2);
+    if (n + 1 < nb) prepare(n + 1, A1);
+    __syncthreads();  // end of phase n
+  };
+  for (int n = 0; n < nb; n += 2) {
+    phase(n, S1{}, S0{});
+    if (n + 1 < nb) phase(n + 1, S0{}, S1{});
+  }
+}
+
+// bf16 round-to-nearest-even of a float, as raw bits (host side).
+uint16_t bf16_bits_h(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  if ((x & 0x7f800000u) == 0x7f800000u) return (uint16_t)(x >> 16);
+  uint32_t lsb = (x >> 16) & 1u;
+  x += 0x7fffu + lsb;
+  return (uint16_t)(x >> 16);
+}
+
+float bf16_float_h(uint16_t b) {
+  uint32_t x = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+template <int N>
+int fused_launch(dgk_dgmesh m, const FusedArgs& a) {
+  if (a.ntiles == 0) return DGK_OK;
+  int grid = m->dev->cu_count;
+  if (grid > a.ntiles) grid = a.ntiles;
+  hipLaunchKernelGGL((fused_bf3_kernel<N>), dim3(grid), dim3(FCfg<N>::NT), 0, m->dev->stream, a,
+                     a.tiles);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+}  // namespace
+
+bool dgk::fused_applies(dgk_dgmesh m) {
+  return (m->N == 5 || m->N == 6) && m->prec == 2 && m->lift_prec == 2 && m->variant != 3;
+}
+
+int dgk::fused_prepare(dgk_dgmesh m) {
+  if (m->fused_frag) return DGK_OK;
+  const int N = m->N, NP = dgk::np_of(N), NFP = dgk::nfp_of(N), NF = 4 * NFP;
+  const int MT = (NP + 15) / 16, KV = (NP + 31) / 32, KL = (NFP + 31) / 32, KS = 3 * KV + 4 * KL;
+  const size_t nfrag = (size_t)MT * KS * 64;
+  // A = [-Dr | -Ds | -Dt | LIFT] in the 16x16x32 BF16 A-operand lane order:
+  // lane l of step s of row tile t holds A[16 t + l % 16][32 s' + 8 (l / 16) + e],
+  // e = 0..7, within the block (volume block s / KV, or face (s - 3 KV) / KL).
+  std::vector<uint16_t> buf(2 * nfrag * 8, 0);
+  const std::vector<double>* D[3] = {&m->Dr_h, &m->Ds_h, &m->Dt_h};
+  for (int t = 0; t < MT; ++t)
+    for (int s = 0; s < KS; ++s)
+      for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 8; ++e) {
+          const int row = 16 * t + (l & 15);
+          double v = 0.0;
+          if (row < NP) {
+            if (s < 3 * KV) {
+              const int k = 32 * (s % KV) + 8 * (l >> 4) + e;
+              if (k < NP) v = -(*D[s / KV])[(size_t)row * NP + k];
+            } else {
+              const int f = (s - 3 * KV) / KL;
+              const int qq = 32 * ((s - 3 * KV) % KL) + 8 * (l >> 4) + e;
+              if (qq < NFP) v = m->LIFT_h[(size_t)row * NF + f * NFP + qq];
+            }
+          }
+          const size_t q = (((size_t)t * KS + s) * 64 + l) * 8 + e;
+          const uint16_t h = bf16_bits_h((float)v);
+          buf[q] = h;
+          buf[nfrag * 8 + q] = bf16_bits_h((float)(v - (double)bf16_float_h(h)));
+        }
+  // the kernel takes the vertices of the reference tet at nodes 0, N,
+  // (N+1)(N+2)/2 - 1 and Np - 1; barycentric weights of the nodes
+  const double V[4][3] = {{-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+  const int vtx[4] = {0, N, (N + 1) * (N + 2) / 2 - 1, NP - 1};
+  for (int v = 0; v < 4; ++v) {
+    const int i = vtx[v];
+    DGK_REQUIRE(std::fabs(m->r_h[i] - V[v][0]) < 1e-10 && std::fabs(m->s_h[i] - V[v][1]) < 1e-10 &&
+                    std::fabs(m->t_h[i] - V[v][2]) < 1e-10,
+                "fused stage: node %d is not vertex %d of the reference tet", i, v);
+  }
+  std::vector<float> lam(3 * NP);
+  for (int i = 0; i < NP; ++i) {
+    lam[i] = (float)(0.5 * (1.0 + m->r_h[i]));
+    lam[NP + i] = (float)(0.5 * (1.0 + m->s_h[i]));
+    lam[2 * NP + i] = (float)(0.5 * (1.0 + m->t_h[i]));
+  }
+  int rc = dgk_malloc(m->dev, buf.size() * 2, buf.data(), &m->fused_frag);
+  if (rc == DGK_OK) rc = dgk_malloc(m->dev, lam.size() * 4, lam.data(), (void**)&m->fused_lam);
+  return rc;
+}
+
+void dgk::fused_release(dgk_dgmesh m) {
+  if (m->fused_frag) dgk_free(m->dev, m->fused_frag);
+  if (m->fused_lam) dgk_free(m->dev, m->fused_lam);
+  m->fused_frag = nullptr;
+  m->fused_lam = nullptr;
+}
+
+int dgk::fused_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
+                     double a1, const double* dt, unsigned long long* maxabs, int mode) {
+  int rc = fused_prepare(m);
+  if (rc != DGK_OK) return rc;
+  FusedArgs a;
+  a.tiles = m->tiles64;
+  a.ntiles = m->ntiles64;
+  a.frag = (const bf16x8*)m->fused_frag;
+  a.face_nodes = m->face_nodes;
+  a.bgeo = m->bgeo;
+  a.K = m->K;
+  a.nbr = m->nbr;
+  a.codes = m->codes;
+  a.lam = m->fused_lam;
+  a.u = u;
+  a.u0 = u0;
+  a.out = out;
+  a.a0 = a0;
+  a.a1 = a1;
+  a.dt = dt;
+  a.maxabs = maxabs;
+  a.mode = mode;
+  switch (m->N) {
+    case 5: return fused_launch<5>(m, a);
+    case 6: return fused_launch<6>(m, a);
+  }
+  dgk::set_error("fused stage: N=%d outside 5..6", m->N);
+  return DGK_ERR_UNSUPPORTED;
+}

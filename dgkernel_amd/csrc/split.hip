@@ -532,6 +532,7 @@ int surface(dgk_dgmesh m, const SurfArgs* a) {
 }  // namespace
 
 int dgk::split_prepare(dgk_dgmesh m) {
+  if (dgk::fused_applies(m)) return dgk::fused_prepare(m);
   if (m->plan && m->rhs && m->lift_frag) return DGK_OK;
   dgk::split_release(m);
   int rc = dgk_deriv_plan_create(m->dev, m->N, m->Dr_h.data(), m->Ds_h.data(), m->Dt_h.data(),
@@ -548,6 +549,7 @@ int dgk::split_prepare(dgk_dgmesh m) {
 }
 
 void dgk::split_release(dgk_dgmesh m) {
+  dgk::fused_release(m);
   if (m->plan) dgk_deriv_plan_destroy(m->plan);
   if (m->rhs) dgk_free(m->dev, m->rhs);
   if (m->lift_frag) dgk_free(m->dev, m->lift_frag);
@@ -564,6 +566,7 @@ int dgk::split_volume(dgk_dgmesh m, const double* u) {
 
 int dgk::split_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
                      double a1, const double* dt, unsigned long long* maxabs, int mode) {
+  if (dgk::fused_applies(m)) return dgk::fused_stage(m, u, u0, out, a0, a1, dt, maxabs, mode);
   int rc = split_volume(m, u);
   if (rc != DGK_OK) return rc;
   return split_surface(m, u, u0, out, a0, a1, dt, maxabs, mode);
