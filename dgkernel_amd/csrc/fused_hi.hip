@@ -127,9 +127,9 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
   // the writes of lanes past the tile's values.
   constexpr int KBR = 32 * KB + 8, DUMP = 32 * KB;
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][16][KBR];
-  // u and u0 of two tiles (+ a dump slot), and their tile records, staged by
-  // the memory waves for the matrix waves' update
-  __shared__ double sU[2][16 * NPU + 1], sU0[2][16 * NPU + 1];
+  // u of two tiles (+ a dump slot), and their tile records, staged by the
+  // memory waves for the matrix waves' update
+  __shared__ double sU[2][16 * NPU + 1];
   __shared__ int64_t sTv[2];
   __shared__ int sTc[2];
   // per element of two tiles: ga, gb, gc and the linear part's correction
@@ -141,6 +141,9 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
   __shared__ __attribute__((aligned(16))) float sLam[NP][4];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[4 * NFP];
+  // LIFT fragments of the matrix waves (their Dr, Ds, Dt fragments are held
+  // in registers): [row tile][face step][hi, lo][lane]
+  __shared__ bf16x8 sFL[MT][4 * KL][2][64];
   {
     uint4* z = (uint4*)&sB[0][0][0][0];
     for (int i = threadIdx.x; i < (int)(sizeof(sB) / 16); i += C::NT) z[i] = uint4{0, 0, 0, 0};
@@ -224,11 +227,16 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
 
   if (w < MT) {
     // ---------------- matrix wave: rows 16 w .. 16 w + 15 ----------------
-    bf16x8 fh[KS], fl[KS];
+    bf16x8 fh[3 * KV], fl[3 * KV];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
+    for (int s = 0; s < 3 * KV; ++s) {
       fh[s] = a.frag[((size_t)w * KS + s) * 64 + lane];
       fl[s] = a.frag[C::NFRAG + ((size_t)w * KS + s) * 64 + lane];
+    }
+#pragma unroll
+    for (int t = 0; t < 4 * KL; ++t) {
+      sFL[w][t][0][lane] = a.frag[((size_t)w * KS + 3 * KV + t) * 64 + lane];
+      sFL[w][t][1][lane] = a.frag[C::NFRAG + ((size_t)w * KS + 3 * KV + t) * 64 + lane];
     }
     const double dt = a.mode == 0 ? *a.dt : 1.0;
     const double a0 = a.mode == 0 ? a.a0 : 0.0, a1 = a.mode == 0 ? a.a1 : 1.0;
@@ -244,6 +252,19 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       const int64_t vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tv >> 32)) << 32) |
                                       (uint32_t)__builtin_amdgcn_readfirstlane((int)tv));
       const int count = __builtin_amdgcn_readfirstlane(sTc[bu]);
+      // u0 of the rows this lane updates (below), loaded during the products;
+      // loads past the tile read zero
+      const int row = 16 * w + c;
+      const int rr = row < NP ? row : NP - 1;
+      double u0v[4] = {0.0, 0.0, 0.0, 0.0};
+      if (a0 != 0.0) {
+        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(a.u0 + vbase), 0, count * NP * 8, 0x00020000);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          u0v[r] = __builtin_bit_cast(
+              double, __builtin_amdgcn_raw_buffer_load_b64(rs0, ((4 * g + r) * NP + rr) * 8, 0, 2));
+      }
       // The products run transposed, B^T A^T: the tile's B as the first
       // operand (its registers are B's as read) and the fragments as the
       // second, so that lane l, register r holds row 16 w + l % 16 of element
@@ -270,9 +291,10 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       for (int t = 0; t < 4 * KL; ++t) {
         const bf16x8 bh = *(const bf16x8*)&sB[bu][0][c][32 * (KV + t) + 8 * g];
         const bf16x8 bl = *(const bf16x8*)&sB[bu][1][c][32 * (KV + t) + 8 * g];
-        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[3 * KV + t], aL, 0, 0, 0);
-        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[3 * KV + t], aL, 0, 0, 0);
-        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[3 * KV + t], aL, 0, 0, 0);
+        const bf16x8 lh = sFL[w][t][0][lane], ll = sFL[w][t][1][lane];
+        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ll, aL, 0, 0, 0);
+        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, lh, aL, 0, 0, 0);
+        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, lh, aL, 0, 0, 0);
       }
       __builtin_amdgcn_s_setprio(0);
       // Stores through a buffer resource sized to the tile: the ones past it
@@ -280,8 +302,6 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       // no branches.
       const __amdgpu_buffer_rsrc_t rsO = __builtin_amdgcn_make_buffer_rsrc(
           (void*)(a.out + vbase), 0, count * NP * 8, 0x00020000);
-      const int row = 16 * w + c;
-      const int rr = row < NP ? row : NP - 1;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int e = 4 * g + r;
@@ -289,8 +309,7 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
         const double rhs = fma(E[0], (double)aR[r],
                                fma(E[1], (double)aS[r], fma(E[2], (double)aT[r], (double)aL[r] + E[3])));
         const double uv = a.mode == 0 ? sU[bu][e * NPU + rr] : 0.0;
-        const double u0v = a0 != 0.0 ? sU0[bu][e * NPU + rr] : 0.0;
-        const double o = fma(a0, u0v, a1 * fma(dt, rhs, uv));
+        const double o = fma(a0, u0v[r], a1 * fma(dt, rhs, uv));
         const bool ok = row < NP && e < count;
         lmax = fmax(lmax, ok ? fabs(o) : 0.0);
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, o), rsO,
@@ -325,13 +344,23 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     const int col = ok ? 32 * (KV + f * KL) + qq : DUMP + (lane & 7);
     fw[k] = (uint32_t)iu | ((uint32_t)col << 12) | ((uint32_t)sl << 21) | ((uint32_t)j << 26);
   }
-  // u, u0 values (pair p = lane + 64 k: element e0 + p / NP, node p % NP) and
+  // Barycentric weights of the node of value k (p = lane + 64 k, node p % NP),
+  // fixed for the launch.
+  float LX[UPL], LY[UPL], LZ[UPL];
+#pragma unroll
+  for (int k = 0; k < UPL; ++k) {
+    const int p = lane + 64 * k;
+    const int i = (p < EPW * NP ? p : EPW * NP - 1) % NP;
+    LX[k] = sLam[i][0];
+    LY[k] = sLam[i][1];
+    LZ[k] = sLam[i][2];
+  }
+  // u values (pair p = lane + 64 k: element e0 + p / NP, node p % NP) and
   // neighbour traces (pair p as above), by slot (compile-time constants, so
   // the arrays stay in registers)
-  double U[2][UPL], U0[2][UPL], P[2][GPL];
+  double U[2][UPL], P[2][GPL];
   int64_t TV[2] = {0, 0};  // tile records by slot
   int TC[2] = {1, 1};
-  const bool with_u0 = a.mode == 0 && a.a0 != 0.0;
   // Loads of tile n (clamped to the block's last tile, so that every phase
   // issues the same loads) into a slot.
   auto issue = [&](int n, auto SL) {
@@ -347,18 +376,6 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       const int64_t i = (int64_t)(e < tl.count ? e : tl.count - 1) * NP + p % NP;
       U[slot][k] = a.u[tl.vbase + i];
     }
-    if (with_u0) {
-#pragma unroll
-      for (int k = 0; k < UPL; ++k) {
-        const int p = lane + 64 * k;
-        const int e = e0 + p / NP;
-        const int64_t i = (int64_t)(e < tl.count ? e : tl.count - 1) * NP + p % NP;
-        U0[slot][k] = a.u0[tl.vbase + i];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < UPL; ++k) U0[slot][k] = 0.0;
-    }
 #pragma unroll
     for (int k = 0; k < GPL; ++k) {
       const int p = lane + 64 * k;
@@ -369,8 +386,8 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     TV[slot] = tl.vbase;
     TC[slot] = tl.count;
   };
-  // Tile n's values (loaded into slot) -> LDS u, u0, the per-element
-  // metrics and correction, and B.
+  // Tile n's values (loaded into slot) -> LDS u, the per-element metrics and
+  // correction, and B.
   auto prepare = [&](int n, auto SL) {
     constexpr int slot = decltype(SL)::value;
     const int lane = lane_now();
@@ -382,12 +399,17 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       const int p = lane + 64 * k;
       const int o = p < EPW * NP ? (e0 + p / NP) * NPU + p % NP : 16 * NPU;
       su[o] = U[slot][k];
-      sU0[bu][o] = U0[slot][k];
     }
     if (q == 0 && lane == 0) {
       sTv[bu] = TV[slot];
       sTc[bu] = TC[slot];
     }
+    wsync();  // the wave's u of the tile is in LDS
+    // The jumps' own traces are read here and used after the volume columns,
+    // whose arithmetic covers their latency.
+    double MV[GPL];
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) MV[k] = su[fw[k] & 4095];
     // f = u^2/2 at the vertex nodes of the wave's elements, read from the
     // lanes that hold them: f(v0) and the slopes f(v1..3) - f(v0) of the
     // linear interpolant, uniform per element
@@ -446,23 +468,22 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
           d2 = L2[jj];
           d3 = L3[jj];
         }
-      const float4 lm = *(const float4*)sLam[i];
       const double x = U[slot][k];
-      const double wv = 0.5 * x * x - (f0 + d1 * (double)lm.x + d2 * (double)lm.y + d3 * (double)lm.z);
+      const double wv =
+          0.5 * x * x - (f0 + d1 * (double)LX[k] + d2 * (double)LY[k] + d3 * (double)LZ[k]);
       const float v = (float)wv;
       const __bf16 h = (__bf16)v;
       uint16_t* b = &sB[bu][0][e0 + j][ok ? i : DUMP + (lane & 7)];
       b[0] = __builtin_bit_cast(uint16_t, h);
       b[16 * KBR] = bf16_of(v - (float)h);  // part 1: 16 rows further
     }
-    wsync();  // the wave's u of the tile is in LDS
     // jump columns: the jump at point qq of face f of element j
 #pragma unroll
     for (int k = 0; k < GPL; ++k) {
       const uint32_t fk = fw[k];
-      const int iu = fk & 4095, col = (fk >> 12) & 511, sl = (fk >> 21) & 31, j = fk >> 26;
-      const FRec& r = R[sl];
-      const double m = su[iu], pv = P[slot][k];
+      const int col = (fk >> 12) & 511, j = fk >> 26;
+      const FRec& r = R[(fk >> 21) & 31];
+      const double m = MV[k], pv = P[slot][k];
       const double x =
           (m - pv) * fma((double)r.fa, m + pv, -(double)r.fb * fmax(fabs(m), fabs(pv)));
       const float v = (float)x;
