@@ -70,6 +70,16 @@ struct FCfg {
   static constexpr size_t NFRAG = (size_t)MT * KS * 64;   // fragments per part
   // vertex nodes of the reference tet in the nodal order (checked on the host)
   static constexpr int VTX[4] = {0, N, (N + 1) * (N + 2) / 2 - 1, NP - 1};
+  __device__ static int vtx(int v) {
+    return v == 0 ? VTX[0] : v == 1 ? VTX[1] : v == 2 ? VTX[2] : VTX[3];
+  }
+  // the 64 values of one step span at most two elements
+  static constexpr bool two_per_step() {
+    for (int k = 0; k < UPL; ++k)
+      if ((64 * k + 63) / NP - (64 * k) / NP > 1) return false;
+    return true;
+  }
+  static_assert(two_per_step(), "a value step spans more than two elements");
 };
 
 // Face record of (face f, element j) of a memory wave's elements: value
@@ -101,13 +111,6 @@ struct FusedArgs {
 
 __device__ __forceinline__ uint16_t bf16_of(float x) {
   return __builtin_bit_cast(uint16_t, (__bf16)x);
-}
-
-__device__ __forceinline__ double readlane_f64(double x, int l) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, x);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // (The tile list is a separate const __restrict__ argument: the compiler may
@@ -344,17 +347,29 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     const int col = ok ? 32 * (KV + f * KL) + qq : DUMP + (lane & 7);
     fw[k] = (uint32_t)iu | ((uint32_t)col << 12) | ((uint32_t)sl << 21) | ((uint32_t)j << 26);
   }
-  // Barycentric weights of the node of value k (p = lane + 64 k, node p % NP),
-  // fixed for the launch.
+  // Per lane and value k (p = lane + 64 k: element j = p / NP, node
+  // i = p % NP), fixed for the launch: the barycentric weights of node i, and
+  // a word with the value's column in B (12 bits; the padding past the
+  // values), whether j is the later of the two elements the lanes of step k
+  // hold (bit 12) and its index in a tile's u (bits 16..27; the dump slot
+  // past the values).
   float LX[UPL], LY[UPL], LZ[UPL];
+  uint32_t vw[UPL];
 #pragma unroll
   for (int k = 0; k < UPL; ++k) {
     const int p = lane + 64 * k;
-    const int i = (p < EPW * NP ? p : EPW * NP - 1) % NP;
+    const bool ok = EPW * NP % 64 == 0 || p < EPW * NP;
+    const int pc = ok ? p : EPW * NP - 1;
+    const int j = pc / NP, i = pc % NP;
     LX[k] = sLam[i][0];
     LY[k] = sLam[i][1];
     LZ[k] = sLam[i][2];
+    const int bo = (e0 + j) * KBR + (ok ? i : DUMP + (lane & 7));
+    const int so = ok ? (e0 + j) * NPU + i : 16 * NPU;
+    vw[k] = (uint32_t)bo | ((uint32_t)(j > (64 * k) / NP) << 12) | ((uint32_t)so << 16);
   }
+  // Lane 4 j + v (< 16) reads u at vertex v of element j from a tile's u.
+  const int vxo = (e0 + ((lane >> 2) & 3)) * NPU + C::vtx(lane & 3);
   // u values (pair p = lane + 64 k: element e0 + p / NP, node p % NP) and
   // neighbour traces (pair p as above), by slot (compile-time constants, so
   // the arrays stay in registers)
@@ -395,45 +410,39 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     const FRec* R = sRec[q][n & 3];
     double* su = sU[bu];
 #pragma unroll
-    for (int k = 0; k < UPL; ++k) {
-      const int p = lane + 64 * k;
-      const int o = p < EPW * NP ? (e0 + p / NP) * NPU + p % NP : 16 * NPU;
-      su[o] = U[slot][k];
-    }
+    for (int k = 0; k < UPL; ++k) su[vw[k] >> 16] = U[slot][k];
     if (q == 0 && lane == 0) {
       sTv[bu] = TV[slot];
       sTc[bu] = TC[slot];
     }
     wsync();  // the wave's u of the tile is in LDS
-    // The jumps' own traces are read here and used after the volume columns,
-    // whose arithmetic covers their latency.
+    // The jumps' own traces and the vertex values are read here; the jumps
+    // use theirs after the volume columns, whose arithmetic covers the
+    // latency.
     double MV[GPL];
 #pragma unroll
     for (int k = 0; k < GPL; ++k) MV[k] = su[fw[k] & 4095];
-    // f = u^2/2 at the vertex nodes of the wave's elements, read from the
-    // lanes that hold them: f(v0) and the slopes f(v1..3) - f(v0) of the
-    // linear interpolant, uniform per element
-    // (made wave-uniform again after the arithmetic, so that they are held
-    // in scalar registers)
-    double L0[EPW], L1[EPW], L2[EPW], L3[EPW];
+    const double xv = su[vxo];
+    const double* G = sGm[q][n & 3][lane & 3];
+    const double ga = G[0], gb = G[1], gc = G[2];
+    // f = u^2/2 at vertex v of element j on lane 4 j + v, its difference
+    // from f at vertex 0 (a quad broadcast), then uniform per element: f(v0)
+    // and the slopes f(v1..3) - f(v0) of the linear interpolant
+    const float fv = (float)(0.5 * xv * xv);
+    const float dv = fv - __builtin_bit_cast(
+                              float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, fv), 0x00, 0xf, 0xf, false));
+    float L0[EPW], L1[EPW], L2[EPW], L3[EPW];
 #pragma unroll
     for (int j = 0; j < EPW; ++j) {
-      double fv[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int p = j * NP + C::VTX[v];
-        const double x = readlane_f64(U[slot][p / 64], p % 64);
-        fv[v] = 0.5 * x * x;
-      }
-      L0[j] = readlane_f64(fv[0], 0);
-      L1[j] = readlane_f64(fv[1] - fv[0], 0);
-      L2[j] = readlane_f64(fv[2] - fv[0], 0);
-      L3[j] = readlane_f64(fv[3] - fv[0], 0);
+      L0[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, fv), 4 * j));
+      L1[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dv), 4 * j + 1));
+      L2[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dv), 4 * j + 2));
+      L3[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dv), 4 * j + 3));
     }
     // per element (lane j < EPW): the metrics and the correction
     // -(ga a1 + gb a2 + gc a3)/2 for the interpolant's slopes
     if (lane < EPW) {
-      double s1 = L1[0], s2 = L2[0], s3 = L3[0];
+      float s1 = L1[0], s2 = L2[0], s3 = L3[0];
 #pragma unroll
       for (int j = 1; j < EPW; ++j)
         if (lane == j) {
@@ -441,39 +450,28 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
           s2 = L2[j];
           s3 = L3[j];
         }
-      const double* G = sGm[q][n & 3][lane];
-      const double ga = G[0], gb = G[1], gc = G[2];
       double* E = sE[bu][e0 + lane];
       E[0] = ga;
       E[1] = gb;
       E[2] = gc;
-      E[3] = -0.5 * (ga * s1 + gb * s2 + gc * s3);
+      E[3] = -0.5 * (ga * (double)s1 + gb * (double)s2 + gc * (double)s3);
     }
-    // volume columns: w = f - Lf at node i of element j
+    // volume columns: w = f - Lf at node i of element j (the interpolant in
+    // FP32: its rounding enters w as the FP32 weights' does)
+    uint16_t* bB = &sB[bu][0][0][0];
 #pragma unroll
     for (int k = 0; k < UPL; ++k) {
-      const int p = lane + 64 * k;
-      const bool ok = EPW * NP % 64 == 0 || p < EPW * NP;
-      const int pc = ok ? p : EPW * NP - 1;
-      const int j = pc / NP, i = pc % NP;
-      // the lanes of step k hold elements jlo .. jhi
+      const uint32_t vk = vw[k];
       const int jlo = (64 * k) / NP;
-      const int jhi = (64 * k + 63) / NP < EPW - 1 ? (64 * k + 63) / NP : EPW - 1;
-      double f0 = L0[jlo], d1 = L1[jlo], d2 = L2[jlo], d3 = L3[jlo];
-#pragma unroll
-      for (int jj = jlo + 1; jj <= jhi; ++jj)
-        if (j >= jj) {
-          f0 = L0[jj];
-          d1 = L1[jj];
-          d2 = L2[jj];
-          d3 = L3[jj];
-        }
+      const int jhi = jlo + 1 < EPW ? jlo + 1 : EPW - 1;
+      const bool js = (vk >> 12) & 1;
+      const float f0 = js ? L0[jhi] : L0[jlo], d1 = js ? L1[jhi] : L1[jlo];
+      const float d2 = js ? L2[jhi] : L2[jlo], d3 = js ? L3[jhi] : L3[jlo];
+      const float lf = fmaf(d3, LZ[k], fmaf(d2, LY[k], fmaf(d1, LX[k], f0)));
       const double x = U[slot][k];
-      const double wv =
-          0.5 * x * x - (f0 + d1 * (double)LX[k] + d2 * (double)LY[k] + d3 * (double)LZ[k]);
-      const float v = (float)wv;
+      const float v = (float)(0.5 * x * x - (double)lf);
       const __bf16 h = (__bf16)v;
-      uint16_t* b = &sB[bu][0][e0 + j][ok ? i : DUMP + (lane & 7)];
+      uint16_t* b = bB + (vk & 4095);
       b[0] = __builtin_bit_cast(uint16_t, h);
       b[16 * KBR] = bf16_of(v - (float)h);  // part 1: 16 rows further
     }
