@@ -195,10 +195,13 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
   // face s / EPW of element EPW q + s % EPW (clamped to the tile).
   int32_t rnb = 0, rcode = 0;
   double rga = 0.0, rgb = 0.0, rgc = 0.0;
+  // (Every lane loads, lane l taking the record of lane l % (4 EPW), and
+  // tiles past the block's last read its last: the loads are the same in
+  // every phase, so the waits for them count exactly.)
+  static_assert((4 * EPW & (4 * EPW - 1)) == 0, "records per wave: a power of two");
   auto rec_load = [&](int qq, int n) {
-    const int ln = lane_now(), rf = ln / EPW, rj = ln % EPW;
-    if (n >= nb || ln >= 4 * EPW) return;
-    const dgk::Tile tl = tile_of(n);
+    const int ln = lane_now() & (4 * EPW - 1), rf = ln / EPW, rj = ln % EPW;
+    const dgk::Tile tl = tile_of(n < nb ? n : nb - 1);
     const int e = EPW * qq + rj < tl.count ? EPW * qq + rj : tl.count - 1;
     const int64_t ge = tl.gbase + e;
     rnb = a.nbr[(int64_t)rf * K + ge];
@@ -496,14 +499,14 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
   // and tile 3's loaded, loads of tiles 0 and 1, tile 0 prepared.
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
-#pragma unroll 1
-  for (int n = 0; n < 3; ++n) {
-    rec_load(q, n);
-    rec_store(q, n);
-  }
-  rec_load(q, 3);
-  wsync();
   if (nb > 0) {
+#pragma unroll 1
+    for (int n = 0; n < 3; ++n) {
+      rec_load(q, n);
+      rec_store(q, n);
+    }
+    rec_load(q, 3);
+    wsync();
     issue(0, S0{});
     issue(1, S1{});
     prepare(0, S0{});
