@@ -385,9 +385,10 @@ def p6_section(dev, seed: int, steps: int = 5, reps: int = 10) -> dict:
     1,572,864-tet periodic P=6 problem (2 slab partitions, u0 = 0.5 + 0.4
     sin(2 pi x) cos(2 pi y) sin(2 pi z) + U(+-1e-6)). SSP-RK3 steps with the
     Dr/Ds/Dt contractions in FP64, in the exact-integer split (I8X6) and in
-    BF16x3 (LIFT in FP64 for both), each after a sustained warm-up and compared
+    BF16x3 (LIFT in FP64 for both), and with D and LIFT in BF16x3 (the fused
+    stage: one launch per stage), each after a sustained warm-up and compared
     with the FP64 run after the same steps; and the physical derivative alone
-    (three outputs) in the same three plans."""
+    (three outputs) in the FP64, I8X6 and BF16x3 plans."""
     from dgkernel_amd import _lib, dg
     from dgkernel_amd import mesh as M
     N = 6
@@ -401,7 +402,7 @@ def p6_section(dev, seed: int, steps: int = 5, reps: int = 10) -> dict:
     out = {"workload": f"Burgers SSP-RK3, 64^3x6={K} periodic tets, P=6, 2 slab partitions, "
                        f"{steps} steps per plan", "stage": {}, "derivative": {}}
     ref, hmin = None, None
-    for name, prec in (("f64", 0), ("i8x6", 4), ("bf16x3_lift64", 2 | 16)):
+    for name, prec in (("f64", 0), ("i8x6", 4), ("bf16x3_lift64", 2 | 16), ("bf16x3", 2)):
         solver = dg.BurgersSolver(dev, st, cfl=0.3, hmin=hmin)
         hmin = solver.hmin
         solver.mesh.set_precision(prec)
@@ -420,6 +421,7 @@ def p6_section(dev, seed: int, steps: int = 5, reps: int = 10) -> dict:
             ref = u
         else:
             rec["err_vs_f64"] = _errs(u, ref)
+            rec["speedup_vs_f64"] = out["stage"]["f64"]["ms_per_step"] / ms
         out["stage"][name] = rec
         log(f"p6 stage {name}: {ms:.3f} ms/step {rec.get('err_vs_f64', '')}")
         del solver
