@@ -97,8 +97,9 @@ struct dgk_dgmesh_s {
   void* lift_frag = nullptr;             // LIFT fragments for lift_prec
   int split_blocks = 0;                  // persistent grid of the surface kernel
   // Fused BF16x3 stage at P = 5, 6 (fused_hi.hip)
-  void* fused_frag = nullptr;            // [-Dr | -Ds | -Dt | LIFT] BF16 hi/lo fragments
-  float* fused_lam = nullptr;            // [3][Np] barycentric weights of the nodes
+  void* fused_frag = nullptr;            // [-Dr | -Ds | -Dt | LIFT] BF16 part fragments
+  float* fused_lam = nullptr;            // [6][Np] barycentric weights of the nodes (FP32, remainders)
+  int fused_npart = 0;                   // BF16 parts of the fragments built (2: BF16x3, 3: BF16x6)
 };
 
 namespace dgk {

@@ -99,7 +99,7 @@ struct FusedArgs {
   int64_t K;
   const int32_t* nbr;         // [4][K]
   const int32_t* codes;       // [K]
-  const float* lam;           // [3][Np] barycentric weights of the nodes
+  const float* lam;           // [6][Np] barycentric weights of the nodes, FP32 and remainders
   const double* u;
   const double* u0;
   double* out;
@@ -116,20 +116,24 @@ __device__ __forceinline__ uint16_t bf16_of(float x) {
 // (The tile list is a separate const __restrict__ argument: the compiler may
 // then read its wave-uniform entries through the scalar cache, whose wait
 // counter does not hold back on the vector loads in flight.)
-template <int N>
+template <int N, int NPART>
 __global__ void __launch_bounds__(FCfg<N>::NT)
-    fused_bf3_kernel(FusedArgs a, const dgk::Tile* __restrict__ tiles) {
+    fused_bf16_kernel(FusedArgs a, const dgk::Tile* __restrict__ tiles) {
+  static_assert(NPART == 2 || NPART == 3, "BF16x3 (two parts) or BF16x6 (three parts)");
   using C = FCfg<N>;
+  // FP64 interpolant for BF16x6, whose products carry less error than its
+  // FP32 rounding would add
+  using LT = typename std::conditional<NPART == 3, double, float>::type;
   constexpr int NP = C::NP, NFP = C::NFP, MT = C::MT, KV = C::KV, KL = C::KL, KS = C::KS;
   constexpr int KB = C::KB, MW = C::MW, EPW = C::EPW, NPU = C::NPU, UPL = C::UPL, GPL = C::GPL;
   constexpr dgk::FmaskTable<N> FM{};
-  // B of two tiles: [buffer][part][element][k] BF16 (hi, lo), k = 32 s + the
+  // B of two tiles: [buffer][part][element][k] BF16 parts, k = 32 s + the
   // step's column: w at the nodes (32 KV columns), then the jumps face by
   // face (32 KL columns each). Rows are padded by 8 columns, so that the 16
   // elements of a 16-byte read fall in distinct banks; the padding also takes
   // the writes of lanes past the tile's values.
   constexpr int KBR = 32 * KB + 8, DUMP = 32 * KB;
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][16][KBR];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][NPART][16][KBR];
   // u of two tiles (+ a dump slot), and their tile records, staged by the
   // memory waves for the matrix waves' update
   __shared__ double sU[2][16 * NPU + 1];
@@ -142,18 +146,22 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
   __shared__ FRec sRec[MW][4][4 * EPW];
   __shared__ double sGm[MW][4][EPW][4];
   __shared__ __attribute__((aligned(16))) float sLam[NP][4];
+  // BF16x6: the FP32 remainders of the weights
+  __shared__ __attribute__((aligned(16))) float sLamLo[NPART == 3 ? NP : 1][4];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[4 * NFP];
   // LIFT fragments of the matrix waves (their Dr, Ds, Dt fragments are held
-  // in registers): [row tile][face step][hi, lo][lane]
-  __shared__ bf16x8 sFL[MT][4 * KL][2][64];
+  // in registers): [row tile][face step][part][lane]
+  __shared__ bf16x8 sFL[MT][4 * KL][NPART][64];
   {
     uint4* z = (uint4*)&sB[0][0][0][0];
     for (int i = threadIdx.x; i < (int)(sizeof(sB) / 16); i += C::NT) z[i] = uint4{0, 0, 0, 0};
     for (int i = threadIdx.x; i < 4 * 24 * NFP; i += C::NT) sFN[i] = a.face_nodes[i];
     for (int i = threadIdx.x; i < 4 * NFP; i += C::NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
-    for (int i = threadIdx.x; i < 4 * NP; i += C::NT)
+    for (int i = threadIdx.x; i < 4 * NP; i += C::NT) {
       sLam[i / 4][i % 4] = i % 4 < 3 ? a.lam[(i % 4) * NP + i / 4] : 0.f;
+      if (NPART == 3) sLamLo[i / 4][i % 4] = i % 4 < 3 ? a.lam[(3 + i % 4) * NP + i / 4] : 0.f;
+    }
   }
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = lane & 15, g = lane >> 4;
@@ -189,6 +197,42 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
+  // the value of lane 4 (l / 4) in lane l (a quad broadcast), and of lane s
+  // in every lane, for FP32 and FP64 values
+  auto quad_first = [](LT x) -> LT {
+    if constexpr (sizeof(LT) == 8) {
+      const uint64_t b = __builtin_bit_cast(uint64_t, x);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, 0x00, 0xf, 0xf, false);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), 0x00, 0xf, 0xf, false);
+      return __builtin_bit_cast(LT, ((uint64_t)hi << 32) | lo);
+    } else {
+      return __builtin_bit_cast(LT, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x00, 0xf, 0xf, false));
+    }
+  };
+  auto readlane_as = [](LT x, int l) -> LT {
+    if constexpr (sizeof(LT) == 8) {
+      const uint64_t b = __builtin_bit_cast(uint64_t, x);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+      return __builtin_bit_cast(LT, ((uint64_t)hi << 32) | lo);
+    } else {
+      return __builtin_bit_cast(LT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+    }
+  };
+  // The BF16 parts of an FP32 value into B (part q 16 q rows further): two
+  // parts hold ~16 bits, three the whole FP32 value.
+  auto put_parts = [](uint16_t* b, float v) {
+    const __bf16 h = (__bf16)v;
+    const float r = v - (float)h;
+    b[0] = __builtin_bit_cast(uint16_t, h);
+    if constexpr (NPART == 3) {
+      const __bf16 m = (__bf16)r;
+      b[16 * KBR] = __builtin_bit_cast(uint16_t, m);
+      b[32 * KBR] = bf16_of(r - (float)m);
+    } else {
+      b[16 * KBR] = bf16_of(r);
+    }
+  };
   // Face records of memory wave q's elements for tile n go to slot n % 4 of
   // sRec[q] (their folded metrics to sGm[q]), written by that wave three
   // phases ahead from loads it issued a phase before: lane s < 4 EPW takes
@@ -217,10 +261,15 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       FRec r;
       r.nb = rnb;
       r.fnrow = (rf * 24 + ((rcode >> (5 * rf)) & 31)) * NFP;
-      double fa, fb;
-      dgk::face_factors(rf, rga, rgb, rgc, fa, fb);
-      r.fa = (float)fa;
-      r.fb = (float)fb;
+      // Fscale n.(1,1,1) of face rf: -gc, -gb, ga + gb + gc, -ga (the face
+      // factors of dgk::face_factors), as coefficients in {-1, 0, 1} of
+      // (ga, gb, gc) rather than a branch on the lane's face
+      const double c0 = (double)(rf == 2) - (double)(rf == 3);
+      const double c1 = (double)(rf == 2) - (double)(rf == 1);
+      const double c2 = (double)(rf == 2) - (double)(rf == 0);
+      const double sn = fma(c0, rga, fma(c1, rgb, c2 * rgc));
+      r.fa = (float)(0.25 * sn);
+      r.fb = (float)(0.5 * fabs(sn));
       sRec[qq][n & 3][ln] = r;
       if (rf == 0) {
         sGm[qq][n & 3][rj][0] = rga;
@@ -233,16 +282,14 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
 
   if (w < MT) {
     // ---------------- matrix wave: rows 16 w .. 16 w + 15 ----------------
-    bf16x8 fh[3 * KV], fl[3 * KV];
+    bf16x8 fa[NPART][3 * KV];
 #pragma unroll
-    for (int s = 0; s < 3 * KV; ++s) {
-      fh[s] = a.frag[((size_t)w * KS + s) * 64 + lane];
-      fl[s] = a.frag[C::NFRAG + ((size_t)w * KS + s) * 64 + lane];
-    }
+    for (int q = 0; q < NPART; ++q) {
 #pragma unroll
-    for (int t = 0; t < 4 * KL; ++t) {
-      sFL[w][t][0][lane] = a.frag[((size_t)w * KS + 3 * KV + t) * 64 + lane];
-      sFL[w][t][1][lane] = a.frag[C::NFRAG + ((size_t)w * KS + 3 * KV + t) * 64 + lane];
+      for (int s = 0; s < 3 * KV; ++s) fa[q][s] = a.frag[q * C::NFRAG + ((size_t)w * KS + s) * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < 4 * KL; ++t)
+        sFL[w][t][q][lane] = a.frag[q * C::NFRAG + ((size_t)w * KS + 3 * KV + t) * 64 + lane];
     }
     const double dt = a.mode == 0 ? *a.dt : 1.0;
     const double a0 = a.mode == 0 ? a.a0 : 0.0, a1 = a.mode == 0 ? a.a1 : 1.0;
@@ -277,30 +324,44 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       // 4 (l / 16) + r and each store below writes 16 consecutive rows of an
       // element. -Dr w, -Ds w, -Dt w and LIFT g accumulate apart; the
       // element's metrics scale them in the update.
+      // The split products, smallest first: BF16x3 b0 a1 + b1 a0 + b0 a0,
+      // BF16x6 b0 a2 + b2 a0 + b1 a1 + b0 a1 + b1 a0 + b0 a0 (part 0 the
+      // leading one).
+      auto prod = [](floatx4 acc, const bf16x8* b, const bf16x8* op) {
+        if constexpr (NPART == 3) {
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], op[2], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[2], op[0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1], op[1], acc, 0, 0, 0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], op[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1], op[0], acc, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0], op[0], acc, 0, 0, 0);
+      };
       floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aS = aR, aT = aR, aL = aR;
       __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int s = 0; s < KV; ++s) {
-        const bf16x8 bh = *(const bf16x8*)&sB[bu][0][c][32 * s + 8 * g];
-        const bf16x8 bl = *(const bf16x8*)&sB[bu][1][c][32 * s + 8 * g];
-        aR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[s], aR, 0, 0, 0);
-        aS = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[KV + s], aS, 0, 0, 0);
-        aT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fl[2 * KV + s], aT, 0, 0, 0);
-        aR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[s], aR, 0, 0, 0);
-        aS = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[KV + s], aS, 0, 0, 0);
-        aT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, fh[2 * KV + s], aT, 0, 0, 0);
-        aR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[s], aR, 0, 0, 0);
-        aS = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[KV + s], aS, 0, 0, 0);
-        aT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, fh[2 * KV + s], aT, 0, 0, 0);
+        bf16x8 b[NPART], oR[NPART], oS[NPART], oT[NPART];
+#pragma unroll
+        for (int q = 0; q < NPART; ++q) {
+          b[q] = *(const bf16x8*)&sB[bu][q][c][32 * s + 8 * g];
+          oR[q] = fa[q][s];
+          oS[q] = fa[q][KV + s];
+          oT[q] = fa[q][2 * KV + s];
+        }
+        aR = prod(aR, b, oR);
+        aS = prod(aS, b, oS);
+        aT = prod(aT, b, oT);
       }
 #pragma unroll
       for (int t = 0; t < 4 * KL; ++t) {
-        const bf16x8 bh = *(const bf16x8*)&sB[bu][0][c][32 * (KV + t) + 8 * g];
-        const bf16x8 bl = *(const bf16x8*)&sB[bu][1][c][32 * (KV + t) + 8 * g];
-        const bf16x8 lh = sFL[w][t][0][lane], ll = sFL[w][t][1][lane];
-        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ll, aL, 0, 0, 0);
-        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, lh, aL, 0, 0, 0);
-        aL = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, lh, aL, 0, 0, 0);
+        bf16x8 b[NPART], oL[NPART];
+#pragma unroll
+        for (int q = 0; q < NPART; ++q) {
+          b[q] = *(const bf16x8*)&sB[bu][q][c][32 * (KV + t) + 8 * g];
+          oL[q] = sFL[w][t][q][lane];
+        }
+        aL = prod(aL, b, oL);
       }
       __builtin_amdgcn_s_setprio(0);
       // Stores through a buffer resource sized to the tile: the ones past it
@@ -431,21 +492,27 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     // f = u^2/2 at vertex v of element j on lane 4 j + v, its difference
     // from f at vertex 0 (a quad broadcast), then uniform per element: f(v0)
     // and the slopes f(v1..3) - f(v0) of the linear interpolant
-    const float fv = (float)(0.5 * xv * xv);
-    const float dv = fv - __builtin_bit_cast(
-                              float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, fv), 0x00, 0xf, 0xf, false));
-    float L0[EPW], L1[EPW], L2[EPW], L3[EPW];
+    const LT fv = (LT)(0.5 * xv * xv);
+    const LT dv = fv - quad_first(fv);
+    // Element j's values, uniform: held for all elements in FP32; in FP64
+    // (64 scalar registers for four elements) read again where used, the
+    // empty asm keeping the compiler from merging the reads into one long
+    // live range.
+    auto elem = [&](int j, LT& f0, LT& d1, LT& d2, LT& d3) {
+      LT x = fv, y = dv;
+      if constexpr (NPART == 3) asm volatile("" : "+v"(x), "+v"(y));
+      f0 = readlane_as(x, 4 * j);
+      d1 = readlane_as(y, 4 * j + 1);
+      d2 = readlane_as(y, 4 * j + 2);
+      d3 = readlane_as(y, 4 * j + 3);
+    };
+    LT L0[EPW], L1[EPW], L2[EPW], L3[EPW];
 #pragma unroll
-    for (int j = 0; j < EPW; ++j) {
-      L0[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, fv), 4 * j));
-      L1[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dv), 4 * j + 1));
-      L2[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dv), 4 * j + 2));
-      L3[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dv), 4 * j + 3));
-    }
+    for (int j = 0; j < EPW; ++j) elem(j, L0[j], L1[j], L2[j], L3[j]);
     // per element (lane j < EPW): the metrics and the correction
     // -(ga a1 + gb a2 + gc a3)/2 for the interpolant's slopes
     if (lane < EPW) {
-      float s1 = L1[0], s2 = L2[0], s3 = L3[0];
+      LT s1 = L1[0], s2 = L2[0], s3 = L3[0];
 #pragma unroll
       for (int j = 1; j < EPW; ++j)
         if (lane == j) {
@@ -459,8 +526,9 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       E[2] = gc;
       E[3] = -0.5 * (ga * (double)s1 + gb * (double)s2 + gc * (double)s3);
     }
-    // volume columns: w = f - Lf at node i of element j (the interpolant in
-    // FP32: its rounding enters w as the FP32 weights' does)
+    // volume columns: w = f - Lf at node i of element j (BF16x3: the
+    // interpolant in FP32, its rounding entering w as the FP32 weights' does;
+    // BF16x6: in FP64 with the weights' FP32 remainders)
     uint16_t* bB = &sB[bu][0][0][0];
 #pragma unroll
     for (int k = 0; k < UPL; ++k) {
@@ -468,15 +536,31 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       const int jlo = (64 * k) / NP;
       const int jhi = jlo + 1 < EPW ? jlo + 1 : EPW - 1;
       const bool js = (vk >> 12) & 1;
-      const float f0 = js ? L0[jhi] : L0[jlo], d1 = js ? L1[jhi] : L1[jlo];
-      const float d2 = js ? L2[jhi] : L2[jlo], d3 = js ? L3[jhi] : L3[jlo];
-      const float lf = fmaf(d3, LZ[k], fmaf(d2, LY[k], fmaf(d1, LX[k], f0)));
+      LT f0, d1, d2, d3;
+      if constexpr (NPART == 3) {
+        LT g0, g1, g2, g3, h0, h1, h2, h3;
+        elem(jlo, g0, g1, g2, g3);
+        elem(jhi, h0, h1, h2, h3);
+        f0 = js ? h0 : g0;
+        d1 = js ? h1 : g1;
+        d2 = js ? h2 : g2;
+        d3 = js ? h3 : g3;
+      } else {
+        f0 = js ? L0[jhi] : L0[jlo];
+        d1 = js ? L1[jhi] : L1[jlo];
+        d2 = js ? L2[jhi] : L2[jlo];
+        d3 = js ? L3[jhi] : L3[jlo];
+      }
+      LT lf;
+      if constexpr (NPART == 3) {
+        const float* lo = sLamLo[((vk >> 16) % NPU < NP ? (vk >> 16) % NPU : 0)];
+        lf = fma(d3, (double)LZ[k] + (double)lo[2],
+                 fma(d2, (double)LY[k] + (double)lo[1], fma(d1, (double)LX[k] + (double)lo[0], f0)));
+      } else {
+        lf = fmaf(d3, LZ[k], fmaf(d2, LY[k], fmaf(d1, LX[k], f0)));
+      }
       const double x = U[slot][k];
-      const float v = (float)(0.5 * x * x - (double)lf);
-      const __bf16 h = (__bf16)v;
-      uint16_t* b = bB + (vk & 4095);
-      b[0] = __builtin_bit_cast(uint16_t, h);
-      b[16 * KBR] = bf16_of(v - (float)h);  // part 1: 16 rows further
+      put_parts(bB + (vk & 4095), (float)(0.5 * x * x - (double)lf));
     }
     // jump columns: the jump at point qq of face f of element j
 #pragma unroll
@@ -487,11 +571,7 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       const double m = MV[k], pv = P[slot][k];
       const double x =
           (m - pv) * fma((double)r.fa, m + pv, -(double)r.fb * fmax(fabs(m), fabs(pv)));
-      const float v = (float)x;
-      const __bf16 h = (__bf16)v;
-      uint16_t* b = &sB[bu][0][e0 + j][col];
-      b[0] = __builtin_bit_cast(uint16_t, h);
-      b[16 * KBR] = bf16_of(v - (float)h);
+      put_parts(&sB[bu][0][e0 + j][col], (float)x);
     }
   };
 
@@ -549,13 +629,13 @@ float bf16_float_h(uint16_t b) {
   return f;
 }
 
-template <int N>
+template <int N, int NPART>
 int fused_launch(dgk_dgmesh m, const FusedArgs& a) {
   if (a.ntiles == 0) return DGK_OK;
   int grid = m->dev->cu_count;
   if (grid > a.ntiles) grid = a.ntiles;
-  hipLaunchKernelGGL((fused_bf3_kernel<N>), dim3(grid), dim3(FCfg<N>::NT), 0, m->dev->stream, a,
-                     a.tiles);
+  hipLaunchKernelGGL((fused_bf16_kernel<N, NPART>), dim3(grid), dim3(FCfg<N>::NT), 0, m->dev->stream,
+                     a, a.tiles);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
 }
@@ -563,18 +643,22 @@ int fused_launch(dgk_dgmesh m, const FusedArgs& a) {
 }  // namespace
 
 bool dgk::fused_applies(dgk_dgmesh m) {
-  return (m->N == 5 || m->N == 6) && m->prec == 2 && m->lift_prec == 2 && m->variant != 3;
+  return (m->N == 5 || m->N == 6) && (m->prec == DGK_PREC_BF16X3 || m->prec == DGK_PREC_BF16X6) &&
+         m->lift_prec == m->prec && m->variant != 3;
 }
 
 int dgk::fused_prepare(dgk_dgmesh m) {
-  if (m->fused_frag) return DGK_OK;
+  const int npart = m->prec == DGK_PREC_BF16X6 ? 3 : 2;
+  if (m->fused_frag && m->fused_npart == npart) return DGK_OK;
+  fused_release(m);
   const int N = m->N, NP = dgk::np_of(N), NFP = dgk::nfp_of(N), NF = 4 * NFP;
   const int MT = (NP + 15) / 16, KV = (NP + 31) / 32, KL = (NFP + 31) / 32, KS = 3 * KV + 4 * KL;
   const size_t nfrag = (size_t)MT * KS * 64;
   // A = [-Dr | -Ds | -Dt | LIFT] in the 16x16x32 BF16 A-operand lane order:
   // lane l of step s of row tile t holds A[16 t + l % 16][32 s' + 8 (l / 16) + e],
-  // e = 0..7, within the block (volume block s / KV, or face (s - 3 KV) / KL).
-  std::vector<uint16_t> buf(2 * nfrag * 8, 0);
+  // e = 0..7, within the block (volume block s / KV, or face (s - 3 KV) / KL);
+  // part q is the BF16 value of what parts 0..q-1 leave of the FP64 entry.
+  std::vector<uint16_t> buf((size_t)npart * nfrag * 8, 0);
   const std::vector<double>* D[3] = {&m->Dr_h, &m->Ds_h, &m->Dt_h};
   for (int t = 0; t < MT; ++t)
     for (int s = 0; s < KS; ++s)
@@ -593,9 +677,12 @@ int dgk::fused_prepare(dgk_dgmesh m) {
             }
           }
           const size_t q = (((size_t)t * KS + s) * 64 + l) * 8 + e;
-          const uint16_t h = bf16_bits_h((float)v);
-          buf[q] = h;
-          buf[nfrag * 8 + q] = bf16_bits_h((float)(v - (double)bf16_float_h(h)));
+          double rest = v;
+          for (int part = 0; part < npart; ++part) {
+            const uint16_t h = bf16_bits_h((float)rest);
+            buf[part * nfrag * 8 + q] = h;
+            rest -= (double)bf16_float_h(h);
+          }
         }
   // the kernel takes the vertices of the reference tet at nodes 0, N,
   // (N+1)(N+2)/2 - 1 and Np - 1; barycentric weights of the nodes
@@ -607,14 +694,18 @@ int dgk::fused_prepare(dgk_dgmesh m) {
                     std::fabs(m->t_h[i] - V[v][2]) < 1e-10,
                 "fused stage: node %d is not vertex %d of the reference tet", i, v);
   }
-  std::vector<float> lam(3 * NP);
+  // [0..2][Np] the FP32 weights, [3..5][Np] their FP32 remainders
+  std::vector<float> lam(6 * NP);
   for (int i = 0; i < NP; ++i) {
-    lam[i] = (float)(0.5 * (1.0 + m->r_h[i]));
-    lam[NP + i] = (float)(0.5 * (1.0 + m->s_h[i]));
-    lam[2 * NP + i] = (float)(0.5 * (1.0 + m->t_h[i]));
+    const double l[3] = {0.5 * (1.0 + m->r_h[i]), 0.5 * (1.0 + m->s_h[i]), 0.5 * (1.0 + m->t_h[i])};
+    for (int d = 0; d < 3; ++d) {
+      lam[d * NP + i] = (float)l[d];
+      lam[(3 + d) * NP + i] = (float)(l[d] - (double)(float)l[d]);
+    }
   }
   int rc = dgk_malloc(m->dev, buf.size() * 2, buf.data(), &m->fused_frag);
   if (rc == DGK_OK) rc = dgk_malloc(m->dev, lam.size() * 4, lam.data(), (void**)&m->fused_lam);
+  if (rc == DGK_OK) m->fused_npart = npart;
   return rc;
 }
 
@@ -623,6 +714,7 @@ void dgk::fused_release(dgk_dgmesh m) {
   if (m->fused_lam) dgk_free(m->dev, m->fused_lam);
   m->fused_frag = nullptr;
   m->fused_lam = nullptr;
+  m->fused_npart = 0;
 }
 
 int dgk::fused_stage(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
@@ -647,9 +739,10 @@ int dgk::fused_stage(dgk_dgmesh m, const double* u, const double* u0, double* ou
   a.dt = dt;
   a.maxabs = maxabs;
   a.mode = mode;
+  const bool six = m->fused_npart == 3;
   switch (m->N) {
-    case 5: return fused_launch<5>(m, a);
-    case 6: return fused_launch<6>(m, a);
+    case 5: return six ? fused_launch<5, 3>(m, a) : fused_launch<5, 2>(m, a);
+    case 6: return six ? fused_launch<6, 3>(m, a) : fused_launch<6, 2>(m, a);
   }
   dgk::set_error("fused stage: N=%d outside 5..6", m->N);
   return DGK_ERR_UNSUPPORTED;
