@@ -73,13 +73,6 @@ struct FCfg {
   __device__ static int vtx(int v) {
     return v == 0 ? VTX[0] : v == 1 ? VTX[1] : v == 2 ? VTX[2] : VTX[3];
   }
-  // the 64 values of one step span at most two elements
-  static constexpr bool two_per_step() {
-    for (int k = 0; k < UPL; ++k)
-      if ((64 * k + 63) / NP - (64 * k) / NP > 1) return false;
-    return true;
-  }
-  static_assert(two_per_step(), "a value step spans more than two elements");
 };
 
 // Face record of (face f, element j) of a memory wave's elements: value
@@ -148,6 +141,9 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
   __shared__ __attribute__((aligned(16))) float sLam[NP][4];
   // BF16x6: the FP32 remainders of the weights
   __shared__ __attribute__((aligned(16))) float sLamLo[NPART == 3 ? NP : 1][4];
+  // per memory wave and element: f(v0) and the slopes f(v1..3) - f(v0) of
+  // the linear interpolant of f = u^2/2
+  __shared__ __attribute__((aligned(16))) LT sV[MW][EPW][4];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[4 * NFP];
   // LIFT fragments of the matrix waves (their Dr, Ds, Dt fragments are held
@@ -209,16 +205,7 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       return __builtin_bit_cast(LT, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x00, 0xf, 0xf, false));
     }
   };
-  auto readlane_as = [](LT x, int l) -> LT {
-    if constexpr (sizeof(LT) == 8) {
-      const uint64_t b = __builtin_bit_cast(uint64_t, x);
-      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
-      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
-      return __builtin_bit_cast(LT, ((uint64_t)hi << 32) | lo);
-    } else {
-      return __builtin_bit_cast(LT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
-    }
-  };
+
   // The BF16 parts of an FP32 value into B (part q 16 q rows further): two
   // parts hold ~16 bits, three the whole FP32 value.
   auto put_parts = [](uint16_t* b, float v) {
@@ -305,19 +292,24 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
       const int64_t vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tv >> 32)) << 32) |
                                       (uint32_t)__builtin_amdgcn_readfirstlane((int)tv));
       const int count = __builtin_amdgcn_readfirstlane(sTc[bu]);
-      // u0 of the rows this lane updates (below), loaded during the products;
-      // loads past the tile read zero
+      // u0 of the rows this lane updates (below), loaded during the products
+      // (BF16x6: during the LIFT products only, so that it is not held across
+      // the volume products, where its registers would spill); loads past
+      // the tile read zero
       const int row = 16 * w + c;
       const int rr = row < NP ? row : NP - 1;
       double u0v[4] = {0.0, 0.0, 0.0, 0.0};
-      if (a0 != 0.0) {
-        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(a.u0 + vbase), 0, count * NP * 8, 0x00020000);
+      auto load_u0 = [&]() {
+        if (a0 != 0.0) {
+          const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+              (void*)(a.u0 + vbase), 0, count * NP * 8, 0x00020000);
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          u0v[r] = __builtin_bit_cast(
-              double, __builtin_amdgcn_raw_buffer_load_b64(rs0, ((4 * g + r) * NP + rr) * 8, 0, 2));
-      }
+          for (int r = 0; r < 4; ++r)
+            u0v[r] = __builtin_bit_cast(
+                double, __builtin_amdgcn_raw_buffer_load_b64(rs0, ((4 * g + r) * NP + rr) * 8, 0, 2));
+        }
+      };
+      if (NPART == 2) load_u0();
       // The products run transposed, B^T A^T: the tile's B as the first
       // operand (its registers are B's as read) and the fragments as the
       // second, so that lane l, register r holds row 16 w + l % 16 of element
@@ -353,6 +345,7 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
         aS = prod(aS, b, oS);
         aT = prod(aT, b, oT);
       }
+      if (NPART == 3) load_u0();
 #pragma unroll
       for (int t = 0; t < 4 * KL; ++t) {
         bf16x8 b[NPART], oL[NPART];
@@ -414,10 +407,12 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
   // Per lane and value k (p = lane + 64 k: element j = p / NP, node
   // i = p % NP), fixed for the launch: the barycentric weights of node i, and
   // a word with the value's column in B (12 bits; the padding past the
-  // values), whether j is the later of the two elements the lanes of step k
-  // hold (bit 12) and its index in a tile's u (bits 16..27; the dump slot
-  // past the values).
-  float LX[UPL], LY[UPL], LZ[UPL];
+  // values), j (bits 12..13) and its index in a tile's u (bits 16..27; the
+  // dump slot past the values).
+  // (BF16x6 reads the weights with their remainders from LDS instead: its
+  // memory waves have no registers to spare.)
+  constexpr int UPLW = NPART == 3 ? 1 : UPL;
+  float LX[UPLW], LY[UPLW], LZ[UPLW];
   uint32_t vw[UPL];
 #pragma unroll
   for (int k = 0; k < UPL; ++k) {
@@ -425,12 +420,14 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     const bool ok = EPW * NP % 64 == 0 || p < EPW * NP;
     const int pc = ok ? p : EPW * NP - 1;
     const int j = pc / NP, i = pc % NP;
-    LX[k] = sLam[i][0];
-    LY[k] = sLam[i][1];
-    LZ[k] = sLam[i][2];
+    if (NPART == 2) {
+      LX[k % UPLW] = sLam[i][0];
+      LY[k % UPLW] = sLam[i][1];
+      LZ[k % UPLW] = sLam[i][2];
+    }
     const int bo = (e0 + j) * KBR + (ok ? i : DUMP + (lane & 7));
     const int so = ok ? (e0 + j) * NPU + i : 16 * NPU;
-    vw[k] = (uint32_t)bo | ((uint32_t)(j > (64 * k) / NP) << 12) | ((uint32_t)so << 16);
+    vw[k] = (uint32_t)bo | ((uint32_t)j << 12) | ((uint32_t)so << 16);
   }
   // Lane 4 j + v (< 16) reads u at vertex v of element j from a tile's u.
   const int vxo = (e0 + ((lane >> 2) & 3)) * NPU + C::vtx(lane & 3);
@@ -489,42 +486,22 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
     const double xv = su[vxo];
     const double* G = sGm[q][n & 3][lane & 3];
     const double ga = G[0], gb = G[1], gc = G[2];
-    // f = u^2/2 at vertex v of element j on lane 4 j + v, its difference
-    // from f at vertex 0 (a quad broadcast), then uniform per element: f(v0)
-    // and the slopes f(v1..3) - f(v0) of the linear interpolant
+    // f = u^2/2 at vertex v of element j on lane 4 j + v and its difference
+    // from f at vertex 0 (a quad broadcast) into the wave's table: f(v0) and
+    // the slopes f(v1..3) - f(v0) of the linear interpolant
     const LT fv = (LT)(0.5 * xv * xv);
     const LT dv = fv - quad_first(fv);
-    // Element j's values, uniform: held for all elements in FP32; in FP64
-    // (64 scalar registers for four elements) read again where used, the
-    // empty asm keeping the compiler from merging the reads into one long
-    // live range.
-    auto elem = [&](int j, LT& f0, LT& d1, LT& d2, LT& d3) {
-      LT x = fv, y = dv;
-      if constexpr (NPART == 3) asm volatile("" : "+v"(x), "+v"(y));
-      f0 = readlane_as(x, 4 * j);
-      d1 = readlane_as(y, 4 * j + 1);
-      d2 = readlane_as(y, 4 * j + 2);
-      d3 = readlane_as(y, 4 * j + 3);
-    };
-    LT L0[EPW], L1[EPW], L2[EPW], L3[EPW];
-#pragma unroll
-    for (int j = 0; j < EPW; ++j) elem(j, L0[j], L1[j], L2[j], L3[j]);
+    if (lane < 4 * EPW) sV[q][lane >> 2][lane & 3] = (lane & 3) == 0 ? fv : dv;
+    wsync();
     // per element (lane j < EPW): the metrics and the correction
     // -(ga a1 + gb a2 + gc a3)/2 for the interpolant's slopes
     if (lane < EPW) {
-      LT s1 = L1[0], s2 = L2[0], s3 = L3[0];
-#pragma unroll
-      for (int j = 1; j < EPW; ++j)
-        if (lane == j) {
-          s1 = L1[j];
-          s2 = L2[j];
-          s3 = L3[j];
-        }
+      const LT* V = sV[q][lane];
       double* E = sE[bu][e0 + lane];
       E[0] = ga;
       E[1] = gb;
       E[2] = gc;
-      E[3] = -0.5 * (ga * (double)s1 + gb * (double)s2 + gc * (double)s3);
+      E[3] = -0.5 * (ga * (double)V[1] + gb * (double)V[2] + gc * (double)V[3]);
     }
     // volume columns: w = f - Lf at node i of element j (BF16x3: the
     // interpolant in FP32, its rounding entering w as the FP32 weights' does;
@@ -533,31 +510,18 @@ __global__ void __launch_bounds__(FCfg<N>::NT)
 #pragma unroll
     for (int k = 0; k < UPL; ++k) {
       const uint32_t vk = vw[k];
-      const int jlo = (64 * k) / NP;
-      const int jhi = jlo + 1 < EPW ? jlo + 1 : EPW - 1;
-      const bool js = (vk >> 12) & 1;
-      LT f0, d1, d2, d3;
-      if constexpr (NPART == 3) {
-        LT g0, g1, g2, g3, h0, h1, h2, h3;
-        elem(jlo, g0, g1, g2, g3);
-        elem(jhi, h0, h1, h2, h3);
-        f0 = js ? h0 : g0;
-        d1 = js ? h1 : g1;
-        d2 = js ? h2 : g2;
-        d3 = js ? h3 : g3;
-      } else {
-        f0 = js ? L0[jhi] : L0[jlo];
-        d1 = js ? L1[jhi] : L1[jlo];
-        d2 = js ? L2[jhi] : L2[jlo];
-        d3 = js ? L3[jhi] : L3[jlo];
-      }
+      const int j = (vk >> 12) & 3;
+      const LT* V = sV[q][j];
+      const LT f0 = V[0], d1 = V[1], d2 = V[2], d3 = V[3];
       LT lf;
       if constexpr (NPART == 3) {
-        const float* lo = sLamLo[((vk >> 16) % NPU < NP ? (vk >> 16) % NPU : 0)];
-        lf = fma(d3, (double)LZ[k] + (double)lo[2],
-                 fma(d2, (double)LY[k] + (double)lo[1], fma(d1, (double)LX[k] + (double)lo[0], f0)));
+        const int i = (int)(vk & 4095) - (e0 + j) * KBR;  // past NP: the padding
+        const float4 hi = *(const float4*)sLam[i < NP ? i : 0];
+        const float4 lo = *(const float4*)sLamLo[i < NP ? i : 0];
+        lf = fma(d3, (double)hi.z + (double)lo.z,
+                 fma(d2, (double)hi.y + (double)lo.y, fma(d1, (double)hi.x + (double)lo.x, f0)));
       } else {
-        lf = fmaf(d3, LZ[k], fmaf(d2, LY[k], fmaf(d1, LX[k], f0)));
+        lf = fmaf(d3, LZ[k % UPLW], fmaf(d2, LY[k % UPLW], fmaf(d1, LX[k % UPLW], f0)));
       }
       const double x = U[slot][k];
       put_parts(bB + (vk & 4095), (float)(0.5 * x * x - (double)lf));
