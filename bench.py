@@ -385,8 +385,9 @@ def p6_section(dev, seed: int, steps: int = 5, reps: int = 10) -> dict:
     1,572,864-tet periodic P=6 problem (2 slab partitions, u0 = 0.5 + 0.4
     sin(2 pi x) cos(2 pi y) sin(2 pi z) + U(+-1e-6)). SSP-RK3 steps with the
     Dr/Ds/Dt contractions in FP64, in the exact-integer split (I8X6) and in
-    BF16x3 (LIFT in FP64 for both), and with D and LIFT in BF16x3 (the fused
-    stage: one launch per stage), each after a sustained warm-up and compared
+    BF16x3 (LIFT in FP64 for both), and with D and LIFT both in BF16x3 or both
+    in BF16x6 (the fused stage: one launch per stage), each after a sustained
+    warm-up and compared
     with the FP64 run after the same steps; and the physical derivative alone
     (three outputs) in the FP64, I8X6 and BF16x3 plans."""
     from dgkernel_amd import _lib, dg
@@ -402,7 +403,7 @@ def p6_section(dev, seed: int, steps: int = 5, reps: int = 10) -> dict:
     out = {"workload": f"Burgers SSP-RK3, 64^3x6={K} periodic tets, P=6, 2 slab partitions, "
                        f"{steps} steps per plan", "stage": {}, "derivative": {}}
     ref, hmin = None, None
-    for name, prec in (("f64", 0), ("i8x6", 4), ("bf16x3_lift64", 2 | 16), ("bf16x3", 2)):
+    for name, prec in (("f64", 0), ("i8x6", 4), ("bf16x3_lift64", 2 | 16), ("bf16x3", 2), ("bf16x6", 3)):
         solver = dg.BurgersSolver(dev, st, cfl=0.3, hmin=hmin)
         hmin = solver.hmin
         solver.mesh.set_precision(prec)
