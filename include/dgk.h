@@ -328,10 +328,11 @@ int dgk_cfl_dt(dgk_device dev, double* maxabs_dev, double cfl, double hmin, doub
  * default. Set the variant first. */
 int dgk_burgers_set_tile(dgk_dgmesh m, int elems_per_block, int blocks_per_cu);
 /* Stage kernel: 0 automatic (FP64 MFMA for N <= 4, split for N = 5, 6,
- * the fused BF16x3 stage for N = 5, 6 with D and LIFT in DGK_PREC_BF16X3),
+ * the fused stage for N = 5, 6 with D and LIFT both in DGK_PREC_BF16X3 or
+ * both in DGK_PREC_BF16X6),
  * 1 VALU (one element per thread, FP64 FMA with scalar operator operands),
  * 2 FP64 MFMA, 3 split (volume kernel + surface kernel, see
- * dgk_burgers_set_precision; also for N = 5, 6 in BF16x3). */
+ * dgk_burgers_set_precision; also for N = 5, 6 in BF16x3 and BF16x6). */
 int dgk_burgers_set_variant(dgk_dgmesh m, int variant);
 /* Tile order of the persistent grid: 1 gives each XCD a contiguous range of
  * tiles (blocks renumbered around the round-robin XCD dispatch), 0 (default)
@@ -354,11 +355,11 @@ int dgk_dgmesh_set_tile_order(dgk_dgmesh m, int order);
  * derivative plan, with the linear-part removal of dgk_phys_deriv_affine)
  * into a mesh-owned FP64 buffer, then a surface kernel (LF flux, LIFT on
  * matrix cores, RK update). The fused kernels (N <= 4, FP64) are used
- * otherwise. At N = 5, 6 with D and LIFT both in DGK_PREC_BF16X3 one fused
- * launch computes the stage (w = u^2/2 minus its vertex interpolant once per
- * element column, Dr/Ds/Dt and LIFT products on BF16 matrix cores scaled by
- * the element's metrics in FP64, LF flux and RK update), unless variant 3
- * asks for the split stage. */
+ * otherwise. At N = 5, 6 with D and LIFT both in DGK_PREC_BF16X3 (or both in
+ * DGK_PREC_BF16X6) one fused launch computes the stage (w = u^2/2 minus its
+ * vertex interpolant once per element column, Dr/Ds/Dt and LIFT products on
+ * BF16 matrix cores scaled by the element's metrics in FP64, LF flux and RK
+ * update), unless variant 3 asks for the split stage. */
 enum { DGK_PREC_LIFT_F64 = 16 };
 int dgk_burgers_set_precision(dgk_dgmesh m, int prec);
 
