@@ -100,8 +100,9 @@ struct dgk_dgmesh_s {
   void* fused_frag = nullptr;            // [-Dr | -Ds | -Dt | LIFT] BF16 part fragments
   float* fused_lam = nullptr;            // [6][Np] barycentric weights of the nodes (FP32, remainders)
   int fused_npart = 0;                   // BF16 parts of the fragments built (2: BF16x3, 3: BF16x6)
-  // Split stage's FP64-LIFT surface kernel at P = 5, 6 (surf_hi.hip)
-  void* surf64_frag = nullptr;           // [MT][Nfp][64] FP64 LIFT fragments
+  // Split stage's FP64/FP32-LIFT surface kernel at P = 5, 6 (surf_hi.hip)
+  void* surf_hi_frag = nullptr;          // [MT][Nfp][64] LIFT fragments, FP64 or FP32
+  int surf_hi_l64 = 0;                   // surf_hi_frag holds FP64 values
 };
 
 namespace dgk {
@@ -121,17 +122,17 @@ int split_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, 
 int split_prepare(dgk_dgmesh m);
 void split_release(dgk_dgmesh m);
 
+// Split stage at P = 5, 6 with LIFT in FP64 or FP32: jumps, LIFT and update
+// in surf_hi.hip (surf_hi_surface) in place of split.hip's surface kernel.
+bool surf_hi_applies(dgk_dgmesh m);
+int surf_hi_prepare(dgk_dgmesh m);
+void surf_hi_release(dgk_dgmesh m);
+int surf_hi_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
+                    double a1, const double* dt, unsigned long long* maxabs, int mode);
 // Fused stage for P = 5, 6 with the D and LIFT contractions both in BF16x3
 // or both in BF16x6 (fused_hi.hip): volume term, flux, LIFT and RK update in
 // one launch; used by split_stage in place of its two launches unless
 // variant 3 asks for them.
-// Split stage at P = 5, 6 with LIFT in FP64: jumps, LIFT and update in
-// surf_hi.hip (surf64_surface) in place of split.hip's surface kernel.
-bool surf64_applies(dgk_dgmesh m);
-int surf64_prepare(dgk_dgmesh m);
-void surf64_release(dgk_dgmesh m);
-int surf64_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
-                   double a1, const double* dt, unsigned long long* maxabs, int mode);
 bool fused_applies(dgk_dgmesh m);
 int fused_prepare(dgk_dgmesh m);
 void fused_release(dgk_dgmesh m);

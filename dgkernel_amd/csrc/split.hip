@@ -545,13 +545,13 @@ int dgk::split_prepare(dgk_dgmesh m) {
   const std::vector<unsigned char> frag = lift_fragments(m->LIFT_h, m->N, m->lift_prec);
   rc = dgk_malloc(m->dev, frag.size(), frag.data(), &m->lift_frag);
   if (rc != DGK_OK) return rc;
-  if (dgk::surf64_applies(m)) return dgk::surf64_prepare(m);
+  if (dgk::surf_hi_applies(m)) return dgk::surf_hi_prepare(m);
   return surface(m, nullptr);
 }
 
 void dgk::split_release(dgk_dgmesh m) {
   dgk::fused_release(m);
-  dgk::surf64_release(m);
+  dgk::surf_hi_release(m);
   if (m->plan) dgk_deriv_plan_destroy(m->plan);
   if (m->rhs) dgk_free(m->dev, m->rhs);
   if (m->lift_frag) dgk_free(m->dev, m->lift_frag);
@@ -576,7 +576,7 @@ int dgk::split_stage(dgk_dgmesh m, const double* u, const double* u0, double* ou
 
 int dgk::split_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
                        double a1, const double* dt, unsigned long long* maxabs, int mode) {
-  if (dgk::surf64_applies(m)) return dgk::surf64_surface(m, u, u0, out, a0, a1, dt, maxabs, mode);
+  if (dgk::surf_hi_applies(m)) return dgk::surf_hi_surface(m, u, u0, out, a0, a1, dt, maxabs, mode);
   SurfArgs a;
   a.tiles = m->tiles64;
   a.ntiles = m->ntiles64;

@@ -1,6 +1,6 @@
 // Surface kernel of the split stage at P = 5, 6 with LIFT in FP64 (the FP64,
-// I8X6 and LIFT-in-FP64 settings): the Lax-Friedrichs jumps, LIFT on FP64
-// matrix cores and the RK update,
+// I8X6 and LIFT-in-FP64 settings) or FP32 (the FP32 setting): the
+// Lax-Friedrichs jumps, LIFT on FP64 or FP32 matrix cores and the RK update,
 //   out = a0 u0 + a1 (u + dt (rhs + LIFT g)),   (mode 0)
 //   out = rhs + LIFT g                         (mode 1),
 // rhs the volume term the derivative plan wrote (FP64, or FP32 for the plans
@@ -8,12 +8,12 @@
 //   g = (u- - u+) (A (u- + u+) - B max(|u-|, |u+|)),
 // A = Fscale n.(1,1,1)/4, B = Fscale |n.(1,1,1)|/2.
 //
-// Block: MT matrix waves (one 16-row tile of LIFT each, its FP64 fragments in
+// Block: MT matrix waves (one 16-row tile of LIFT each, its fragments in
 // registers) and 4 memory waves (4 elements of every 16-element tile each),
 // one tile per phase between block barriers; in phase n
 //   matrix waves: tile n's rhs and u0 rows loaded during the product
-//                 G^T LIFT^T (v_mfma_f64_16x16x4_f64, G the tile's jumps from
-//                 LDS, transposed so that 16 lanes hold 16 consecutive rows
+//                 G^T LIFT^T (v_mfma_f64_16x16x4_f64 or v_mfma_f32_16x16x4_f32,
+//                 G the tile's jumps from LDS, transposed so that 16 lanes hold 16 consecutive rows
 //                 of an element) -> RK update, 16 consecutive rows per store;
 //   memory waves: the face records of tile n+3 (loaded in phase n-1) into
 //                 LDS; the loads of tile n+4's records and of tile n+2 (u and
@@ -23,7 +23,9 @@
 // The fused BF16 stage (fused_hi.hip) walks its tiles the same way; here the
 // memory waves have no volume columns to form, and u is loaded once by this
 // kernel (own traces and update) where the one-wave-per-tile surface kernel
-// of split.hip reads it twice.
+// of split.hip reads it twice. With FP32 LIFT the jumps are formed in FP64 and
+// rounded to FP32 for the product, whose FP32 sums are added to the volume
+// term in FP64 (the rounding of the split.hip kernel's FP32 form).
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -34,10 +36,11 @@
 namespace {
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 template <int N>
-struct SCfg64 {
+struct SCfgHi {
   static constexpr int NP = dgk::Tet<N>::Np, NFP = dgk::Tet<N>::Nfp;
   static constexpr int MT = (NP + 15) / 16;  // row tiles of LIFT = matrix waves
   static constexpr int KF = NFP;             // k = 4 steps over the 4 Nfp columns
@@ -45,7 +48,7 @@ struct SCfg64 {
   static constexpr int EPW = 16 / MW;        // elements per memory wave
   static constexpr int NT = 64 * (MT + MW);
   static constexpr int NPU = NP | 1;         // LDS row stride of u, doubles
-  static constexpr int GRS = 4 * NFP + 1;    // LDS row stride of the jumps (+ a dump column)
+  static constexpr int GRS = 4 * NFP + 1;    // LDS row stride of the jumps (+ a dump column), odd
   static constexpr int UPL = (EPW * NP + 63) / 64;        // u values per lane
   static constexpr int GPL = (EPW * 4 * NFP + 63) / 64;  // trace pairs per lane
 };
@@ -53,14 +56,14 @@ struct SCfg64 {
 // Face record of (face f, element j) of a memory wave's elements: value
 // offset of the neighbour's node 0, row of the orientation table, flux
 // factors.
-struct FRec64 {
+struct FRecHi {
   int32_t nb, fnrow;
   double fa, fb;
 };
 
-struct Surf64Args {
+struct SurfHiArgs {
   int ntiles;
-  const double* frag;         // [MT][KF][64] LIFT fragments
+  const void* frag;           // [MT][KF][64] LIFT fragments, FP64 or FP32 (LT)
   const uint8_t* face_nodes;  // [4][24][Nfp]
   const double* bgeo;         // [19][K]; rows 0..2 the folded metrics
   int64_t K;
@@ -78,22 +81,25 @@ struct Surf64Args {
 
 // (The tile list is a separate const __restrict__ argument: the compiler may
 // then read its wave-uniform entries through the scalar cache.)
-template <int N, bool R32>
-__global__ void __launch_bounds__(SCfg64<N>::NT)
-    surf64_kernel(Surf64Args a, const dgk::Tile* __restrict__ tiles) {
-  using C = SCfg64<N>;
+// LT: type of LIFT and of the jumps in the product (double or float).
+template <int N, bool R32, typename LT>
+__global__ void __launch_bounds__(SCfgHi<N>::NT)
+    surf_hi_kernel(SurfHiArgs a, const dgk::Tile* __restrict__ tiles) {
+  constexpr bool L64 = std::is_same<LT, double>::value;
+  using AccT = typename std::conditional<L64, doublex4, floatx4>::type;
+  using C = SCfgHi<N>;
   constexpr int NP = C::NP, NFP = C::NFP, MT = C::MT, KF = C::KF, MW = C::MW, EPW = C::EPW;
   constexpr int NPU = C::NPU, GRS = C::GRS, UPL = C::UPL, GPL = C::GPL;
   constexpr dgk::FmaskTable<N> FM{};
   // jumps of two tiles, [element][face f, point qq] (+ a dump column); u of
   // two tiles (+ a dump slot) and their tile records, for the update
-  __shared__ double sG[2][16 * GRS];
+  __shared__ LT sG[2][16 * GRS];
   __shared__ double sU[2][16 * NPU + 1];
   __shared__ int64_t sTv[2];
   __shared__ int sTc[2];
   // per memory wave: face records of its elements for four tiles (slot =
   // tile % 4)
-  __shared__ FRec64 sRec[MW][4][4 * EPW];
+  __shared__ FRecHi sRec[MW][4][4 * EPW];
   __shared__ uint8_t sFN[4 * 24 * NFP];
   __shared__ uint8_t sFM[4 * NFP];
   for (int i = threadIdx.x; i < 4 * 24 * NFP; i += C::NT) sFN[i] = a.face_nodes[i];
@@ -137,9 +143,9 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
   if (w < MT) {
     // ---------------- matrix wave: LIFT rows 16 w .. 16 w + 15 ----------------
     // fragment s: LIFT[16 w + l % 16][4 s + l / 16] in lane l
-    double lf[KF];
+    LT lf[KF];
 #pragma unroll
-    for (int s = 0; s < KF; ++s) lf[s] = a.frag[((size_t)w * KF + s) * 64 + lane];
+    for (int s = 0; s < KF; ++s) lf[s] = ((const LT*)a.frag)[((size_t)w * KF + s) * 64 + lane];
     const double dt = a.mode == 0 ? *a.dt : 1.0;
     const double a0 = a.mode == 0 ? a.a0 : 0.0, a1 = a.mode == 0 ? a.a1 : 1.0;
     // The fragments and dt are complete before the loop: without this wait
@@ -155,9 +161,12 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
       const int count = __builtin_amdgcn_readfirstlane(sTc[bu]);
       const int row = 16 * w + c;
       const int rr = row < NP ? row : NP - 1;
-      // rhs and u0 of the rows this lane updates (element g + 4 r, row
-      // 16 w + c), loaded during the product through buffer resources sized
-      // to the tile (loads past it read zero)
+      // The product's register r of lane l holds row 16 w + l % 16 of element
+      // g + 4 r (FP64: result rows interleave as l / 16 + 4 r) or 4 g + r
+      // (FP32).
+      auto elem = [&](int r) { return L64 ? g + 4 * r : 4 * g + r; };
+      // rhs and u0 of the rows this lane updates, loaded during the product
+      // through buffer resources sized to the tile (loads past it read zero)
       const int nbytes = count * NP * 8;
       const __amdgpu_buffer_rsrc_t rsR = __builtin_amdgcn_make_buffer_rsrc(
           (void*)((const char*)a.rhs + vbase * (R32 ? 4 : 8)), 0, R32 ? nbytes / 2 : nbytes, 0x00020000);
@@ -167,21 +176,25 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
       double rv[4], u0v[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int off = (g + 4 * r) * NP + rr;
+        const int off = elem(r) * NP + rr;
         if constexpr (R32)
           rv[r] = (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsR, off * 4, 0, 0));
         else
           rv[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsR, off * 8, 0, 0));
         u0v[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs0, off * 8, 0, 2));
       }
-      // G^T LIFT^T: the FP64 product's register r of lane l holds result row
-      // l / 16 + 4 r (rows interleaved, unlike the FP32 form), i.e. row
-      // 16 w + l % 16 of element l / 16 + 4 r
-      doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+      // G^T LIFT^T
+      AccT acc = {};
       __builtin_amdgcn_s_setprio(1);
+      if constexpr (L64) {
 #pragma unroll
-      for (int s = 0; s < KF; ++s)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sG[bu][c * GRS + 4 * s + g], lf[s], acc, 0, 0, 0);
+        for (int s = 0; s < KF; ++s)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sG[bu][c * GRS + 4 * s + g], lf[s], acc, 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int s = 0; s < KF; ++s)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sG[bu][c * GRS + 4 * s + g], lf[s], acc, 0, 0, 0);
+      }
       __builtin_amdgcn_s_setprio(0);
       // Stores through a buffer resource sized to the tile: the ones past it
       // (elements >= count, rows >= Np) are dropped, so the partial tile takes
@@ -190,9 +203,9 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
           __builtin_amdgcn_make_buffer_rsrc((void*)(a.out + vbase), 0, nbytes, 0x00020000);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int e = g + 4 * r;
+        const int e = elem(r);
         const double uv = a.mode == 0 ? sU[bu][e * NPU + rr] : 0.0;
-        const double o = fma(a0, u0v[r], a1 * fma(dt, rv[r] + acc[r], uv));
+        const double o = fma(a0, u0v[r], a1 * fma(dt, rv[r] + (double)acc[r], uv));
         const bool ok = row < NP && e < count;
         lmax = fmax(lmax, ok ? fabs(o) : 0.0);
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, o), rsO,
@@ -236,7 +249,7 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
     if (n >= nb) return;
     const int ln = lane_now(), rf = ln / EPW;
     if (ln < 4 * EPW) {
-      FRec64 r;
+      FRecHi r;
       r.nb = rnb;
       r.fnrow = (rf * 24 + ((rcode >> (5 * rf)) & 31)) * NFP;
       // Fscale n.(1,1,1) of face rf: -gc, -gb, ga + gb + gc, -ga (the face
@@ -278,7 +291,7 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
     constexpr int slot = decltype(SL)::value;
     const int nn = n < nb ? n : nb - 1;
     const dgk::Tile tl = tile_of(nn);
-    const FRec64* R = sRec[q][nn & 3];
+    const FRecHi* R = sRec[q][nn & 3];
     const int lane = lane_now();
 #pragma unroll
     for (int k = 0; k < UPL; ++k) {
@@ -291,7 +304,7 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
     for (int k = 0; k < GPL; ++k) {
       const int p = lane + 64 * k;
       const int pp = p < EPW * 4 * NFP ? p : EPW * 4 * NFP - 1;
-      const FRec64& r = R[pp / NFP];
+      const FRecHi& r = R[pp / NFP];
       P[slot][k] = a.u[r.nb + sFN[r.fnrow + pp % NFP]];
     }
     TV[slot] = tl.vbase;
@@ -302,7 +315,7 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
     constexpr int slot = decltype(SL)::value;
     const int lane = lane_now();
     const int bu = n & 1;
-    const FRec64* R = sRec[q][n & 3];
+    const FRecHi* R = sRec[q][n & 3];
     double* su = sU[bu];
 #pragma unroll
     for (int k = 0; k < UPL; ++k) {
@@ -315,14 +328,14 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
       sTc[bu] = TC[slot];
     }
     wsync();  // the wave's u of the tile is in LDS
-    double* G = sG[bu];
+    LT* G = sG[bu];
 #pragma unroll
     for (int k = 0; k < GPL; ++k) {
       const uint32_t fk = fw[k];
       const int col = (fk >> 12) & 511, j = fk >> 26;
-      const FRec64& r = R[(fk >> 21) & 31];
+      const FRecHi& r = R[(fk >> 21) & 31];
       const double m = su[fk & 4095], pv = P[slot][k];
-      G[(e0 + j) * GRS + col] = (m - pv) * fma(r.fa, m + pv, -r.fb * fmax(fabs(m), fabs(pv)));
+      G[(e0 + j) * GRS + col] = (LT)((m - pv) * fma(r.fa, m + pv, -r.fb * fmax(fabs(m), fabs(pv))));
     }
   };
 
@@ -360,12 +373,12 @@ __global__ void __launch_bounds__(SCfg64<N>::NT)
   }
 }
 
-template <int N, bool R32>
-int surf64_launch(dgk_dgmesh m, const Surf64Args& a) {
+template <int N, bool R32, typename LT>
+int surf_hi_launch(dgk_dgmesh m, const SurfHiArgs& a) {
   if (a.ntiles == 0) return DGK_OK;
   int grid = m->dev->cu_count;
   if (grid > a.ntiles) grid = a.ntiles;
-  hipLaunchKernelGGL((surf64_kernel<N, R32>), dim3(grid), dim3(SCfg64<N>::NT), 0, m->dev->stream, a,
+  hipLaunchKernelGGL((surf_hi_kernel<N, R32, LT>), dim3(grid), dim3(SCfgHi<N>::NT), 0, m->dev->stream, a,
                      m->tiles64);
   DGK_LAUNCH_CHECK();
   return DGK_OK;
@@ -373,15 +386,17 @@ int surf64_launch(dgk_dgmesh m, const Surf64Args& a) {
 
 }  // namespace
 
-bool dgk::surf64_applies(dgk_dgmesh m) {
-  return (m->N == 5 || m->N == 6) && m->lift_prec == DGK_PREC_F64;
+bool dgk::surf_hi_applies(dgk_dgmesh m) {
+  return (m->N == 5 || m->N == 6) && (m->lift_prec == DGK_PREC_F64 || m->lift_prec == DGK_PREC_F32);
 }
 
-int dgk::surf64_prepare(dgk_dgmesh m) {
-  if (m->surf64_frag) return DGK_OK;
+int dgk::surf_hi_prepare(dgk_dgmesh m) {
+  if (m->surf_hi_frag) return DGK_OK;
   const int N = m->N, NP = dgk::np_of(N), NFP = dgk::nfp_of(N), NF = 4 * NFP;
   const int MT = (NP + 15) / 16, KF = NFP;
+  const bool l64 = m->lift_prec == DGK_PREC_F64;
   // fragment s of row tile t: lane l holds LIFT[16 t + l % 16][4 s + l / 16]
+  // (rounded to FP32 for the FP32 form)
   std::vector<double> frag((size_t)MT * KF * 64, 0.0);
   for (int t = 0; t < MT; ++t)
     for (int s = 0; s < KF; ++s)
@@ -389,21 +404,26 @@ int dgk::surf64_prepare(dgk_dgmesh m) {
         const int row = 16 * t + (l & 15), k = 4 * s + (l >> 4);
         if (row < NP && k < NF) frag[((size_t)t * KF + s) * 64 + l] = m->LIFT_h[(size_t)row * NF + k];
       }
-  return dgk_malloc(m->dev, frag.size() * 8, frag.data(), &m->surf64_frag);
+  m->surf_hi_l64 = l64 ? 1 : 0;
+  if (l64) return dgk_malloc(m->dev, frag.size() * 8, frag.data(), &m->surf_hi_frag);
+  const std::vector<float> f32(frag.begin(), frag.end());
+  return dgk_malloc(m->dev, f32.size() * 4, f32.data(), &m->surf_hi_frag);
 }
 
-void dgk::surf64_release(dgk_dgmesh m) {
-  if (m->surf64_frag) dgk_free(m->dev, m->surf64_frag);
-  m->surf64_frag = nullptr;
+void dgk::surf_hi_release(dgk_dgmesh m) {
+  if (m->surf_hi_frag) dgk_free(m->dev, m->surf_hi_frag);
+  m->surf_hi_frag = nullptr;
 }
 
-int dgk::surf64_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
-                        double a1, const double* dt, unsigned long long* maxabs, int mode) {
-  int rc = surf64_prepare(m);
+int dgk::surf_hi_surface(dgk_dgmesh m, const double* u, const double* u0, double* out, double a0,
+                         double a1, const double* dt, unsigned long long* maxabs, int mode) {
+  int rc = surf_hi_prepare(m);
   if (rc != DGK_OK) return rc;
-  Surf64Args a;
+  const bool l64 = m->lift_prec == DGK_PREC_F64;
+  DGK_REQUIRE(m->surf_hi_l64 == (l64 ? 1 : 0), "split surface: LIFT fragments prepared for another precision");
+  SurfHiArgs a;
   a.ntiles = m->ntiles64;
-  a.frag = (const double*)m->surf64_frag;
+  a.frag = m->surf_hi_frag;
   a.face_nodes = m->face_nodes;
   a.bgeo = m->bgeo;
   a.K = m->K;
@@ -420,8 +440,12 @@ int dgk::surf64_surface(dgk_dgmesh m, const double* u, const double* u0, double*
   a.mode = mode;
   const bool r32 = m->rhs32 != 0;
   switch (m->N) {
-    case 5: return r32 ? surf64_launch<5, true>(m, a) : surf64_launch<5, false>(m, a);
-    case 6: return r32 ? surf64_launch<6, true>(m, a) : surf64_launch<6, false>(m, a);
+    case 5:
+      if (l64) return r32 ? surf_hi_launch<5, true, double>(m, a) : surf_hi_launch<5, false, double>(m, a);
+      return r32 ? surf_hi_launch<5, true, float>(m, a) : surf_hi_launch<5, false, float>(m, a);
+    case 6:
+      if (l64) return r32 ? surf_hi_launch<6, true, double>(m, a) : surf_hi_launch<6, false, double>(m, a);
+      return r32 ? surf_hi_launch<6, true, float>(m, a) : surf_hi_launch<6, false, float>(m, a);
   }
   dgk::set_error("split surface: N=%d outside 5..6", m->N);
   return DGK_ERR_UNSUPPORTED;
