@@ -35,6 +35,7 @@
 // 32-wide k step).
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "dgmesh_internal.h"
@@ -830,6 +831,338 @@ __global__ void __launch_bounds__(64 * WPB, 2) av_surf_kernel(AvArgs a, StageAv 
   }
 }
 
+// ---------------------------------------------------------------------------
+// av_surf_hi_kernel: the surface half of the viscous stage on the block
+// layout of the split stage's P = 5, 6 surface kernel (surf_hi.hip): MT
+// matrix waves (one 16-row tile of LIFT each, FP64 fragments in registers)
+// and 4 memory waves (4 elements of every 16-element tile each), one tile per
+// phase between block barriers; in phase n
+//   matrix waves: tile n's vol and u0 rows loaded during the product
+//                 G^T LIFT^T (v_mfma_f64_16x16x4_f64, result rows l/16 + 4 r),
+//                 then out = a0 u0 + a1 (u + dt (vol + LIFT g)) with 16
+//                 consecutive rows per store;
+//   memory waves: the face records of tile n+3 into LDS, the loads of tile
+//                 n+4's records and of tile n+2 (u, the neighbour traces of
+//                 u, the own and neighbour traces of sig, the neighbour eps,
+//                 gathered through the records stored a phase before), and
+//                 tile n+1's u into LDS and its jumps
+//                   -Fscale ((sig- + sig+)/2 + tau (u- - u+))  [+ LF jump (BURG)]
+//                 (tau = tau0 max(eps-, eps+); a boundary face, its own
+//                 neighbour, has sig+ = -sig- and u+ = u-) into G.
+// MODE 0: a0 u0 + ...; 1: a0 = 0; 2: out = vol + LIFT g.
+// ---------------------------------------------------------------------------
+template <int N>
+struct AvHiCfg {
+  static constexpr int NP = Tet<N>::Np, NFP = Tet<N>::Nfp, NF = 4 * NFP;
+  static constexpr int MT = (NP + 15) / 16;  // row tiles of LIFT = matrix waves
+  static constexpr int KF = NFP;             // k steps of 4 over the 4 Nfp columns
+  static constexpr int MW = 4;               // memory waves
+  static constexpr int EPW = 16 / MW;        // elements per memory wave
+  static constexpr int NT = 64 * (MT + MW);
+  static constexpr int NPU = NP | 1;         // LDS row stride of u
+  static constexpr int GRS = NF + 1;         // LDS row stride of the jumps (+ a dump column), odd
+  static constexpr int UPL = (EPW * NP + 63) / 64;   // u values per lane
+  static constexpr int GPL = (EPW * NF + 63) / 64;   // trace pairs per lane
+};
+
+// Face record of (face f, element j) of a memory wave's elements: value
+// offset of the neighbour's node 0, row of the orientation table, rows of
+// the own and neighbour sig traces ([K][4] face rows of Nfp values), the
+// neighbour element, boundary flag, own eps, Fscale, LF factors.
+struct AvHiRec {
+  int32_t nb, fnrow, smrow, sprow, ne, bnd;
+  double eo, fs, fa, fb;
+};
+
+template <int N, int MODE, bool BURG>
+__global__ void __launch_bounds__(AvHiCfg<N>::NT)
+    av_surf_hi_kernel(AvArgs a, StageAv o, const Tile* __restrict__ tiles) {
+  using C = AvHiCfg<N>;
+  constexpr int NP = C::NP, NFP = C::NFP, NF = C::NF, MT = C::MT, KF = C::KF, MW = C::MW;
+  constexpr int EPW = C::EPW, NPU = C::NPU, GRS = C::GRS, UPL = C::UPL, GPL = C::GPL;
+  constexpr FmaskTable<N> FM{};
+  static_assert(16 * NPU < 4096 && GRS < 512, "packed pair words");
+  __shared__ double sG[2][16 * GRS];
+  __shared__ double sU[2][16 * NPU + 1];
+  __shared__ int64_t sTv[2];
+  __shared__ int sTc[2];
+  __shared__ AvHiRec sRec[MW][4][4 * EPW];
+  __shared__ uint8_t sFN[4 * 24 * NFP];
+  __shared__ uint8_t sFP[4 * 24 * NFP];  // neighbour's face point of each (face, code, point)
+  __shared__ uint8_t sFM[NF];
+  for (int i = threadIdx.x; i < NF; i += C::NT) sFM[i] = (uint8_t)FM.v[i / NFP][i % NFP];
+  for (int i = threadIdx.x; i < 4 * 24 * NFP; i += C::NT) {
+    const int nn = a.face_nodes[i];
+    sFN[i] = (uint8_t)nn;
+    const int f2 = ((i / NFP) % 24) / 6;
+    int fp2 = 0;
+    for (int qq = 0; qq < NFP; ++qq)
+      if (FM.v[f2][qq] == nn) fp2 = qq;
+    sFP[i] = (uint8_t)fp2;
+  }
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15, g = lane >> 4;
+  const int64_t K = a.K;
+  // Tiles of this block: with a grid that is a multiple of 8 (block b on XCD
+  // b % 8), each XCD walks one contiguous eighth of the tile list.
+  int t0 = blockIdx.x, t1 = a.ntiles, ts = gridDim.x;
+  if (gridDim.x % 8 == 0) {
+    const int x = blockIdx.x % 8;
+    t0 = (int)((int64_t)a.ntiles * x / 8) + blockIdx.x / 8;
+    t1 = (int)((int64_t)a.ntiles * (x + 1) / 8);
+    ts = gridDim.x / 8;
+  }
+  const int nb = t0 < t1 ? (t1 - t0 + ts - 1) / ts : 0;
+  auto tile_of = [&](int n) -> Tile { return tile_uniform(tiles[t0 + n * ts]); };
+  // Lane-derived values are rederived in each step (an empty asm hides the
+  // lane index) rather than held across the tile loop.
+  auto lane_now = [&]() {
+    int l = lane;
+    asm volatile("" : "+v"(l));
+    return l;
+  };
+  __syncthreads();
+
+  if (w < MT) {
+    // ---------------- matrix wave: LIFT rows 16 w .. 16 w + 15 ----------------
+    // fragment s: LIFT[16 w + l % 16][4 s + l / 16] in lane l
+    double lf[KF];
+#pragma unroll
+    for (int s = 0; s < KF; ++s) {
+      const int row = 16 * w + c;
+      lf[s] = row < NP ? a.LIFT[row * NF + 4 * s + g] : 0.0;
+    }
+    const double dt = MODE == 2 ? 1.0 : *o.dt;
+    const double a0 = MODE == 0 ? o.a0 : 0.0, a1 = MODE == 2 ? 1.0 : o.a1;
+    // The fragments and dt are complete before the loop: without this wait
+    // the compiler counts them as pending at the loop head.
+    __builtin_amdgcn_s_waitcnt(0);
+    double lmax = 0.0;
+    __syncthreads();  // phase -1: the memory waves prepare tile 0
+    for (int n = 0; n < nb; ++n) {
+      const int bu = n & 1;
+      const int64_t tv = sTv[bu];
+      const int64_t vbase = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tv >> 32)) << 32) |
+                                      (uint32_t)__builtin_amdgcn_readfirstlane((int)tv));
+      const int count = __builtin_amdgcn_readfirstlane(sTc[bu]);
+      const int row = 16 * w + c;
+      const int rr = row < NP ? row : NP - 1;
+      // vol and u0 of the rows this lane updates (element g + 4 r, row
+      // 16 w + c), loaded during the product through buffer resources sized
+      // to the tile (loads past it read zero)
+      const int nbytes = count * NP * 8;
+      const __amdgpu_buffer_rsrc_t rsV =
+          __builtin_amdgcn_make_buffer_rsrc((void*)(a.vol + vbase), 0, nbytes, 0x00020000);
+      const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(MODE == 0 ? o.u0 + vbase : a.vol), 0, MODE == 0 ? nbytes : 0, 0x00020000);
+      double rv[4], u0v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int off = ((g + 4 * r) * NP + rr) * 8;
+        rv[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsV, off, 0, 0));
+        u0v[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs0, off, 0, 2));
+      }
+      // G^T LIFT^T: register r of lane l holds row 16 w + l % 16 of element
+      // l / 16 + 4 r
+      double4v acc = (double4v){0.0, 0.0, 0.0, 0.0};
+      __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+      for (int s = 0; s < KF; ++s)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sG[bu][c * GRS + 4 * s + g], lf[s], acc, 0, 0, 0);
+      __builtin_amdgcn_s_setprio(0);
+      // Stores through a buffer resource sized to the tile: the ones past it
+      // (elements >= count, rows >= Np) are dropped.
+      const __amdgpu_buffer_rsrc_t rsO =
+          __builtin_amdgcn_make_buffer_rsrc((void*)(o.out + vbase), 0, nbytes, 0x00020000);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int e = g + 4 * r;
+        const double uv = MODE == 2 ? 0.0 : sU[bu][e * NPU + rr];
+        const double v = fma(a0, u0v[r], a1 * fma(dt, rv[r] + acc[r], uv));
+        const bool ok = row < NP && e < count;
+        lmax = fmax(lmax, ok ? fabs(v) : 0.0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rsO,
+                                              row < NP ? (e * NP + row) * 8 : 0x40000000, 0, 0);
+      }
+      __syncthreads();  // end of phase n
+    }
+    if (MODE != 2 && o.maxabs) {
+      lmax = wmax(lmax);
+      if (lane == 0 && lmax > 0.0)
+        atomicMax((unsigned long long*)o.maxabs, (unsigned long long)__double_as_longlong(lmax));
+    }
+    return;
+  }
+
+  // ---------------- memory wave q: elements EPW q .. EPW q + EPW - 1 ----------------
+  const int q = w - MT;
+  const int e0 = EPW * q;
+  // Face records of this wave's elements for tile n go to slot n % 4 of
+  // sRec[q], written three phases ahead from loads issued a phase before:
+  // lane s < 4 EPW takes face s / EPW of element e0 + s % EPW (clamped to the
+  // tile). Every lane loads (lane l the record of lane l % (4 EPW)), and
+  // tiles past the block's last read its last, so that every phase issues
+  // the same loads.
+  static_assert((4 * EPW & (4 * EPW - 1)) == 0, "records per wave: a power of two");
+  int32_t rnb = 0, rne = 0, rcode = 0, rge = 0;
+  double reo = 0.0, rfs = 0.0, rga = 0.0, rgb = 0.0, rgc = 0.0;
+  auto rec_load = [&](int n) {
+    const int ln = lane_now() & (4 * EPW - 1), rf = ln / EPW, rj = ln % EPW;
+    const Tile tl = tile_of(n < nb ? n : nb - 1);
+    const int e = e0 + rj < tl.count ? e0 + rj : tl.count - 1;
+    const int64_t ge = tl.gbase + e;
+    rge = (int32_t)ge;
+    rnb = a.nbr[(int64_t)rf * K + ge];
+    rne = a.nbe[(int64_t)rf * K + ge];
+    rcode = a.codes[ge];
+    reo = a.eps_in[ge];
+    rfs = a.geo[(13 + 4 * rf) * K + ge];
+    if (BURG) {
+      rga = o.bgeo[ge];
+      rgb = o.bgeo[K + ge];
+      rgc = o.bgeo[2 * K + ge];
+    }
+  };
+  auto rec_store = [&](int n) {
+    if (n >= nb) return;
+    const int ln = lane_now(), rf = ln / EPW;
+    if (ln < 4 * EPW) {
+      const int code = (rcode >> (5 * rf)) & 31, f2 = code / 6;
+      AvHiRec r;
+      r.nb = rnb;
+      r.fnrow = (rf * 24 + code) * NFP;
+      r.smrow = rge * 4 + rf;
+      r.sprow = rne * 4 + f2;
+      r.ne = rne;
+      r.bnd = rne == rge && f2 == rf;
+      r.eo = reo;
+      r.fs = rfs;
+      // Fscale n.(1,1,1) of face rf: -gc, -gb, ga + gb + gc, -ga, as
+      // coefficients in {-1, 0, 1} of (ga, gb, gc) rather than a branch on
+      // the lane's face
+      const double c0 = (double)(rf == 2) - (double)(rf == 3);
+      const double c1 = (double)(rf == 2) - (double)(rf == 1);
+      const double c2 = (double)(rf == 2) - (double)(rf == 0);
+      const double sn = BURG ? fma(c0, rga, fma(c1, rgb, c2 * rgc)) : 0.0;
+      r.fa = 0.25 * sn;
+      r.fb = 0.5 * fabs(sn);
+      sRec[q][n & 3][ln] = r;
+    }
+  };
+  // Per lane and trace pair k (pair p = lane + 64 k: slot p / NFP = f EPW + j,
+  // point qq = p % NFP), fixed for the launch: index of the own trace in a
+  // tile's u (12 bits), column of the jump in G (9 bits; the dump column past
+  // the pairs), record slot (5 bits), element (2 bits), point (5 bits).
+  uint32_t fw[GPL];
+#pragma unroll
+  for (int k = 0; k < GPL; ++k) {
+    const int p = lane + 64 * k;
+    const bool ok = EPW * NF % 64 == 0 || p < EPW * NF;
+    const int pp = ok ? p : EPW * NF - 1;
+    const int sl = pp / NFP, qq = pp % NFP, f = sl / EPW, j = sl % EPW;
+    const int iu = (e0 + j) * NPU + sFM[f * NFP + qq];
+    const int col = ok ? f * NFP + qq : NF;
+    fw[k] = (uint32_t)iu | ((uint32_t)col << 12) | ((uint32_t)sl << 21) | ((uint32_t)j << 26);
+  }
+  // u values (pair p = lane + 64 k: element e0 + p / NP, node p % NP) and,
+  // per trace pair, u+, sig-, sig+ and eps+, by slot (compile-time
+  // constants, so the arrays stay in registers)
+  double U[2][UPL], P[2][GPL], SM[2][GPL], SP[2][GPL], EP[2][GPL];
+  int64_t TV[2] = {0, 0};
+  int TC[2] = {1, 1};
+  auto issue = [&](int n, auto SL) {
+    constexpr int slot = decltype(SL)::value;
+    const int nn = n < nb ? n : nb - 1;
+    const Tile tl = tile_of(nn);
+    const AvHiRec* R = sRec[q][nn & 3];
+    const int lane = lane_now();
+#pragma unroll
+    for (int k = 0; k < UPL; ++k) {
+      const int p = lane + 64 * k;
+      const int e = e0 + p / NP;
+      const int64_t i = (int64_t)(e < tl.count ? e : tl.count - 1) * NP + p % NP;
+      U[slot][k] = a.u[tl.vbase + i];
+    }
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) {
+      const int p = lane + 64 * k;
+      const int pp = p < EPW * NF ? p : EPW * NF - 1;
+      const int qq = pp % NFP;
+      const AvHiRec& r = R[pp / NFP];
+      P[slot][k] = a.u[r.nb + sFN[r.fnrow + qq]];
+      SM[slot][k] = a.sig[(int64_t)r.smrow * NFP + qq];
+      SP[slot][k] = a.sig[(int64_t)r.sprow * NFP + sFP[r.fnrow + qq]];
+      EP[slot][k] = a.eps_in[r.ne];
+    }
+    TV[slot] = tl.vbase;
+    TC[slot] = tl.count;
+  };
+  const double tau0 = a.tau0;
+  auto prepare = [&](int n, auto SL) {
+    constexpr int slot = decltype(SL)::value;
+    const int lane = lane_now();
+    const int bu = n & 1;
+    const AvHiRec* R = sRec[q][n & 3];
+    double* su = sU[bu];
+#pragma unroll
+    for (int k = 0; k < UPL; ++k) {
+      const int p = lane + 64 * k;
+      const int o2 = p < EPW * NP ? (e0 + p / NP) * NPU + p % NP : 16 * NPU;
+      su[o2] = U[slot][k];
+    }
+    if (q == 0 && lane == 0) {
+      sTv[bu] = TV[slot];
+      sTc[bu] = TC[slot];
+    }
+    wave_fence();  // the wave's u of the tile is in LDS
+    double* G = sG[bu];
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) {
+      const uint32_t fk = fw[k];
+      const int col = (fk >> 12) & 511, j = fk >> 26;
+      const AvHiRec& r = R[(fk >> 21) & 31];
+      const double m = su[fk & 4095], pv = P[slot][k];
+      const double sm = SM[slot][k], sn = r.bnd ? -sm : SP[slot][k];
+      const double tau = tau0 * fmax(r.eo, EP[slot][k]);
+      double x = -r.fs * (0.5 * (sm + sn) + tau * (m - pv));
+      if (BURG) x = fma(m - pv, fma(r.fa, m + pv, -r.fb * fmax(fabs(m), fabs(pv))), x);
+      G[(e0 + j) * GRS + col] = x;
+    }
+  };
+  // Prologue (phase -1 of the matrix waves): records of tiles 0..2 stored and
+  // tile 3's loaded, loads of tiles 0 and 1, tile 0 prepared.
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  if (nb > 0) {
+#pragma unroll 1
+    for (int n = 0; n < 3; ++n) {
+      rec_load(n);
+      rec_store(n);
+    }
+    rec_load(3);
+    wave_fence();
+    issue(0, S0{});
+    issue(1, S1{});
+    prepare(0, S0{});
+  }
+  __syncthreads();  // end of phase -1
+  // Phase n: the records of tile n+3 stored, the loads of tile n+4's records
+  // and of tile n+2 into slot n & 1, then tile n+1 prepared from slot
+  // (n+1) & 1, loaded a phase ago.
+  auto phase = [&](int n, auto A1, auto A2) {
+    rec_store(n + 3);
+    rec_load(n + 4);
+    wave_fence();
+    issue(n + 2, A2);
+    if (n + 1 < nb) prepare(n + 1, A1);
+    __syncthreads();  // end of phase n
+  };
+  for (int n = 0; n < nb; n += 2) {
+    phase(n, S1{}, S0{});
+    if (n + 1 < nb) phase(n + 1, S0{}, S1{});
+  }
+}
+
 // dt = min(cfl hmin / max|u|, cfl_v hmin^2 / max eps), clipped to t_end;
 // scal = [max|u|, dt, t, max eps]; max|u| and max eps are cleared for the
 // next step's reductions.
@@ -1521,14 +1854,41 @@ int av_surf_launch(dgk_dgmesh m, const AvArgs& r, const StageAv& o) {
   return DGK_OK;
 }
 
+// The surface half on the memory-wave / matrix-wave block layout: one block
+// of MT + 4 waves per resident slot, grid a multiple of 8 while the tiles
+// allow.
+template <int N, int MODE, bool BURG>
+int av_surf_hi_launch(dgk_dgmesh m, const AvArgs& r, const StageAv& o) {
+  static const int per_cu = [] {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, av_surf_hi_kernel<N, MODE, BURG>,
+                                                     AvHiCfg<N>::NT, 0) != hipSuccess)
+      v = 1;
+    return v < 1 ? 1 : v;
+  }();
+  const int grid = std::max(1, std::min(r.ntiles, m->dev->cu_count * per_cu));
+  hipLaunchKernelGGL((av_surf_hi_kernel<N, MODE, BURG>), dim3(grid), dim3(AvHiCfg<N>::NT), 0,
+                     m->dev->stream, r, o, r.tiles);
+  DGK_LAUNCH_CHECK();
+  return DGK_OK;
+}
+
+// Surface kernel of the viscous stage: the memory-wave form at P = 4 (the
+// one-wave-per-tile form below it).
+template <int N, int MODE, bool BURG>
+int av_surf_any(dgk_dgmesh m, const AvArgs& r, const StageAv& o) {
+  if constexpr (N == 4) return av_surf_hi_launch<N, MODE, BURG>(m, r, o);
+  return av_surf_launch<N, MODE, BURG>(m, r, o);
+}
+
 template <int N>
 int av_run_n(dgk_dgmesh m, const AvArgs& r, const StageAv& o, int mode, bool burg) {
   int rc = burg ? av_vol_launch<N, true>(m, r) : av_vol_launch<N, false>(m, r);
   if (rc != DGK_OK) return rc;
   if (mode == 2)
-    return burg ? av_surf_launch<N, 2, true>(m, r, o) : av_surf_launch<N, 2, false>(m, r, o);
-  if (mode == 0) return av_surf_launch<N, 0, true>(m, r, o);
-  return av_surf_launch<N, 1, true>(m, r, o);
+    return burg ? av_surf_any<N, 2, true>(m, r, o) : av_surf_any<N, 2, false>(m, r, o);
+  if (mode == 0) return av_surf_any<N, 0, true>(m, r, o);
+  return av_surf_any<N, 1, true>(m, r, o);
 }
 
 // Volume kernel, then the surface kernel: mode 0/1 stage (a0 != 0 / a0 == 0),
