@@ -843,7 +843,7 @@ __global__ void __launch_bounds__(64 * WPB, 2) av_surf_kernel(AvArgs a, StageAv 
 //                 consecutive rows per store;
 //   memory waves: the face records of tile n+3 into LDS, the loads of tile
 //                 n+4's records and of tile n+2 (u, the neighbour traces of
-//                 u, the own and neighbour traces of sig, the neighbour eps,
+//                 u, the own and neighbour traces of sig, the neighbours' eps,
 //                 gathered through the records stored a phase before), and
 //                 tile n+1's u into LDS and its jumps
 //                   -Fscale ((sig- + sig+)/2 + tau (u- - u+))  [+ LF jump (BURG)]
@@ -868,10 +868,12 @@ struct AvHiCfg {
 // Face record of (face f, element j) of a memory wave's elements: value
 // offset of the neighbour's node 0, row of the orientation table, rows of
 // the own and neighbour sig traces ([K][4] face rows of Nfp values), the
-// neighbour element, boundary flag, own eps, Fscale, LF factors.
+// neighbour element, boundary flag, own eps, Fscale, LF factors, and the
+// neighbour's eps (loaded with the tile's traces, written when the jumps are
+// formed).
 struct AvHiRec {
   int32_t nb, fnrow, smrow, sprow, ne, bnd;
-  double eo, fs, fa, fb;
+  double eo, fs, fa, fb, ep;
 };
 
 template <int N, int MODE, bool BURG>
@@ -1064,10 +1066,10 @@ __global__ void __launch_bounds__(AvHiCfg<N>::NT)
     const int col = ok ? f * NFP + qq : NF;
     fw[k] = (uint32_t)iu | ((uint32_t)col << 12) | ((uint32_t)sl << 21) | ((uint32_t)j << 26);
   }
-  // u values (pair p = lane + 64 k: element e0 + p / NP, node p % NP) and,
-  // per trace pair, u+, sig-, sig+ and eps+, by slot (compile-time
-  // constants, so the arrays stay in registers)
-  double U[2][UPL], P[2][GPL], SM[2][GPL], SP[2][GPL], EP[2][GPL];
+  // u values (pair p = lane + 64 k: element e0 + p / NP, node p % NP), per
+  // trace pair u+, sig- and sig+, and eps+ of lane l's record (l < 4 EPW), by
+  // slot (compile-time constants, so the arrays stay in registers)
+  double U[2][UPL], P[2][GPL], SM[2][GPL], SP[2][GPL], EP[2];
   int64_t TV[2] = {0, 0};
   int TC[2] = {1, 1};
   auto issue = [&](int n, auto SL) {
@@ -1092,8 +1094,8 @@ __global__ void __launch_bounds__(AvHiCfg<N>::NT)
       P[slot][k] = a.u[r.nb + sFN[r.fnrow + qq]];
       SM[slot][k] = a.sig[(int64_t)r.smrow * NFP + qq];
       SP[slot][k] = a.sig[(int64_t)r.sprow * NFP + sFP[r.fnrow + qq]];
-      EP[slot][k] = a.eps_in[r.ne];
     }
+    EP[slot] = a.eps_in[R[lane & (4 * EPW - 1)].ne];
     TV[slot] = tl.vbase;
     TC[slot] = tl.count;
   };
@@ -1114,7 +1116,8 @@ __global__ void __launch_bounds__(AvHiCfg<N>::NT)
       sTv[bu] = TV[slot];
       sTc[bu] = TC[slot];
     }
-    wave_fence();  // the wave's u of the tile is in LDS
+    if (lane < 4 * EPW) sRec[q][n & 3][lane].ep = EP[slot];
+    wave_fence();  // the wave's u of the tile and eps+ of its records are in LDS
     double* G = sG[bu];
 #pragma unroll
     for (int k = 0; k < GPL; ++k) {
@@ -1123,7 +1126,7 @@ __global__ void __launch_bounds__(AvHiCfg<N>::NT)
       const AvHiRec& r = R[(fk >> 21) & 31];
       const double m = su[fk & 4095], pv = P[slot][k];
       const double sm = SM[slot][k], sn = r.bnd ? -sm : SP[slot][k];
-      const double tau = tau0 * fmax(r.eo, EP[slot][k]);
+      const double tau = tau0 * fmax(r.eo, r.ep);
       double x = -r.fs * (0.5 * (sm + sn) + tau * (m - pv));
       if (BURG) x = fma(m - pv, fma(r.fa, m + pv, -r.fb * fmax(fabs(m), fabs(pv))), x);
       G[(e0 + j) * GRS + col] = x;
